@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: atom-steps/s (energy + forces + virial) of the level-24 MTP on
+the 32,000-atom 3-species FCC workload, one GPU, FP64.
+
+A step is one full force evaluation of the padded neighbor arrays already
+resident in HBM: radial basis -> moments -> contraction -> energy ->
+adjoint -> per-slot gradients -> Newton-3 force gather -> virial.
+
+Prints ONE JSON line.  ``roofline`` is the atom kernel's achieved
+algorithmic FP64 FLOP rate (fixed per-atom FLOP formula)
+over its average launch time, measured with HIP events on its own stream.
+``cpu_baseline`` times the FP64 CPU restatement of the reference's
+full-tensor algorithm on a bounded sample of the same workload on the host
+cores; the same leg runs the compact CPU oracle over the whole workload
+and reports the GPU force error against it.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "atom-steps/s (E+F) level-24 MTP 32k atoms, 1 GPU; force err vs FP64 ref"
+PEAK_FP64_TFLOPS = 78.6
+
+
+def algorithmic_flops(nbr_counts, N, R, B, T, S):
+    """FP64 FLOPs of one force evaluation in the compact formulation (FMA = 2)."""
+    fwd = 12 + 3 * (N - 2) + (2 * R * N + 2 * R) + 4 * B
+    adj = 14 * B + (4 * R * N + 5 * N) + 15 + 24
+    n = np.asarray(nbr_counts, dtype=np.float64)
+    return float(n.sum() * (fwd + adj) + len(n) * (3 * T + 5 * T + 2 * S))
+
+
+def algorithmic_bytes(nbr_counts):
+    n = np.asarray(nbr_counts, dtype=np.float64)
+    return float((32.0 * n + 40.0).sum())
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, nb, gpu_F, threads):
+    """Reference-algorithm (full-tensor tree) CPU timing on a bounded sample,
+    plus the compact CPU oracle on the whole workload (and the GPU force
+    error against it)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    O.build()
+    import tempfile
+
+    jm = importlib.import_module("jax-mtp_amd")
+    with tempfile.NamedTemporaryFile("w", suffix=".mtp", delete=False) as f:
+        f.write(jm.format_mtp(w.params))
+        path = f.name
+    P = O.read_mtp(path)
+    os.unlink(path)
+    orc = O.Oracle(P, with_tree=True)
+    it, js, rij, jt = nb["itypes"], nb["js"], nb["rij"], nb["jtypes"]
+    # calibrate the tree path on 64 atoms, then size the sample for ~15 s
+    t0 = time.perf_counter()
+    orc.tree_eval(it[:64], rij[:64], jt[:64], True, threads)
+    t_cal = time.perf_counter() - t0
+    n_s = int(min(len(it), max(64, 64 * 5.0 / max(t_cal, 1e-3))))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.tree_eval(it[:n_s], rij[:n_s], jt[:n_s], True, threads)
+        times.append(time.perf_counter() - t0)
+    tree_rate = n_s / float(np.mean(times))
+    # compact oracle, whole workload, one pass (also the force reference)
+    t0 = time.perf_counter()
+    E, F, s6, Ei, g = orc.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, w.natoms, w.natoms,
+                                                           form="compact", nthreads=threads)
+    t_comp = time.perf_counter() - t0
+    ferr = float(np.max(np.abs(gpu_F - F)) / np.max(np.abs(F)))
+    return dict(
+        value=tree_rate, unit="atom-steps/s", cores=threads, kind="port",
+        sample=f"first {n_s} of {w.natoms} atoms of the same workload, full-tensor JSON-tree FP64 restatement of "
+               f"jax_backend.py (E + per-slot gradients), OpenMP, mean of 3 passes after a 64-atom calibration",
+        compact_oracle=dict(value=w.natoms / t_comp, unit="atom-steps/s", cores=threads,
+                            sample=f"all {w.natoms} atoms, one pass, E+F+stress"),
+    ), ferr, F
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ncell", type=int, default=20, help="FCC cells per edge (20 -> 32,000 atoms)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the MTP evaluator's multi-GPU path (spatial domain decomposition "
+              "with a halo exchange) is a design, not part of this build. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    jm = importlib.import_module("jax-mtp_amd")
+    w = jm.configs.fcc32k_level24(seed=args.seed, ncell=args.ncell)
+    t0 = time.perf_counter()
+    nb = w.neighbors()
+    log(f"[bench] workload {w.name}: {w.natoms} atoms, neighbors built in {time.perf_counter() - t0:.2f}s, "
+        f"mean n={nb['counts'].mean():.2f} max n={nb['counts'].max()}")
+    dev = torch.device("cuda")
+    it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
+    js = torch.as_tensor(nb["js"], dtype=torch.int32, device=dev)
+    rij = torch.as_tensor(nb["rij"], dtype=torch.float64, device=dev)
+    jt = torch.as_tensor(nb["jtypes"], dtype=torch.int32, device=dev)
+    A, K = js.shape
+    pot = jm.MTPPotential(w.params, max_atoms=A, max_nbr=K)
+    info = pot.info()
+    L = jm.capi.lib()
+    out = torch.empty(14, dtype=torch.float64, device=dev)
+    F = torch.empty((A, 3), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def step():
+        jm.capi.check(L.mtp_compute_padded_device(pot.ctx, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(),
+                                                  jt.data_ptr(), 3, w.volume, A, A, out.data_ptr(), F.data_ptr(),
+                                                  None, sptr), pot.ctx)
+
+    results = {}
+    for variant in ("fp64", "radial_fp32"):
+        pot.set_precision(variant == "radial_fp32")
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        jm.capi.check(L.mtp_profile(pot.ctx, 1), pot.ctx)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms_total = ev0.elapsed_time(ev1)
+        import ctypes
+
+        kms = ctypes.c_double()
+        nl = ctypes.c_int()
+        jm.capi.check(L.mtp_profile_read(pot.ctx, ctypes.byref(kms), ctypes.byref(nl)), pot.ctx)
+        jm.capi.check(L.mtp_profile(pot.ctx, 0), pot.ctx)
+        status = float(out[13].item())
+        if status != 0.0:
+            raise RuntimeError("non-finite energy or virial")
+        results[variant] = dict(ms_per_step=ms_total / args.steps, wall_ms_per_step=1e3 * wall / args.steps,
+                                atom_kernel_ms=kms.value / max(nl.value, 1), launches=nl.value,
+                                F=F.cpu().numpy().copy(), E=float(out[0].item()))
+        log(f"[bench] {variant}: {ms_total / args.steps:.3f} ms/step (events), atom kernel "
+            f"{kms.value / max(nl.value, 1):.3f} ms, E={results[variant]['E']:.12e}")
+
+    r = results["fp64"]
+    flops = algorithmic_flops(nb["counts"][: w.natoms], info["radial_basis_size"], info["radial_funcs_count"],
+                              info["basic_count"], info["times_count"], info["scalar_count"])
+    achieved = flops / (r["atom_kernel_ms"] * 1e-3) / 1e12
+    value = w.natoms / (r["ms_per_step"] * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded FCC lattice, N(0,0.1) parameters)",
+        "config": {"workload": w.name, "natoms": w.natoms, "level": info["level"], "species": 3,
+                   "radial_basis_size": info["radial_basis_size"], "radial_funcs_count": info["radial_funcs_count"],
+                   "cutoff": w.cutoff, "max_nbr": K, "mean_neighbors": float(nb["counts"].mean()),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "atom_kernel<double>", "kernel_ms": r["atom_kernel_ms"],
+                     "algorithmic_flop_per_launch": flops,
+                     "algorithmic_bytes_per_launch": algorithmic_bytes(nb["counts"][: w.natoms])},
+        "radial_fp32": {"value": w.natoms / (results["radial_fp32"]["ms_per_step"] * 1e-3),
+                        "ms_per_step": results["radial_fp32"]["ms_per_step"],
+                        "atom_kernel_ms": results["radial_fp32"]["atom_kernel_ms"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, len(os.sched_getaffinity(0)))
+        log(f"[bench] cpu baseline on {threads} threads")
+        cb, ferr, Fref = cpu_baseline(w, nb, r["F"], threads)
+        line["cpu_baseline"] = cb
+        f32 = results["radial_fp32"]["F"]
+        line["force_err_vs_fp64_oracle"] = {
+            "fp64": ferr,
+            "radial_fp32": float(np.max(np.abs(f32 - Fref)) / np.max(np.abs(Fref))),
+            "norm": "max|F-F_ref| / max|F_ref| over all atoms and components",
+        }
+    print(json.dumps(line), flush=True)
+    pot.close()
+
+
+if __name__ == "__main__":
+    main()

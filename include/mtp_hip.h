@@ -1,0 +1,140 @@
+/*
+ * mtp_hip.h — C ABI of the MI355X (gfx950) Moment Tensor Potential evaluator.
+ *
+ * This library replaces the reference's runtime bridge, which called an
+ * exported JAX function through embedded CPython:
+ *   ZeroOverhead::ZeroOverheadManager / ZeroOverheadContext
+ *     (pair_style_files/zero_overhead.hpp:53-88, zero_overhead.cpp:149-411)
+ * and the XLA program it ran:
+ *   calc_energy_forces_stress_padded (jax_backend.py:374-460).
+ *
+ * Plain pointers and sizes only.  All floating point is FP64.  Every entry
+ * point returns 0 on success or a negative MTP_ERR_* code; the message is
+ * available from mtp_last_error(ctx) (or mtp_global_error() when no context
+ * exists yet).  A context is not re-entrant: use one per host thread.
+ */
+#ifndef MTP_HIP_H
+#define MTP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTP_OK 0
+#define MTP_ERR_ARG -1      /* invalid argument or null pointer          */
+#define MTP_ERR_FILE -2     /* potential file missing or malformed       */
+#define MTP_ERR_CAPACITY -3 /* natoms > max_atoms or count > max_nbr     */
+#define MTP_ERR_HIP -4      /* HIP runtime error                         */
+#define MTP_ERR_NONFINITE -5 /* NaN/Inf in energy or virial             */
+#define MTP_ERR_UNSUPPORTED -6 /* basis layout or size not supported     */
+
+typedef struct mtp_ctx mtp_ctx;
+
+/* Open a trained .mtp potential and size device buffers for at most
+ * max_atoms atoms of max_nbr neighbor slots each.
+ * Replaces PairJaxMTP::settings/init_style + ZeroOverheadContext
+ * (pair_jax_mtp.cpp:228-245, 264-282; zero_overhead.cpp:149-169, 408-411).  The first pair_style argument is
+ * the .mtp file (the reference took a jax.export .bin).  cutoff is the
+ * MTP_CUTOFF argument (used by the packing helper); debug_level is kept. */
+int mtp_open(const char *mtp_path, int max_atoms, int max_nbr, double cutoff, int debug_level, mtp_ctx **out);
+
+/* Same, from the text of a .mtp file held in memory. */
+int mtp_open_text(const char *mtp_text, int max_atoms, int max_nbr, double cutoff, int debug_level, mtp_ctx **out);
+
+/* Release the context and its device memory. */
+void mtp_close(mtp_ctx *ctx);
+
+/* Last error message of ctx ("" if none); mtp_global_error() for errors
+ * raised before a context existed (e.g. by mtp_open). */
+const char *mtp_last_error(const mtp_ctx *ctx);
+const char *mtp_global_error(void);
+
+/* Radial-basis precision: 0 = FP64 (default), 1 = FP32 radial basis with
+ * FP64 moments, contraction, energy and virial. */
+int mtp_set_precision(mtp_ctx *ctx, int radial_fp32);
+
+/* Basis description: level (conversion.py:14-29), species count, R, N,
+ * alpha_moments_count, B, T, S.  Any pointer may be NULL. */
+int mtp_info(const mtp_ctx *ctx, int *level, int *species_count, int *radial_funcs_count, int *radial_basis_size,
+             int *alpha_moments_count, int *basic_count, int *times_count, int *scalar_count);
+
+/* Replace the trainable parameters in place (moment_coeffs[S] in .mtp
+ * order, species_coeffs[Sp], radial_coeffs[Sp][Sp][R][N]); NULL keeps a
+ * group.  Mirrors MTPData.parameters' setter (data.py:74-102). */
+int mtp_set_params(mtp_ctx *ctx, const double *moment_coeffs, const double *species_coeffs,
+                   const double *radial_coeffs, double scaling);
+
+/* Host-array evaluation with the data contract of
+ * ZeroOverheadManager::execute_potential (zero_overhead.hpp:58-64), as the
+ * pair style calls it (pair_jax_mtp.cpp:183-189):
+ * rij_rows[i] holds 3*max_nbr doubles (slot k at 3k..3k+2), nbr_rows[i] and
+ * nbr_type_rows[i] max_nbr ints, nbr_counts[i] the valid prefix; rows exist
+ * for i < natoms.  nneigh_total is accepted and unused (as in the reference).
+ * Outputs: energy (sum of E_i, i < natoms), forces[natoms*3], stress6 in the
+ * reference convention (jax_backend.py:357-360: (W+W^T)/(2V), order xx yy zz
+ * yz xz xy), virial6 in the LAMMPS convention (-sym(W), order xx yy zz xy xz
+ * yz, energy units).  stress6/virial6 may be NULL.  Synchronous. */
+int mtp_compute(mtp_ctx *ctx, int natoms, int nneigh_total, const double *const *rij_rows, const int *types,
+                const int *const *nbr_rows, const int *nbr_counts, const int *const *nbr_type_rows, double volume,
+                double *energy, double *forces, double *stress6, double *virial6);
+
+/* Device-resident evaluation with the 8 dynamic arguments of
+ * calc_energy_forces_stress_padded (jax_backend.py:374-392; dtypes as
+ * compile_cuda.py:225-232 but FP64): d_itypes[nrows], d_js[nrows*nslots],
+ * d_rijs[nrows*nslots*3], d_jtypes[nrows*nslots], cell_rank, volume,
+ * natoms_energy, natoms_force.  nrows <= max_atoms, nslots <= max_nbr.
+ * Device outputs: d_out[14] = {E, stress6 (reference), virial6 (LAMMPS),
+ * status}, d_forces[nrows*3]; d_eatom[nrows] (per-atom energies) may be
+ * NULL.  stream is a hipStream_t (NULL = the context's stream).  Returns
+ * after enqueueing; call mtp_sync(ctx, stream) before reading results. */
+int mtp_compute_padded_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js,
+                              const double *d_rijs, const int *d_jtypes, int cell_rank, double volume,
+                              int natoms_energy, int natoms_force, double *d_out, double *d_forces,
+                              double *d_eatom, void *stream);
+
+/* Device pointer of the per-slot gradients dE_i/dr_ik [nrows][nslots][3] of
+ * the last device evaluation (valid until the next call on ctx). */
+const double *mtp_last_slot_gradients(const mtp_ctx *ctx);
+
+/* Wait for work enqueued on stream (NULL = the context's stream). */
+int mtp_sync(mtp_ctx *ctx, void *stream);
+
+/* Live per-kernel timing (no reference counterpart; the reference only
+ * timed whole calls, compile_cuda.py:252-275).  While enabled, HIP events
+ * are recorded on the launch stream around every atom-kernel launch;
+ * mtp_profile_read returns the summed atom-kernel time (ms) and the launch
+ * count since the last read, then resets. */
+int mtp_profile(mtp_ctx *ctx, int enable);
+int mtp_profile_read(mtp_ctx *ctx, double *atom_kernel_ms, int *launches);
+
+/* ---- host-only helpers (no GPU work; usable without a device) -------- */
+
+/* Parse, validate and plan a .mtp file without touching the GPU.  Returns
+ * the same fields as mtp_info plus the LDS bytes the atom kernel needs for
+ * max_nbr slots. */
+int mtp_check_file(const char *mtp_path, int max_nbr, int *level, int *species_count, int *alpha_moments_count,
+                   int *basic_count, int *times_count, int *scalar_count, long long *lds_bytes);
+
+/* Neighbor packing of PairJaxMTP::compute (pair_jax_mtp.cpp:96-173) for a
+ * full neighbor list and an orthogonal box:
+ * pos[max_atoms][3*max_nbr], nbr/ntype[max_atoms][max_nbr], counts[max_atoms],
+ * atom_types[max_atoms].  Returns nneigh_total (>= 0) or MTP_ERR_CAPACITY
+ * when nlocal > max_atoms.  Neighbor overflow truncates like the reference;
+ * *n_truncated (may be NULL) receives the number of dropped in-cutoff
+ * neighbors. */
+long long mtp_pack_full_list(int nlocal, const double *x, const int *type, const int *tag, int inum,
+                             const int *ilist, const int *numneigh, const int *const *firstneigh, double cutoff,
+                             const double *prd, const int *periodic, int max_atoms, int max_nbr, double *pos,
+                             int *nbr, int *ntype, int *counts, int *atom_types, long long *n_truncated);
+
+/* Staging of zero_overhead.cpp:61-117 (update_atom_data_zero_copy) in FP64:
+ * padded [rows][max_nbr] arrays with sentinels j = natoms, type = -1,
+ * r = 2000, itype = -1, then the valid prefix of each row i < natoms. */
+int mtp_stage_host(int rows, int max_nbr, int natoms, const double *const *rij_rows, const int *types,
+                   const int *const *nbr_rows, const int *nbr_counts, const int *const *nbr_type_rows,
+                   int *itypes_out, int *js_out, double *rij_out, int *jtypes_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MTP_HIP_H */

@@ -1,0 +1,507 @@
+// C ABI of the MI355X MTP evaluator (include/mtp_hip.h).
+//
+// One context = one potential, one HIP stream and device buffers sized at
+// open time for max_atoms x max_nbr (the reference sized its NumPy staging
+// pool the same way, zero_overhead.cpp:123-150).  No Python anywhere.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mtp_hip.h"
+#include "mtp_format.hpp"
+#include "mtp_kernels.hpp"
+#include "mtp_plan.hpp"
+#include "nbr_pack.hpp"
+
+using namespace mtp;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& s)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = s;
+}
+
+template <typename T>
+T* dalloc(size_t n, hipError_t& e)
+{
+    void* p = nullptr;
+    if (e == hipSuccess && n) e = hipMalloc(&p, n * sizeof(T));
+    return (T*)p;
+}
+
+}  // namespace
+
+struct mtp_ctx {
+    HostPlan plan;
+    DevPlan dev{};
+    int max_atoms = 0, max_nbr = 0, debug_level = 0;
+    double cutoff = 0.0;
+    int radial_fp32 = 0;
+    size_t lds_bytes = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    std::vector<void*> dptrs;
+    // parameter and plan buffers
+    double *d_radial = nullptr, *d_species = nullptr, *d_moment = nullptr, *d_xi = nullptr;
+    // per-call buffers
+    double *d_rij = nullptr, *d_Ei = nullptr, *d_g = nullptr, *d_Wi = nullptr, *d_F = nullptr, *d_out = nullptr;
+    int *d_it = nullptr, *d_js = nullptr, *d_jt = nullptr, *d_cnt = nullptr, *d_off = nullptr, *d_rev = nullptr;
+    // pinned staging
+    double* h_rij = nullptr;
+    int *h_it = nullptr, *h_js = nullptr, *h_jt = nullptr;
+    double* h_F = nullptr;
+    double* h_out = nullptr;
+    // live per-kernel timing (events recorded around each atom-kernel launch)
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+
+    int fail(int code, const std::string& msg)
+    {
+        err = msg;
+        if (debug_level > 0) std::fprintf(stderr, "[mtp_hip] %s\n", msg.c_str());
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* where)
+    {
+        return fail(MTP_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+    }
+    template <typename T>
+    T* alloc(size_t n, hipError_t& e)
+    {
+        T* p = dalloc<T>(n, e);
+        if (p) dptrs.push_back(p);
+        return p;
+    }
+    ~mtp_ctx()
+    {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : dptrs) hipFree(p);
+        if (h_rij) hipHostFree(h_rij);
+        if (h_it) hipHostFree(h_it);
+        if (h_js) hipHostFree(h_js);
+        if (h_jt) hipHostFree(h_jt);
+        if (h_F) hipHostFree(h_F);
+        if (h_out) hipHostFree(h_out);
+        for (auto& pr : ev_pool) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t upload(T* d, const std::vector<T>& h, hipStream_t s)
+{
+    if (h.empty()) return hipSuccess;
+    return hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int upload_params(mtp_ctx* c)
+{
+    const MtpFile& f = c->plan.file;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(c->d_radial, f.radial_coeffs, c->stream);
+    if (e == hipSuccess) e = upload(c->d_species, f.species_coeffs, c->stream);
+    if (e == hipSuccess) e = upload(c->d_moment, f.moment_coeffs, c->stream);
+    if (e == hipSuccess) e = upload(c->d_xi, c->plan.xi_of, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "parameter upload");
+    c->dev.scaling = f.scaling;
+    return MTP_OK;
+}
+
+int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int debug_level, mtp_ctx** out)
+{
+    if (!out) { set_global_error("mtp_open: out is NULL"); return MTP_ERR_ARG; }
+    *out = nullptr;
+    if (max_atoms <= 0 || max_nbr <= 1 || cutoff <= 0.0) {
+        // pair_jax_mtp.cpp:241-243 and 268-269
+        set_global_error("max_atoms, max_neighbors, and cutoff must be positive (max_neighbors > 1)");
+        return MTP_ERR_ARG;
+    }
+    if (max_nbr > kAtomBlock) {
+        set_global_error("max_neighbors above " + std::to_string(kAtomBlock) + " is not supported");
+        return MTP_ERR_UNSUPPORTED;
+    }
+    auto* c = new mtp_ctx();
+    c->max_atoms = max_atoms; c->max_nbr = max_nbr; c->cutoff = cutoff; c->debug_level = debug_level;
+    std::string err;
+    if (!build_plan(file, 4, c->plan, err)) {
+        set_global_error("potential: " + err);
+        delete c;
+        return MTP_ERR_FILE;
+    }
+    const HostPlan& p = c->plan;
+    const MtpFile& f = p.file;
+    c->lds_bytes = atom_lds_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max, p.n_parts);
+    hipError_t e = hipGetDevice(&c->device);
+    int lds_max = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    if (e != hipSuccess) {
+        set_global_error(std::string("no usable HIP device: ") + hipGetErrorString(e));
+        delete c;
+        return MTP_ERR_HIP;
+    }
+    if ((long long)c->lds_bytes > (long long)lds_max) {
+        set_global_error("potential needs " + std::to_string(c->lds_bytes) + " bytes of LDS per atom, device has " +
+                         std::to_string(lds_max));
+        delete c;
+        return MTP_ERR_UNSUPPORTED;
+    }
+    e = set_atom_kernel_lds(c->lds_bytes);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);  // default priority
+    // plan buffers
+    auto* mono = c->alloc<I4>(p.mono.size(), e);
+    auto* fwd_tgt = c->alloc<int>(p.fwd_tgt.size(), e);
+    auto* fwd_beg = c->alloc<int>(p.fwd_beg.size(), e);
+    auto* fwd_pair = c->alloc<uint32_t>(p.fwd_pair.size(), e);
+    auto* fwd_mult = c->alloc<double>(p.fwd_mult.size(), e);
+    auto* rev_x = c->alloc<int>(p.rev_x.size(), e);
+    auto* rev_beg = c->alloc<int>(p.rev_beg.size(), e);
+    auto* rev_pair = c->alloc<uint32_t>(p.rev_pair.size(), e);
+    auto* rev_mult = c->alloc<double>(p.rev_mult.size(), e);
+    auto* part_blk = c->alloc<I2>(p.part_blk.size(), e);
+    auto* mapping = c->alloc<int>(f.alpha_moment_mapping.size(), e);
+    c->d_radial = c->alloc<double>(f.radial_coeffs.size(), e);
+    c->d_species = c->alloc<double>(f.species_coeffs.size(), e);
+    c->d_moment = c->alloc<double>(f.moment_coeffs.size() ? f.moment_coeffs.size() : 1, e);
+    c->d_xi = c->alloc<double>(p.xi_of.size(), e);
+    // per-call buffers
+    const size_t A = (size_t)max_atoms, K = (size_t)max_nbr;
+    c->d_rij = c->alloc<double>(A * K * 3, e);
+    c->d_it = c->alloc<int>(A, e);
+    c->d_js = c->alloc<int>(A * K, e);
+    c->d_jt = c->alloc<int>(A * K, e);
+    c->d_Ei = c->alloc<double>(A, e);
+    c->d_g = c->alloc<double>(A * K * 3, e);
+    c->d_Wi = c->alloc<double>(A * 9, e);
+    c->d_F = c->alloc<double>(A * 3, e);
+    c->d_out = c->alloc<double>(16, e);
+    c->d_cnt = c->alloc<int>(2 * A, e);
+    c->d_off = c->alloc<int>(A + 1, e);
+    c->d_rev = c->alloc<int>(A * K, e);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rij, A * K * 3 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_it, A * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_js, A * K * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_jt, A * K * sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_F, A * 3 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_out, 16 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = upload(mono, p.mono, c->stream);
+    if (e == hipSuccess) e = upload(fwd_tgt, p.fwd_tgt, c->stream);
+    if (e == hipSuccess) e = upload(fwd_beg, p.fwd_beg, c->stream);
+    if (e == hipSuccess) e = upload(fwd_pair, p.fwd_pair, c->stream);
+    if (e == hipSuccess) e = upload(fwd_mult, p.fwd_mult, c->stream);
+    if (e == hipSuccess) e = upload(rev_x, p.rev_x, c->stream);
+    if (e == hipSuccess) e = upload(rev_beg, p.rev_beg, c->stream);
+    if (e == hipSuccess) e = upload(rev_pair, p.rev_pair, c->stream);
+    if (e == hipSuccess) e = upload(rev_mult, p.rev_mult, c->stream);
+    if (e == hipSuccess) e = upload(part_blk, p.part_blk, c->stream);
+    if (e == hipSuccess) e = upload(mapping, f.alpha_moment_mapping, c->stream);
+    if (e != hipSuccess) {
+        set_global_error(std::string("device allocation/upload failed: ") + hipGetErrorString(e));
+        delete c;
+        return MTP_ERR_HIP;
+    }
+    DevPlan& d = c->dev;
+    d.Sp = f.species_count; d.R = f.radial_funcs_count; d.N = f.radial_basis_size;
+    d.B = f.B(); d.T = f.T(); d.amc = f.alpha_moments_count; d.S = f.S(); d.nu_max = p.nu_max;
+    d.scaling = f.scaling; d.min_dist = f.min_dist; d.max_dist = f.max_dist;
+    d.radial_coeffs = c->d_radial; d.species_coeffs = c->d_species; d.moment_coeffs = c->d_moment;
+    d.mapping = mapping; d.xi_of = c->d_xi;
+    std::memcpy(d.blk_off, p.blk_off, sizeof(d.blk_off));
+    std::memcpy(d.mu_max_of_nu, p.mu_max_of_nu, sizeof(d.mu_max_of_nu));
+    d.n_mono = (int)p.mono.size(); d.mono = mono;
+    d.n_depth = p.n_depth;
+    for (int q = 0; q < kMaxDepth + 2; ++q) { d.fwd_layer[q] = p.fwd_layer[q]; d.rev_layer[q] = p.rev_layer[q]; }
+    d.fwd_tgt = fwd_tgt; d.fwd_beg = fwd_beg; d.fwd_pair = fwd_pair; d.fwd_mult = fwd_mult;
+    d.rev_x = rev_x; d.rev_beg = rev_beg; d.rev_pair = rev_pair; d.rev_mult = rev_mult;
+    d.n_parts = p.n_parts;
+    for (int q = 0; q < 9; ++q) d.part_beg[q] = p.part_beg[q];
+    d.part_blk = part_blk;
+    int rc = upload_params(c);
+    if (rc) {
+        set_global_error(c->err);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MTP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mtp_global_error(void)
+{
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+    return copy.c_str();
+}
+
+const char* mtp_last_error(const mtp_ctx* ctx) { return ctx ? ctx->err.c_str() : mtp_global_error(); }
+
+int mtp_open(const char* mtp_path, int max_atoms, int max_nbr, double cutoff, int debug_level, mtp_ctx** out)
+{
+    if (!mtp_path) { set_global_error("mtp_open: path is NULL"); return MTP_ERR_ARG; }
+    MtpFile f;
+    std::string err;
+    if (!read_mtp_file(mtp_path, f, err)) { set_global_error(err); return MTP_ERR_FILE; }
+    return open_common(std::move(f), max_atoms, max_nbr, cutoff, debug_level, out);
+}
+
+int mtp_open_text(const char* text, int max_atoms, int max_nbr, double cutoff, int debug_level, mtp_ctx** out)
+{
+    if (!text) { set_global_error("mtp_open_text: text is NULL"); return MTP_ERR_ARG; }
+    MtpFile f;
+    std::string err;
+    if (!parse_mtp_text(text, f, err)) { set_global_error("potential text: " + err); return MTP_ERR_FILE; }
+    return open_common(std::move(f), max_atoms, max_nbr, cutoff, debug_level, out);
+}
+
+void mtp_close(mtp_ctx* ctx) { delete ctx; }
+
+int mtp_set_precision(mtp_ctx* ctx, int radial_fp32)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    ctx->radial_fp32 = radial_fp32 ? 1 : 0;
+    return MTP_OK;
+}
+
+int mtp_info(const mtp_ctx* ctx, int* level, int* species_count, int* radial_funcs_count, int* radial_basis_size,
+             int* alpha_moments_count, int* basic_count, int* times_count, int* scalar_count)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    const MtpFile& f = ctx->plan.file;
+    if (level) *level = ctx->plan.level;
+    if (species_count) *species_count = f.species_count;
+    if (radial_funcs_count) *radial_funcs_count = f.radial_funcs_count;
+    if (radial_basis_size) *radial_basis_size = f.radial_basis_size;
+    if (alpha_moments_count) *alpha_moments_count = f.alpha_moments_count;
+    if (basic_count) *basic_count = f.B();
+    if (times_count) *times_count = f.T();
+    if (scalar_count) *scalar_count = f.S();
+    return MTP_OK;
+}
+
+int mtp_set_params(mtp_ctx* ctx, const double* moment_coeffs, const double* species_coeffs,
+                   const double* radial_coeffs, double scaling)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    MtpFile& f = ctx->plan.file;
+    if (moment_coeffs) std::memcpy(f.moment_coeffs.data(), moment_coeffs, f.moment_coeffs.size() * sizeof(double));
+    if (species_coeffs) std::memcpy(f.species_coeffs.data(), species_coeffs, f.species_coeffs.size() * sizeof(double));
+    if (radial_coeffs) std::memcpy(f.radial_coeffs.data(), radial_coeffs, f.radial_coeffs.size() * sizeof(double));
+    if (std::isfinite(scaling) && scaling > 0) f.scaling = scaling;
+    refresh_plan_params(ctx->plan);
+    return upload_params(ctx);
+}
+
+int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const int* d_js,
+                              const double* d_rijs, const int* d_jtypes, int cell_rank, double volume,
+                              int natoms_energy, int natoms_force, double* d_out, double* d_forces, double* d_eatom,
+                              void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (nrows < 0 || nrows > ctx->max_atoms)
+        return ctx->fail(MTP_ERR_CAPACITY, "rows " + std::to_string(nrows) + " exceed max_atoms " +
+                                               std::to_string(ctx->max_atoms));
+    if (nslots <= 0 || nslots > ctx->max_nbr)
+        return ctx->fail(MTP_ERR_CAPACITY, "slots " + std::to_string(nslots) + " exceed max_neighbors " +
+                                               std::to_string(ctx->max_nbr));
+    if (nrows > 0 && (!d_itypes || !d_js || !d_rijs || !d_jtypes || !d_forces))
+        return ctx->fail(MTP_ERR_ARG, "null device array");
+    if (!d_out) return ctx->fail(MTP_ERR_ARG, "null output");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    AtomArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.rij = d_rijs; A.itypes = d_itypes; A.jtypes = d_jtypes; A.js = d_js;
+    A.Ei = d_eatom ? d_eatom : ctx->d_Ei;
+    A.g = ctx->d_g; A.Wi = ctx->d_Wi;
+    ForceArgs F{};
+    F.n_e = natoms_energy; F.n_f = natoms_force; F.cell_rank = cell_rank; F.volume = volume;
+    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.out = d_out;
+    hipEvent_t eb = nullptr, ee = nullptr;
+    if (ctx->profiling) {
+        if (ctx->ev_used == ctx->ev_pool.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+                return ctx->fail(MTP_ERR_HIP, "event creation failed");
+            ctx->ev_pool.push_back({a, b});
+        }
+        eb = ctx->ev_pool[ctx->ev_used].first;
+        ee = ctx->ev_pool[ctx->ev_used].second;
+        ++ctx->ev_used;
+    }
+    hipError_t e = launch_forces(ctx->dev, A, F, ctx->lds_bytes, ctx->radial_fp32 != 0, s, eb, ee);
+    if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
+    return MTP_OK;
+}
+
+int mtp_profile(mtp_ctx* ctx, int enable)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return ctx->hip_fail(e, "profile");
+    ctx->profiling = enable != 0;
+    ctx->ev_used = 0;
+    return MTP_OK;
+}
+
+int mtp_profile_read(mtp_ctx* ctx, double* atom_kernel_ms, int* launches)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    double tot = 0.0;
+    for (size_t q = 0; q < ctx->ev_used; ++q) {
+        hipError_t e = hipEventSynchronize(ctx->ev_pool[q].second);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_pool[q].first, ctx->ev_pool[q].second);
+        if (e != hipSuccess) return ctx->hip_fail(e, "profile read");
+        tot += ms;
+    }
+    if (atom_kernel_ms) *atom_kernel_ms = tot;
+    if (launches) *launches = (int)ctx->ev_used;
+    ctx->ev_used = 0;
+    return MTP_OK;
+}
+
+const double* mtp_last_slot_gradients(const mtp_ctx* ctx) { return ctx ? ctx->d_g : nullptr; }
+
+int mtp_sync(mtp_ctx* ctx, void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    hipError_t e = hipStreamSynchronize(stream ? (hipStream_t)stream : ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "stream synchronize");
+    return MTP_OK;
+}
+
+int mtp_stage_host(int rows, int max_nbr, int natoms, const double* const* rij_rows, const int* types,
+                   const int* const* nbr_rows, const int* nbr_counts, const int* const* nbr_type_rows,
+                   int* itypes_out, int* js_out, double* rij_out, int* jtypes_out)
+{
+    if (rows < 0 || max_nbr <= 0 || natoms < 0 || natoms > rows) return MTP_ERR_ARG;
+    if (natoms > 0 && (!rij_rows || !types || !nbr_rows || !nbr_counts || !nbr_type_rows)) return MTP_ERR_ARG;
+    if (!itypes_out || !js_out || !rij_out || !jtypes_out) return MTP_ERR_ARG;
+    for (int i = 0; i < natoms; ++i)
+        if (nbr_counts[i] < 0 || nbr_counts[i] > max_nbr) return MTP_ERR_CAPACITY;
+    const size_t tot = (size_t)rows * max_nbr;
+    for (size_t q = 0; q < tot; ++q) { js_out[q] = natoms; jtypes_out[q] = -1; }
+    for (size_t q = 0; q < tot * 3; ++q) rij_out[q] = 2000.0;
+    for (int i = 0; i < rows; ++i) itypes_out[i] = -1;
+    if (natoms > 0) std::memcpy(itypes_out, types, (size_t)natoms * sizeof(int));
+    for (int i = 0; i < natoms; ++i) {
+        const int n = nbr_counts[i];
+        const size_t base = (size_t)i * max_nbr;
+        if (n > 0) {
+            std::memcpy(js_out + base, nbr_rows[i], (size_t)n * sizeof(int));
+            std::memcpy(jtypes_out + base, nbr_type_rows[i], (size_t)n * sizeof(int));
+            std::memcpy(rij_out + base * 3, rij_rows[i], (size_t)n * 3 * sizeof(double));
+        }
+    }
+    return MTP_OK;
+}
+
+int mtp_compute(mtp_ctx* ctx, int natoms, int nneigh_total, const double* const* rij_rows, const int* types,
+                const int* const* nbr_rows, const int* nbr_counts, const int* const* nbr_type_rows, double volume,
+                double* energy, double* forces, double* stress6, double* virial6)
+{
+    (void)nneigh_total;
+    if (!ctx) return MTP_ERR_ARG;
+    if (natoms < 0) return ctx->fail(MTP_ERR_ARG, "natoms is negative");
+    if (natoms > ctx->max_atoms)
+        return ctx->fail(MTP_ERR_CAPACITY, "natoms " + std::to_string(natoms) + " exceeds max_atoms " +
+                                               std::to_string(ctx->max_atoms));
+    if (!energy || (natoms > 0 && !forces)) return ctx->fail(MTP_ERR_ARG, "null output pointer");
+    const int K = ctx->max_nbr;
+    int rc = mtp_stage_host(natoms, K, natoms, rij_rows, types, nbr_rows, nbr_counts, nbr_type_rows, ctx->h_it,
+                            ctx->h_js, ctx->h_rij, ctx->h_jt);
+    if (rc == MTP_ERR_CAPACITY) return ctx->fail(rc, "a neighbor count exceeds max_neighbors");
+    if (rc) return ctx->fail(rc, "null input array");
+    hipError_t e = hipSuccess;
+    const size_t A = (size_t)natoms;
+    if (natoms > 0) {
+        e = hipMemcpyAsync(ctx->d_it, ctx->h_it, A * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_js, ctx->h_js, A * K * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_jt, ctx->h_jt, A * K * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_rij, ctx->h_rij, A * K * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "upload");
+    }
+    rc = mtp_compute_padded_device(ctx, natoms, K, ctx->d_it, ctx->d_js, ctx->d_rij, ctx->d_jt, 3, volume, natoms,
+                                   natoms, ctx->d_out, ctx->d_F, nullptr, ctx->stream);
+    if (rc) return rc;
+    if (natoms > 0) e = hipMemcpyAsync(ctx->h_F, ctx->d_F, A * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_out, ctx->d_out, 14 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "evaluation");
+    *energy = ctx->h_out[0];
+    if (natoms > 0) std::memcpy(forces, ctx->h_F, A * 3 * sizeof(double));
+    if (stress6) std::memcpy(stress6, ctx->h_out + 1, 6 * sizeof(double));
+    if (virial6) std::memcpy(virial6, ctx->h_out + 7, 6 * sizeof(double));
+    if (ctx->h_out[13] != 0.0) return ctx->fail(MTP_ERR_NONFINITE, "non-finite energy or virial");
+    return MTP_OK;
+}
+
+int mtp_check_file(const char* mtp_path, int max_nbr, int* level, int* species_count, int* alpha_moments_count,
+                   int* basic_count, int* times_count, int* scalar_count, long long* lds_bytes)
+{
+    if (!mtp_path) { set_global_error("mtp_check_file: path is NULL"); return MTP_ERR_ARG; }
+    MtpFile f;
+    std::string err;
+    if (!read_mtp_file(mtp_path, f, err)) { set_global_error(err); return MTP_ERR_FILE; }
+    HostPlan p;
+    if (!build_plan(f, 4, p, err)) { set_global_error("potential: " + err); return MTP_ERR_FILE; }
+    if (level) *level = p.level;
+    if (species_count) *species_count = f.species_count;
+    if (alpha_moments_count) *alpha_moments_count = f.alpha_moments_count;
+    if (basic_count) *basic_count = f.B();
+    if (times_count) *times_count = f.T();
+    if (scalar_count) *scalar_count = f.S();
+    if (lds_bytes)
+        *lds_bytes = (long long)atom_lds_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
+                                               p.n_parts);
+    return MTP_OK;
+}
+
+long long mtp_pack_full_list(int nlocal, const double* x, const int* type, const int* tag, int inum,
+                             const int* ilist, const int* numneigh, const int* const* firstneigh, double cutoff,
+                             const double* prd, const int* periodic, int max_atoms, int max_nbr, double* pos,
+                             int* nbr, int* ntype, int* counts, int* atom_types, long long* n_truncated)
+{
+    if (nlocal < 0 || max_nbr <= 0 || max_atoms <= 0) return MTP_ERR_ARG;
+    if (nlocal > max_atoms) return MTP_ERR_CAPACITY;
+    if (!x || !type || !tag || !ilist || !numneigh || !firstneigh || !prd || !periodic || !pos || !nbr || !ntype ||
+        !counts || !atom_types)
+        return MTP_ERR_ARG;
+    for (int ii = 0; ii < inum; ++ii)
+        if (ilist[ii] < 0 || ilist[ii] >= nlocal) return MTP_ERR_ARG;
+    // x is [nall][3] contiguous; build row pointers like LAMMPS double**
+    int nall = nlocal;
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = ilist[ii];
+        for (int jj = 0; jj < numneigh[i]; ++jj) {
+            const int j = firstneigh[i][jj] & 0x1FFFFFFF;
+            if (j + 1 > nall) nall = j + 1;
+        }
+    }
+    std::vector<const double*> xr((size_t)nall);
+    for (int q = 0; q < nall; ++q) xr[q] = x + 3 * (size_t)q;
+    OrthoMinImage mi{{prd[0], prd[1], prd[2]}, {periodic[0], periodic[1], periodic[2]}};
+    std::vector<int> t2d;
+    PackOut o{pos, nbr, ntype, counts, atom_types};
+    return pack_full_list(nlocal, xr.data(), type, tag, inum, ilist, numneigh, firstneigh, cutoff, max_atoms,
+                          max_nbr, mi, o, t2d, n_truncated);
+}
+
+}  // extern "C"

@@ -1,0 +1,526 @@
+// HIP kernels for one MTP force evaluation on gfx950 (CDNA4).
+//
+// Replaces the XLA program exported from jax_backend.py:374-460
+// (calc_energy_forces_stress_padded): per-atom energy and its gradient with
+// respect to every neighbor vector (jax_backend.py:20-106, value_and_grad),
+// the Newton-3 force scatter (:263-303), the stress (:355-370) and the masked
+// energy sum (:431-456).  Everything computes in FP64 except, optionally,
+// the radial basis (Chebyshev terms and f_mu), which the radial-FP32 variant
+// evaluates in FP32.
+//
+// Kernel map:
+//   atom_kernel     one 256-thread workgroup per atom row.  LDS holds the
+//                   atom's moments m[amc] and adjoints dm[amc] in FP64.
+//                     A  radial basis per slot, compaction of valid slots
+//                     B  basic moments  m[mu,alpha] = sum_k f_mu(r_k) u_k^alpha
+//                     C  contraction layers (alpha_index_times, by depth)
+//                     D  E_i = species_coeff + xi . m[mapping]
+//                     E  reverse layers: dm = dE/dm (gather form)
+//                     F  g_k = dE_i/dr_k from dm of the basic moments
+//                     G  per-atom virial W_i = sum_k r_k (x) g_k
+//   zero/count/scan/fill/force kernels: deterministic Newton-3 force gather
+//   totals_kernel   E = sum_{i<n_e} E_i, W = sum_i W_i -> stress, virial
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mtp_kernels.hpp"
+
+namespace mtp {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int wrap_species(int t, int Sp)
+{
+    if (t < 0) t += Sp;
+    if (t < 0) t = 0;
+    if (t >= Sp) t = Sp - 1;
+    return t;
+}
+
+// Radial basis of one slot (jax_backend.py:83-95, 108-160).  Writes f[mu]
+// and df[mu] = d f_mu / d r for mu < R.  Real is double or float.
+template <typename Real>
+__device__ __forceinline__ void radial_slot(const DevPlan& P, double r, int ti, int tk, double* f, double* df)
+{
+    const Real rmin = (Real)P.min_dist, rmax = (Real)P.max_dist;
+    const Real rr = (Real)r;
+    const Real range_inv = (Real)1 / (rmax - rmin);
+    const Real x = ((Real)2 * rr - (rmin + rmax)) * range_inv;
+    const Real dxdr = (Real)2 * range_inv;
+    const Real s = (rmax - rr) * (rmax - rr);
+    const Real ds = (Real)-2 * (rmax - rr);
+    const double* c = P.radial_coeffs + (size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R * P.N;
+    Real af[kMaxR], ad[kMaxR];
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m) {
+        af[m] = m < P.R ? (Real)c[m * P.N] : (Real)0;
+        ad[m] = 0;
+    }
+    if (P.N > 1) {
+#pragma unroll
+        for (int m = 0; m < kMaxR; ++m)
+            if (m < P.R) { af[m] += (Real)c[m * P.N + 1] * x; ad[m] += (Real)c[m * P.N + 1]; }
+        Real t2 = 1, t1 = x, d2 = 0, d1 = 1;
+        for (int n = 2; n < P.N; ++n) {
+            const Real t0 = (Real)2 * x * t1 - t2;
+            const Real d0 = (Real)2 * t1 + (Real)2 * x * d1 - d2;
+#pragma unroll
+            for (int m = 0; m < kMaxR; ++m)
+                if (m < P.R) { af[m] += (Real)c[m * P.N + n] * t0; ad[m] += (Real)c[m * P.N + n] * d0; }
+            t2 = t1; t1 = t0; d2 = d1; d1 = d0;
+        }
+    }
+    const Real sc = (Real)P.scaling;
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m) {
+        if (m < P.R) {
+            f[m] = (double)(sc * s * af[m]);
+            df[m] = (double)(sc * (ds * af[m] + s * ad[m] * dxdr));
+        }
+    }
+}
+
+// Canonical index of monomial (a, b, NU-a-b) inside a degree-NU block.
+__host__ __device__ constexpr int mono_idx(int NU, int a, int b) { return (NU - a) * (NU - a + 1) / 2 + (NU - a - b); }
+
+// H = sum_alpha d[alpha] grad_u u^alpha over one (mu, NU) block, written as a
+// sum over the degree NU-1 monomials (Euler: u . H = NU * sum d P).
+template <int NU>
+__device__ __forceinline__ void grad_block(const double* __restrict__ d, const double* px, const double* py,
+                                           const double* pz, double& Hx, double& Hy, double& Hz)
+{
+#pragma unroll
+    for (int a = NU - 1; a >= 0; --a) {
+#pragma unroll
+        for (int b = NU - 1 - a; b >= 0; --b) {
+            const int c = NU - 1 - a - b;
+            const double P = px[a] * py[b] * pz[c];
+            Hx += (double)(a + 1) * d[mono_idx(NU, a + 1, b)] * P;
+            Hy += (double)(b + 1) * d[mono_idx(NU, a, b + 1)] * P;
+            Hz += (double)(c + 1) * d[mono_idx(NU, a, b)] * P;
+        }
+    }
+}
+
+__device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, const double* px, const double* py,
+                                                    const double* pz, double& Hx, double& Hy, double& Hz)
+{
+    switch (nu) {
+        case 1: grad_block<1>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 2: grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 3: grad_block<3>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 4: grad_block<4>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 5: grad_block<5>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 6: grad_block<6>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 7: grad_block<7>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 8: grad_block<8>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 9: grad_block<9>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 10: grad_block<10>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 11: grad_block<11>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 12: grad_block<12>(d, px, py, pz, Hx, Hy, Hz); break;
+        default: break;
+    }
+}
+
+// LDS layout (in doubles) shared by host sizing and the kernel.
+__host__ __device__ inline AtomLds atom_lds_layout(int amc, int nslots, int R, int nu_max, int n_parts)
+{
+    AtomLds L;
+    int mlen = amc;
+    if (mlen < n_parts * nslots * 3) mlen = n_parts * nslots * 3;
+    int dlen = amc;
+    if (dlen < nslots * 3 * (nu_max + 1)) dlen = nslots * 3 * (nu_max + 1);
+    L.m = 0;
+    L.dm = L.m + mlen;
+    L.f = L.dm + dlen;
+    L.fp = L.f + R * nslots;
+    L.vx = L.fp + R * nslots;
+    L.vy = L.vx + nslots;
+    L.vz = L.vy + nslots;
+    L.r = L.vz + nslots;
+    L.red = L.r + nslots;
+    L.ints = L.red + kBlock;                 // int region starts here (in doubles)
+    L.total_doubles = L.ints + (2 * nslots + 8 + 1) / 2 + 1;
+    return L;
+}
+
+size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts)
+{
+    return sizeof(double) * (size_t)atom_lds_layout(amc, nslots, R, nu_max, n_parts).total_doubles;
+}
+
+template <typename RadialReal>
+__global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    const int row = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int K = A.nslots;
+    const AtomLds L = atom_lds_layout(P.amc, K, P.R, P.nu_max, P.n_parts);
+    double* m = lds + L.m;
+    double* dm = lds + L.dm;
+    double* f = lds + L.f;
+    double* fp = lds + L.fp;
+    double* vx = lds + L.vx;
+    double* vy = lds + L.vy;
+    double* vz = lds + L.vz;
+    double* rr = lds + L.r;
+    double* red = lds + L.red;
+    int* ints = reinterpret_cast<int*>(lds + L.ints);
+    int* slot_of = ints;          // [K] compact index -> slot
+    int* cidx = ints + K;         // [K] slot -> compact index or -1
+    int* wave_cnt = ints + 2 * K; // [4]
+    int* nvp = wave_cnt + 4;
+
+    const int ti = A.itypes[row];
+    const double* rij_row = A.rij + (size_t)row * K * 3;
+    const int* jt_row = A.jtypes + (size_t)row * K;
+
+    // ---- A: radial basis and compaction of valid slots (slot order kept).
+    // The host guarantees K <= kBlock.
+    {
+        const int k = tid;
+        double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        bool valid = false;
+        if (k < K) {
+            v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+        }
+        const uint64_t mask = __ballot(valid);
+        const int lane = tid & (kWave - 1), wave = tid / kWave;
+        if (lane == 0) wave_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+        const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+        if (k < K) cidx[k] = valid ? pos : -1;
+        if (valid) {
+            double fv[kMaxR], dfv[kMaxR];
+            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
+            vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
+            slot_of[pos] = k;
+        }
+        if (tid == 0) {
+            int tot = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) tot += wave_cnt[w];
+            *nvp = tot;
+        }
+        __syncthreads();
+    }
+    const int nv = *nvp;
+    for (int x = P.B + tid; x < P.amc; x += kBlock) m[x] = 0.0;
+
+    // ---- B: basic moments.  Power table u^p in the dm region.
+    const int np1 = P.nu_max + 1;
+    double* pw = dm;
+    for (int t = tid; t < nv * 3; t += kBlock) {
+        const int k = t / 3, q = t - 3 * k;
+        const double inv = 1.0 / rr[k];
+        const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
+        double* row_pw = pw + (size_t)t * np1;
+        double v = 1.0;
+        row_pw[0] = 1.0;
+        for (int p = 1; p < np1; ++p) { v *= u; row_pw[p] = v; }
+    }
+    __syncthreads();
+    for (int t = tid; t < P.n_mono; t += kBlock) {
+        const I4 mo = P.mono[t];
+        const int nu = mo.w & 0xff, alpha = mo.w >> 8;
+        const int mumax = P.mu_max_of_nu[nu];
+        double acc[kMaxR];
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+        for (int k = 0; k < nv; ++k) {
+            const double* pk = pw + (size_t)k * 3 * np1;
+            const double Pm = pk[mo.x] * pk[np1 + mo.y] * pk[2 * np1 + mo.z];
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (mu <= mumax) acc[mu] += f[mu * K + k] * Pm;
+        }
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu)
+            if (mu <= mumax) m[P.blk_off[mu][nu] + alpha] = acc[mu];
+    }
+    __syncthreads();
+
+    // ---- C: contraction layers
+    for (int d = 1; d <= P.n_depth; ++d) {
+        const int t0 = P.fwd_layer[d], t1 = P.fwd_layer[d + 1];
+        for (int t = t0 + tid; t < t1; t += kBlock) {
+            double s = 0.0;
+            const int e1 = P.fwd_beg[t + 1];
+            for (int e = P.fwd_beg[t]; e < e1; ++e) {
+                const uint32_t pr = P.fwd_pair[e];
+                s += P.fwd_mult[e] * m[pr & 0xffffu] * m[pr >> 16];
+            }
+            m[P.fwd_tgt[t]] = s;
+        }
+        __syncthreads();
+    }
+
+    // ---- D: site energy
+    {
+        double acc = 0.0;
+        for (int s = tid; s < P.S; s += kBlock) acc += P.moment_coeffs[s] * m[P.mapping[s]];
+        red[tid] = acc;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
+    }
+
+    // ---- E: reverse layers (deepest first), gather form
+    for (int li = 0; li <= P.n_depth; ++li) {
+        const int t0 = P.rev_layer[li], t1 = P.rev_layer[li + 1];
+        for (int t = t0 + tid; t < t1; t += kBlock) {
+            const int x = P.rev_x[t];
+            double s = P.xi_of[x];
+            const int e1 = P.rev_beg[t + 1];
+            for (int e = P.rev_beg[t]; e < e1; ++e) {
+                const uint32_t pr = P.rev_pair[e];
+                s += P.rev_mult[e] * m[pr & 0xffffu] * dm[pr >> 16];
+            }
+            dm[x] = s;
+        }
+        __syncthreads();
+    }
+
+    // ---- F: per-slot gradient g_k = dE_i/dr_k.  Thread = (slot, part).
+    double* gpart = m;  // m is no longer needed
+    {
+        const int kk = tid & (kWave - 1), part = tid / kWave;
+        for (int k = kk; k < nv && part < P.n_parts; k += kWave) {
+            const double ir = 1.0 / rr[k];
+            const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
+            double px[kMaxNu + 1], py[kMaxNu + 1], pz[kMaxNu + 1];
+            px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+            for (int p = 1; p <= kMaxNu; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
+            double sdf = 0.0, sfn = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
+            const int b1 = P.part_beg[part + 1];
+            for (int bi = P.part_beg[part]; bi < b1; ++bi) {
+                const I2 bl = P.part_blk[bi];
+                const int mu = bl.x, nu = bl.y;
+                const double* d = dm + P.blk_off[mu][nu];
+                const double fm = f[mu * K + k], fpm = fp[mu * K + k];
+                if (nu == 0) {
+                    sdf += fpm * d[0];
+                } else {
+                    double Hx = 0.0, Hy = 0.0, Hz = 0.0;
+                    grad_block_dispatch(nu, d, px, py, pz, Hx, Hy, Hz);
+                    const double uH = ux * Hx + uy * Hy + uz * Hz;
+                    sdf += fpm * uH * (1.0 / (double)nu);
+                    sfn += fm * uH;
+                    Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
+                }
+            }
+            double* gp = gpart + ((size_t)part * K + k) * 3;
+            gp[0] = sdf * ux + (Gx - sfn * ux) * ir;
+            gp[1] = sdf * uy + (Gy - sfn * uy) * ir;
+            gp[2] = sdf * uz + (Gz - sfn * uz) * ir;
+        }
+    }
+    __syncthreads();
+    double* gk = dm;  // dm no longer needed
+    for (int t = tid; t < nv * 3; t += kBlock) {
+        double s = 0.0;
+        for (int part = 0; part < P.n_parts; ++part) s += gpart[(size_t)part * K * 3 + t];
+        gk[t] = s;
+    }
+    __syncthreads();
+    double* g_row = A.g + (size_t)row * K * 3;
+    for (int t = tid; t < K * 3; t += kBlock) {
+        const int k = t / 3, q = t - 3 * k;
+        const int c = cidx[k];
+        g_row[t] = c >= 0 ? gk[3 * c + q] : 0.0;
+    }
+    // ---- G: per-atom virial W_i[a][b] = sum_k r_k[a] g_k[b]
+    if (tid < 9) {
+        const int a = tid / 3, b = tid - 3 * (tid / 3);
+        const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
+        double s = 0.0;
+        for (int k = 0; k < nv; ++k) s += va[k] * gk[3 * k + b];
+        A.Wi[(size_t)row * 9 + tid] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Newton-3 force gather (jax_backend.py:263-303) without FP64 atomics:
+// a reverse-neighbor CSR is built per call; each atom sums its incoming
+// slots in increasing slot order, so results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+__global__ void zero_ints_kernel(int* p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void rev_count_kernel(const int* js, int nrows, int nslots, int n_f, int* cnt)
+{
+    const long long total = (long long)nrows * nslots;
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < total;
+         s += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(s / nslots);
+        const int j = js[s];
+        if (i < n_f && j >= 0 && j < n_f && j < nrows) atomicAdd(&cnt[j], 1);
+    }
+}
+
+// Exclusive scan of cnt[0..n) into off[0..n] by one workgroup.
+__global__ void __launch_bounds__(1024) scan_kernel(const int* cnt, int n, int* off)
+{
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < n ? cnt[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 1; w < 1024; w <<= 1) {
+            const int add = threadIdx.x >= w ? buf[threadIdx.x - w] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) off[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) off[n] = carry;
+}
+
+__global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, const int* off, int* cur, int* rev)
+{
+    const long long total = (long long)nrows * nslots;
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < total;
+         s += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(s / nslots);
+        const int j = js[s];
+        if (i < n_f && j >= 0 && j < n_f && j < nrows) {
+            const int p = atomicAdd(&cur[j], 1);
+            rev[off[j] + p] = (int)s;
+        }
+    }
+}
+
+__global__ void force_kernel(const double* g, int nrows, int nslots, int n_f, const int* off, int* rev, double* F)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nrows) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (a < n_f) {
+        const double* ga = g + (size_t)a * nslots * 3;
+        for (int k = 0; k < nslots; ++k) { fx += ga[3 * k]; fy += ga[3 * k + 1]; fz += ga[3 * k + 2]; }
+        const int s0 = off[a], s1 = off[a + 1];
+        // insertion sort of the (short) incoming segment -> fixed summation order
+        for (int i = s0 + 1; i < s1; ++i) {
+            const int v = rev[i];
+            int j = i - 1;
+            while (j >= s0 && rev[j] > v) { rev[j + 1] = rev[j]; --j; }
+            rev[j + 1] = v;
+        }
+        for (int i = s0; i < s1; ++i) {
+            const double* gs = g + (size_t)rev[i] * 3;
+            fx -= gs[0]; fy -= gs[1]; fz -= gs[2];
+        }
+    }
+    F[3 * (size_t)a] = fx; F[3 * (size_t)a + 1] = fy; F[3 * (size_t)a + 2] = fz;
+}
+
+// out[0] = E, out[1..6] = stress (reference Voigt xx yy zz yz xz xy, /V),
+// out[7..12] = LAMMPS virial (xx yy zz xy xz yz, energy units), out[13] =
+// status (0 ok, 1 non-finite energy or virial).
+__global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const double* Wi, int nrows, int n_e,
+                                                      int cell_rank, double volume, double* out)
+{
+    __shared__ double buf[1024];
+    __shared__ double acc[10];
+    const int tid = threadIdx.x;
+    for (int q = 0; q < 10; ++q) {
+        double s = 0.0;
+        if (q == 0) {
+            for (int i = tid; i < n_e && i < nrows; i += 1024) s += Ei[i];
+        } else {
+            for (int i = tid; i < nrows; i += 1024) s += Wi[(size_t)i * 9 + (q - 1)];
+        }
+        buf[tid] = s;
+        __syncthreads();
+        for (int w = 512; w > 0; w >>= 1) {
+            if (tid < w) buf[tid] += buf[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) acc[q] = buf[0];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double* W = acc + 1;
+        out[0] = acc[0];
+        const int idx[6] = {0, 4, 8, 5, 2, 1};
+        for (int q = 0; q < 6; ++q) {
+            const int a = idx[q] / 3, b = idx[q] % 3;
+            out[1 + q] = cell_rank == 3 ? (W[3 * a + b] + W[3 * b + a]) * (0.5 / volume) : __builtin_nan("");
+        }
+        // LAMMPS virial = -sym(W), order xx yy zz xy xz yz
+        out[7] = -W[0];
+        out[8] = -W[4];
+        out[9] = -W[8];
+        out[10] = -0.5 * (W[1] + W[3]);
+        out[11] = -0.5 * (W[2] + W[6]);
+        out[12] = -0.5 * (W[5] + W[7]);
+        bool bad = !isfinite(out[0]);
+        for (int q = 0; q < 9; ++q) bad = bad || !isfinite(W[q]);
+        out[13] = bad ? 1.0 : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t set_atom_kernel_lds(size_t bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)atom_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
+                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end)
+{
+    if (A.nrows <= 0) {
+        hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, 0, 0, F.cell_rank, F.volume,
+                           F.out);
+        return hipGetLastError();
+    }
+    if (ev_begin) hipEventRecord(ev_begin, stream);
+    if (radial_fp32)
+        hipLaunchKernelGGL(atom_kernel<float>, dim3(A.nrows), dim3(kBlock), lds_bytes, stream, P, A);
+    else
+        hipLaunchKernelGGL(atom_kernel<double>, dim3(A.nrows), dim3(kBlock), lds_bytes, stream, P, A);
+    if (ev_end) hipEventRecord(ev_end, stream);
+    const int n = A.nrows;
+    const long long slots = (long long)A.nrows * A.nslots;
+    int gs = (int)((slots + 255) / 256);
+    if (gs > 65536) gs = 65536;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, F.cnt, 2 * n);
+    hipLaunchKernelGGL(rev_count_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.cnt);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, F.cnt, n, F.off);
+    hipLaunchKernelGGL(rev_fill_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.off,
+                       F.cnt + n, F.rev);
+    hipLaunchKernelGGL(force_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
+                       F.off, F.rev, F.F);
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, A.nrows, F.n_e, F.cell_rank,
+                       F.volume, F.out);
+    return hipGetLastError();
+}
+
+}  // namespace mtp

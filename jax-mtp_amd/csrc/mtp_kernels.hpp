@@ -1,0 +1,46 @@
+// Launch interface of the HIP kernels (mtp_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mtp_plan.hpp"
+
+namespace mtp {
+
+// Padded per-row inputs/outputs of one evaluation (the array contract of
+// calc_energy_forces_stress_padded, jax_backend.py:374-421, in FP64).
+struct AtomArgs {
+    int nrows, nslots;     // A rows, K slots per row
+    const double* rij;     // [A][K][3]
+    const int* itypes;     // [A]
+    const int* jtypes;     // [A][K]
+    const int* js;         // [A][K]
+    double* Ei;            // [A]      site energies
+    double* g;             // [A][K][3] dE_i/dr_ik
+    double* Wi;            // [A][9]   per-atom sum_k r_ik (x) g_ik
+};
+
+struct ForceArgs {
+    int n_e, n_f, cell_rank;
+    double volume;
+    int* cnt;              // [2A] scratch (counts, cursors)
+    int* off;              // [A+1]
+    int* rev;              // [A*K]
+    double* F;             // [A][3]
+    double* out;           // [14]: E, stress6, virial6, status
+};
+
+struct AtomLds {
+    int m, dm, f, fp, vx, vy, vz, r, red, ints, total_doubles;
+};
+
+size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
+hipError_t set_atom_kernel_lds(size_t bytes);
+// ev_begin / ev_end (may be null) are recorded on `stream` around the atom
+// kernel launch, for live per-kernel timing.
+hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
+                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin = nullptr,
+                         hipEvent_t ev_end = nullptr);
+
+constexpr int kAtomBlock = 256;  // threads per atom workgroup; max slots per row
+
+}  // namespace mtp

@@ -1,0 +1,148 @@
+// Plan construction from the .mtp alpha tables (see mtp_plan.hpp).
+#include "mtp_plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace mtp {
+
+void refresh_plan_params(HostPlan& p)
+{
+    const MtpFile& f = p.file;
+    p.xi_of.assign(f.alpha_moments_count, 0.0);
+    for (int s = 0; s < f.S(); ++s) p.xi_of[f.alpha_moment_mapping[s]] += f.moment_coeffs[s];
+}
+
+bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err)
+{
+    if (!validate_mtp(f, err)) return false;
+    p = HostPlan();
+    p.file = f;
+    p.level = level_from_moments_count(f.alpha_moments_count);
+    const int amc = f.alpha_moments_count, B = f.B(), T = f.T(), R = f.radial_funcs_count;
+    if (R > kMaxR) { err = "radial_funcs_count above " + std::to_string(kMaxR) + " is not supported"; return false; }
+    if (f.radial_basis_size > kMaxN) { err = "radial_basis_size above " + std::to_string(kMaxN) + " is not supported"; return false; }
+    if (amc > 65535) { err = "alpha_moments_count above 65535 is not supported"; return false; }
+
+    // --- basic moments as canonical (mu, nu) blocks -----------------------
+    for (auto& row : p.blk_off) for (int& v : row) v = -1;
+    for (int& v : p.mu_max_of_nu) v = -1;
+    int b = 0;
+    while (b < B) {
+        const int* e = &f.alpha_index_basic[4 * b];
+        int mu = e[0], nu = e[1] + e[2] + e[3];
+        if (nu > kMaxNu) { err = "tensor order above " + std::to_string(kMaxNu) + " is not supported"; return false; }
+        int n = (nu + 1) * (nu + 2) / 2;
+        if (b + n > B) { err = "alpha_index_basic is not made of complete symmetric tensor blocks"; return false; }
+        int q = 0;
+        for (int a = nu; a >= 0; --a)
+            for (int bb = nu - a; bb >= 0; --bb, ++q) {
+                const int* r = &f.alpha_index_basic[4 * (b + q)];
+                if (r[0] != mu || r[1] != a || r[2] != bb || r[3] != nu - a - bb) {
+                    err = "alpha_index_basic is not in canonical (mu, nu) block order";
+                    return false;
+                }
+            }
+        if (p.blk_off[mu][nu] != -1) { err = "alpha_index_basic repeats a (mu, nu) block"; return false; }
+        p.blk_off[mu][nu] = b;
+        p.mu_max_of_nu[nu] = std::max(p.mu_max_of_nu[nu], mu);
+        p.nu_max = std::max(p.nu_max, nu);
+        b += n;
+    }
+    // every mu <= mu_max_of_nu[nu] must have the block (kernels loop mu = 0..mu_max)
+    for (int nu = 0; nu <= p.nu_max; ++nu)
+        for (int mu = 0; mu <= p.mu_max_of_nu[nu]; ++mu)
+            if (p.blk_off[mu][nu] < 0) {
+                err = "basic (mu, nu) blocks are not contiguous in mu";
+                return false;
+            }
+    for (int nu = 0; nu <= p.nu_max; ++nu) {
+        int q = 0;
+        for (int a = nu; a >= 0; --a)
+            for (int bb = nu - a; bb >= 0; --bb, ++q) p.mono.push_back({a, bb, nu - a - bb, nu | (q << 8)});
+    }
+
+    // --- dependency depths ------------------------------------------------
+    p.depth.assign(amc, 0);
+    for (int e = 0; e < T; ++e) {
+        const int* t = &f.alpha_index_times[4 * e];
+        p.depth[t[3]] = std::max(p.depth[t[3]], 1 + std::max(p.depth[t[0]], p.depth[t[1]]));
+    }
+    p.n_depth = amc ? *std::max_element(p.depth.begin(), p.depth.end()) : 0;
+    if (p.n_depth > kMaxDepth) { err = "contraction depth above " + std::to_string(kMaxDepth); return false; }
+
+    // --- forward layers: targets grouped by depth, entries in file order ---
+    std::vector<std::vector<int>> entries_of(amc);
+    for (int e = 0; e < T; ++e) entries_of[f.alpha_index_times[4 * e + 3]].push_back(e);
+    p.fwd_layer.assign(kMaxDepth + 2, 0);
+    p.fwd_beg.push_back(0);
+    for (int d = 1; d <= p.n_depth; ++d) {
+        p.fwd_layer[d] = (int)p.fwd_tgt.size();
+        for (int x = 0; x < amc; ++x) {
+            if (p.depth[x] != d || entries_of[x].empty()) continue;
+            p.fwd_tgt.push_back(x);
+            for (int e : entries_of[x]) {
+                const int* t = &f.alpha_index_times[4 * e];
+                p.fwd_pair.push_back((uint32_t)t[0] | ((uint32_t)t[1] << 16));
+                p.fwd_mult.push_back((double)t[2]);
+            }
+            p.fwd_beg.push_back((int)p.fwd_pair.size());
+        }
+    }
+    for (int d = p.n_depth + 1; d < kMaxDepth + 2; ++d) p.fwd_layer[d] = (int)p.fwd_tgt.size();
+
+    // --- reverse layers: moments grouped by depth, deepest first ----------
+    std::vector<std::vector<std::pair<int, int>>> uses(amc);  // (entry, role)
+    for (int e = 0; e < T; ++e) {
+        const int* t = &f.alpha_index_times[4 * e];
+        uses[t[0]].push_back({e, 0});
+        uses[t[1]].push_back({e, 1});
+    }
+    p.rev_layer.assign(kMaxDepth + 2, 0);
+    p.rev_beg.push_back(0);
+    int li = 0;
+    for (int d = p.n_depth; d >= 0; --d, ++li) {
+        p.rev_layer[li] = (int)p.rev_x.size();
+        for (int x = 0; x < amc; ++x) {
+            if (p.depth[x] != d) continue;
+            p.rev_x.push_back(x);
+            for (auto& u : uses[x]) {
+                const int* t = &f.alpha_index_times[4 * u.first];
+                int other = u.second == 0 ? t[1] : t[0];
+                p.rev_pair.push_back((uint32_t)other | ((uint32_t)t[3] << 16));
+                p.rev_mult.push_back((double)t[2]);
+            }
+            p.rev_beg.push_back((int)p.rev_pair.size());
+        }
+    }
+    for (; li < kMaxDepth + 2; ++li) p.rev_layer[li] = (int)p.rev_x.size();
+
+    // --- per-neighbor gradient: blocks split into n_parts by cost ----------
+    if (n_parts < 1) n_parts = 1;
+    if (n_parts > 8) n_parts = 8;
+    p.n_parts = n_parts;
+    std::vector<std::pair<int, I2>> blocks;  // (cost, (mu, nu))
+    for (int mu = 0; mu < kMaxR; ++mu)
+        for (int nu = 0; nu <= kMaxNu; ++nu)
+            if (p.blk_off[mu][nu] >= 0) blocks.push_back({nu == 0 ? 1 : nu * (nu + 1) / 2 + 2, I2{mu, nu}});
+    std::stable_sort(blocks.begin(), blocks.end(), [](auto& a, auto& c) { return a.first > c.first; });
+    std::vector<std::vector<I2>> parts(n_parts);
+    std::vector<int> load(n_parts, 0);
+    for (auto& bl : blocks) {
+        int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        parts[k].push_back(bl.second);
+        load[k] += bl.first;
+    }
+    p.part_beg.assign(9, 0);
+    for (int k = 0; k < n_parts; ++k) {
+        p.part_beg[k] = (int)p.part_blk.size();
+        for (auto& bl : parts[k]) p.part_blk.push_back(bl);
+    }
+    for (int k = n_parts; k < 9; ++k) p.part_beg[k] = (int)p.part_blk.size();
+
+    refresh_plan_params(p);
+    return true;
+}
+
+}  // namespace mtp

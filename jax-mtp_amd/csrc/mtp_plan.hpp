@@ -1,0 +1,98 @@
+// Evaluation plan: the .mtp alpha tables re-laid for the HIP kernels.
+//
+// The reference evaluates the level-specific basis either as a full-tensor
+// contraction tree (jax_backend.py:162-247) or, in MLIP form, as a
+// sequential walk over alpha_index_times (conversion.py:95-102).  The plan
+// keeps the MLIP form (same values, ~50x fewer FLOPs at level 24) and
+// reorganises it for a GPU:
+//
+//  * basic moments are grouped into (mu, nu) blocks of complete symmetric
+//    tensors in canonical monomial order (verified; every reference level
+//    2..28 has this layout);
+//  * alpha_index_times is split into dependency layers ("depths"): a target
+//    moment's depth is 1 + the max depth of its operands.  Within a layer
+//    every target is independent, so one thread owns one target and sums
+//    its entries in file order (no atomics, deterministic);
+//  * the reverse (adjoint) pass is a gather: for every moment x, the list of
+//    (other operand, target, multiplier) of the entries that read x, so that
+//    dE/dm[x] = xi(x) + sum mult * m[other] * dE/dm[target], processed from
+//    the deepest layer down.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mtp_format.hpp"
+
+namespace mtp {
+
+struct I4 { int x, y, z, w; };
+struct I2 { int x, y; };
+
+constexpr int kMaxR = 8;      // radial functions (mu) supported
+constexpr int kMaxNu = 12;    // tensor order supported
+constexpr int kMaxN = 32;     // Chebyshev terms supported
+constexpr int kMaxDepth = 8;  // contraction layers supported
+
+// Device-side view of the plan (all pointers are device pointers).
+struct DevPlan {
+    int Sp, R, N, B, T, amc, S, nu_max;
+    double scaling, min_dist, max_dist;
+    const double* radial_coeffs;   // [Sp][Sp][R][N]
+    const double* species_coeffs;  // [Sp]
+    const double* moment_coeffs;   // [S]
+    const int* mapping;            // [S]
+    const double* xi_of;           // [amc]  sum of moment_coeffs mapped to each moment
+    // basic (mu, nu) blocks
+    int blk_off[kMaxR][kMaxNu + 1];  // offset of block (mu, nu) in m[], -1 if absent
+    int mu_max_of_nu[kMaxNu + 1];    // largest mu with a (mu, nu) block, -1 if none
+    int n_mono;                      // monomials of degree 0..nu_max
+    const I4* mono;                // [n_mono] (a, b, c, nu | alpha_in_block << 8)
+    // forward layers
+    int n_depth;
+    int fwd_layer[kMaxDepth + 2];    // target index ranges per depth 1..n_depth (prefix, into fwd_tgt)
+    const int* fwd_tgt;              // [n_targets] target moment
+    const int* fwd_beg;              // [n_targets + 1] entry ranges
+    const uint32_t* fwd_pair;        // [T] i1 | i2 << 16
+    const double* fwd_mult;          // [T]
+    // reverse layers, deepest first
+    int rev_layer[kMaxDepth + 2];    // moment index ranges (prefix, into rev_x)
+    const int* rev_x;                // [n_rev] moment
+    const int* rev_beg;              // [n_rev + 1]
+    const uint32_t* rev_pair;        // [2T] other | target << 16
+    const double* rev_mult;          // [2T]
+    // per-neighbor gradient: (mu, nu) blocks split into parts
+    int n_parts;
+    int part_beg[9];                 // prefix into part_blk
+    const I2* part_blk;            // (mu, nu)
+};
+
+struct HostPlan {
+    MtpFile file;
+    int level = -1;
+    int nu_max = 0, n_depth = 0, n_parts = 4;
+    std::vector<int> depth;  // [amc]
+    std::vector<I4> mono;
+    std::vector<int> fwd_tgt, fwd_beg;
+    std::vector<uint32_t> fwd_pair;
+    std::vector<double> fwd_mult;
+    std::vector<int> fwd_layer;
+    std::vector<int> rev_x, rev_beg;
+    std::vector<uint32_t> rev_pair;
+    std::vector<double> rev_mult;
+    std::vector<int> rev_layer;
+    std::vector<double> xi_of;
+    std::vector<I2> part_blk;
+    std::vector<int> part_beg;
+    int blk_off[kMaxR][kMaxNu + 1];
+    int mu_max_of_nu[kMaxNu + 1];
+};
+
+// Build the plan from a validated file.  n_parts splits the per-neighbor
+// gradient work.  Returns false with a message for unsupported layouts.
+bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err);
+
+// Recompute the parameter-dependent part (xi_of) after a coefficient update.
+void refresh_plan_params(HostPlan& p);
+
+}  // namespace mtp

@@ -1,0 +1,167 @@
+"""Python host interface of the HIP evaluator.
+
+``MTPPotential.calc_energy_forces_stress_padded`` mirrors the reference's
+entry point of the same name (jax_backend.py:374-460): the same 8 dynamic
+arguments, the same padding/masking semantics, and the same outputs
+(energy, forces[A,3], stress6 in the reference's Voigt order), in FP64.
+PyTorch only supplies device memory and the stream; all arithmetic runs in
+the HIP kernels of libmtp_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .mtp_io import MTPData, format_mtp, read_mtp
+
+
+class _DevArray:
+    """Minimal __cuda_array_interface__ view of a device pointer."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
+def _i32(t, dev):
+    import torch
+
+    return torch.as_tensor(t, dtype=torch.int32, device=dev).contiguous()
+
+
+def _f64(t, dev):
+    import torch
+
+    return torch.as_tensor(t, dtype=torch.float64, device=dev).contiguous()
+
+
+class MTPPotential:
+    """One potential on one GPU (an ``mtp_ctx``)."""
+
+    def __init__(self, params, max_atoms: int, max_nbr: int, cutoff: float | None = None, debug_level: int = 0,
+                 radial_fp32: bool = False):
+        if isinstance(params, (str, bytes)) or hasattr(params, "__fspath__"):
+            params = read_mtp(params)
+        self.params: MTPData = params
+        self.max_atoms = int(max_atoms)
+        self.max_nbr = int(max_nbr)
+        L = capi.lib()
+        ctx = ctypes.c_void_p()
+        cut = float(cutoff if cutoff is not None else params.max_dist)
+        capi.check(L.mtp_open_text(format_mtp(params).encode(), self.max_atoms, self.max_nbr, cut, int(debug_level),
+                                   ctypes.byref(ctx)))
+        self.ctx = ctx
+        self.set_precision(radial_fp32)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            capi.lib().mtp_close(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, radial_fp32: bool):
+        self.radial_fp32 = bool(radial_fp32)
+        capi.check(capi.lib().mtp_set_precision(self.ctx, int(self.radial_fp32)), self.ctx)
+
+    def info(self) -> dict:
+        v = [ctypes.c_int() for _ in range(8)]
+        capi.check(capi.lib().mtp_info(self.ctx, *[ctypes.byref(x) for x in v]), self.ctx)
+        keys = ["level", "species_count", "radial_funcs_count", "radial_basis_size", "alpha_moments_count",
+                "basic_count", "times_count", "scalar_count"]
+        return {k: x.value for k, x in zip(keys, v)}
+
+    def set_params(self, moment_coeffs=None, species_coeffs=None, radial_coeffs=None, scaling=None):
+        def ptr(a):
+            if a is None:
+                return None, None
+            arr = np.ascontiguousarray(a, dtype=np.float64)
+            return arr, arr.ctypes.data_as(ctypes.c_void_p)
+
+        m, mp = ptr(moment_coeffs)
+        s, sp = ptr(species_coeffs)
+        r, rp = ptr(radial_coeffs)
+        capi.check(capi.lib().mtp_set_params(self.ctx, mp, sp, rp, float(scaling) if scaling else float("nan")),
+                   self.ctx)
+        if m is not None:
+            self.params.moment_coeffs = m.copy()
+        if s is not None:
+            self.params.species_coeffs = s.copy()
+        if r is not None:
+            self.params.radial_coeffs = r.reshape(self.params.radial_coeffs.shape).copy()
+        if scaling:
+            self.params.scaling = float(scaling)
+
+    # ------------------------------------------------------------------
+    def calc_energy_forces_stress_padded(self, itypes, all_js, all_rijs, all_jtypes, cell_rank, volume,
+                                         natoms_energy, natoms_force, return_all: bool = False, sync: bool = True):
+        """jax_backend.py:374-460 on the GPU.
+
+        Inputs may be numpy arrays or torch tensors (device tensors are used
+        in place).  Returns (energy, forces[A,3], stress6) as torch float64
+        tensors on the device; with ``return_all`` also a dict holding
+        ``virial6`` (LAMMPS order), ``eatom`` [A], ``status`` and the per-slot
+        gradients ``g`` [A,K,3].
+        """
+        import torch
+
+        dev = torch.device("cuda")
+        it = _i32(itypes, dev)
+        js = _i32(all_js, dev)
+        rij = _f64(all_rijs, dev)
+        jt = _i32(all_jtypes, dev)
+        A, K = js.shape
+        if rij.shape != (A, K, 3) or jt.shape != (A, K) or it.shape != (A,):
+            raise ValueError("inconsistent padded shapes")
+        out = torch.empty(14, dtype=torch.float64, device=dev)
+        F = torch.empty((A, 3), dtype=torch.float64, device=dev)
+        eatom = torch.empty(A, dtype=torch.float64, device=dev) if return_all else None
+        stream = torch.cuda.current_stream().cuda_stream
+        L = capi.lib()
+        capi.check(L.mtp_compute_padded_device(
+            self.ctx, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(), jt.data_ptr(), int(cell_rank),
+            float(volume), int(natoms_energy), int(natoms_force), out.data_ptr(), F.data_ptr(),
+            eatom.data_ptr() if eatom is not None else None, stream), self.ctx)
+        if sync:
+            capi.check(L.mtp_sync(self.ctx, stream), self.ctx)
+        E, stress = out[0], out[1:7]
+        if not return_all:
+            return E, F, stress
+        gptr = L.mtp_last_slot_gradients(self.ctx)
+        g = torch.as_tensor(_DevArray(gptr, (A, K, 3), "<f8"), device=dev).clone()
+        return E, F, stress, dict(virial6=out[7:13], eatom=eatom, status=out[13], g=g)
+
+    # ------------------------------------------------------------------
+    def compute_rows(self, natoms, rij_rows, types, nbr_rows, nbr_counts, nbr_type_rows, volume):
+        """The pair style's call (zero_overhead.hpp:58-64) from numpy row arrays.
+
+        rij_rows [natoms, 3*K], nbr_rows/nbr_type_rows [natoms, K].
+        Returns (energy, forces[natoms,3], stress6, virial6).
+        """
+        K = self.max_nbr
+        rij_rows = np.ascontiguousarray(rij_rows, dtype=np.float64).reshape(natoms, 3 * K)
+        nbr_rows = np.ascontiguousarray(nbr_rows, dtype=np.int32).reshape(natoms, K)
+        nbr_type_rows = np.ascontiguousarray(nbr_type_rows, dtype=np.int32).reshape(natoms, K)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        counts = np.ascontiguousarray(nbr_counts, dtype=np.int32)
+
+        def rows(a):
+            return (ctypes.c_void_p * max(natoms, 1))(*[a[i].ctypes.data for i in range(natoms)])
+
+        pr, pn, pt = rows(rij_rows), rows(nbr_rows), rows(nbr_type_rows)
+        E = ctypes.c_double()
+        F = np.zeros((natoms, 3))
+        s6 = np.zeros(6)
+        v6 = np.zeros(6)
+        vp = ctypes.c_void_p
+        capi.check(capi.lib().mtp_compute(
+            self.ctx, natoms, int(counts.sum()), ctypes.cast(pr, vp), types.ctypes.data_as(vp), ctypes.cast(pn, vp),
+            counts.ctypes.data_as(vp), ctypes.cast(pt, vp), float(volume), ctypes.byref(E),
+            F.ctypes.data_as(vp), s6.ctypes.data_as(vp), v6.ctypes.data_as(vp)), self.ctx)
+        return E.value, F, s6, v6

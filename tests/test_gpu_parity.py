@@ -1,0 +1,148 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+Tolerance (FP64 path): norm-wise relative error <= 1e-10 for the energy,
+the forces (max-abs error / max-abs reference force) and the stress.
+FP32-radial variant: its error against the FP64 oracle is reported and
+bounded by 1e-4 (norm-wise) on forces.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=float); b = np.asarray(b, dtype=float)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / den) if den > 0 else float(np.max(np.abs(a - b)))
+
+
+def _run_gpu(jm, w, nb, radial_fp32=False, n_e=None, n_f=None, rows=None):
+    A = nb["js"].shape[0]
+    pot = jm.MTPPotential(w.params, max_atoms=A, max_nbr=nb["js"].shape[1], radial_fp32=radial_fp32)
+    n_e = w.natoms if n_e is None else n_e
+    n_f = w.natoms if n_f is None else n_f
+    E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           w.volume, n_e, n_f, return_all=True)
+    out = dict(E=float(E.item()), F=F.cpu().numpy(), s6=s6.cpu().numpy(), v6=extra["virial6"].cpu().numpy(),
+               eatom=extra["eatom"].cpu().numpy(), g=extra["g"].cpu().numpy(), status=float(extra["status"].item()))
+    pot.close()
+    return out
+
+
+def _run_oracle(O, w, nb, n_e=None, n_f=None):
+    orc = oracle_for(O, w.params)
+    n_e = w.natoms if n_e is None else n_e
+    n_f = w.natoms if n_f is None else n_f
+    E, F, s6, Ei, g = orc.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           w.volume, n_e, n_f)
+    return dict(E=E, F=F, s6=s6, eatom=Ei, g=g)
+
+
+def _check(gpu, ref, tol=TOL):
+    print(f"E {gpu['E']:.17g} ref {ref['E']:.17g} | rel F {_rel(gpu['F'], ref['F']):.3e} "
+          f"| rel s6 {_rel(gpu['s6'], ref['s6']):.3e} | rel Ei {_rel(gpu['eatom'], ref['eatom']):.3e}", flush=True)
+    assert gpu["status"] == 0.0
+    assert abs(gpu["E"] - ref["E"]) <= tol * max(abs(ref["E"]), np.max(np.abs(ref["eatom"])))
+    assert _rel(gpu["eatom"], ref["eatom"]) <= tol
+    assert _rel(gpu["g"], ref["g"]) <= tol
+    assert _rel(gpu["F"], ref["F"]) <= tol
+    assert _rel(gpu["s6"], ref["s6"]) <= tol
+
+
+@pytest.mark.parametrize("level", [2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24])
+def test_levels_random_cells(jm, O, level):
+    w = jm.configs.small_cluster(level, 3, 24, seed=100 + level, cell=8.0, cutoff=5.0, max_nbr=80)
+    nb = w.neighbors()
+    print(f"[case] level={level} natoms={w.natoms} maxn={nb['counts'].max()}", flush=True)
+    _check(_run_gpu(jm, w, nb), _run_oracle(O, w, nb))
+
+
+def test_fcc256_level10(jm, O):
+    w = jm.configs.fcc256_level10(seed=0)
+    nb = w.neighbors()
+    print(f"[case] fcc256 level10 maxn={nb['counts'].max()}", flush=True)
+    gpu, ref = _run_gpu(jm, w, nb), _run_oracle(O, w, nb)
+    _check(gpu, ref)
+    # periodic cell: total force vanishes
+    assert np.abs(gpu["F"].sum(axis=0)).max() <= 1e-10 * np.abs(gpu["F"]).max()
+
+
+def test_bcc2889_level16(jm, O):
+    w = jm.configs.bcc2889_level16(seed=0)
+    nb = w.neighbors()
+    print(f"[case] bcc2889 level16 maxn={nb['counts'].max()}", flush=True)
+    _check(_run_gpu(jm, w, nb), _run_oracle(O, w, nb))
+
+
+def test_masks_and_padding_rows(jm, O):
+    """natoms_energy / natoms_force below the row count (jax_backend.py:279-301, 447-452)."""
+    w = jm.configs.small_cluster(12, 2, 20, seed=9, cell=7.5, cutoff=5.0, max_nbr=64)
+    nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, w.cutoff, w.max_nbr, rows=24)
+    print("[case] masks rows=24 natoms=20 n_e=17 n_f=15", flush=True)
+    _check(_run_gpu(jm, w, nb, n_e=17, n_f=15), _run_oracle(O, w, nb, n_e=17, n_f=15))
+
+
+def test_fcc32k_level24_full_size(jm, O):
+    """The 32,000-atom level-24 workload at full size against the oracle."""
+    w = jm.configs.fcc32k_level24(seed=0)
+    nb = w.neighbors()
+    print(f"[case] fcc32k level24 maxn={nb['counts'].max()}", flush=True)
+    gpu = _run_gpu(jm, w, nb)
+    ref = _run_oracle(O, w, nb)
+    _check(gpu, ref)
+    assert np.abs(gpu["F"].sum(axis=0)).max() <= 1e-9 * np.abs(gpu["F"]).max()
+
+
+def test_radial_fp32_variant_error(jm, O):
+    w = jm.configs.fcc32k_level24(seed=0, ncell=6)
+    nb = w.neighbors()
+    print(f"[case] fp32-radial fcc{w.natoms} level24", flush=True)
+    gpu = _run_gpu(jm, w, nb, radial_fp32=True)
+    ref = _run_oracle(O, w, nb)
+    errF = _rel(gpu["F"], ref["F"])
+    errE = abs(gpu["E"] - ref["E"]) / np.max(np.abs(ref["eatom"]))
+    print(f"FP32-radial: force err {errF:.3e}, energy err {errE:.3e}", flush=True)
+    assert errF < 1e-4 and errE < 1e-4
+
+
+def test_host_rows_api_matches_device_api(jm):
+    """mtp_compute (pair-style row contract) equals the device entry bit for bit."""
+    w = jm.configs.small_cluster(16, 2, 30, seed=4, cell=8.5, cutoff=5.0, max_nbr=64)
+    nb = w.neighbors()
+    pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=64)
+    E, F, s6, v6 = pot.compute_rows(w.natoms, nb["rij"].reshape(w.natoms, -1), nb["itypes"], nb["js"], nb["counts"],
+                                    nb["jtypes"], w.volume)
+    E2, F2, s62, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                              w.volume, w.natoms, w.natoms, return_all=True)
+    assert E == float(E2.item())
+    np.testing.assert_array_equal(F, F2.cpu().numpy())
+    np.testing.assert_array_equal(s6, s62.cpu().numpy())
+    np.testing.assert_array_equal(v6, extra["virial6"].cpu().numpy())
+    # LAMMPS virial = -V * stress, reordered xx yy zz xy xz yz
+    np.testing.assert_allclose(v6, -w.volume * s6[[0, 1, 2, 5, 4, 3]], rtol=1e-12, atol=1e-14)
+    pot.close()
+
+
+def test_deterministic(jm):
+    w = jm.configs.small_cluster(20, 2, 40, seed=8, cell=9.0, cutoff=5.0, max_nbr=80)
+    nb = w.neighbors()
+    a = _run_gpu(jm, w, nb)
+    b = _run_gpu(jm, w, nb)
+    assert a["E"] == b["E"]
+    np.testing.assert_array_equal(a["F"], b["F"])
+    np.testing.assert_array_equal(a["s6"], b["s6"])
+
+
+def test_capacity_errors(jm):
+    w = jm.configs.small_cluster(10, 1, 8, seed=1, cell=6.0, cutoff=5.0, max_nbr=40)
+    nb = w.neighbors()
+    pot = jm.MTPPotential(w.params, max_atoms=4, max_nbr=40)
+    with pytest.raises(jm.capi.MTPError) as e:
+        pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3, w.volume, 8, 8)
+    assert e.value.code == jm.capi.MTP_ERR_CAPACITY
+    pot.close()

@@ -107,6 +107,30 @@ int mtp_sync(mtp_ctx *ctx, void *stream);
 int mtp_profile(mtp_ctx *ctx, int enable);
 int mtp_profile_read(mtp_ctx *ctx, double *atom_kernel_ms, int *launches);
 
+/* ---- training (packed batches; device arrays) ------------------------
+ * A batch of nconf configurations is nrows concatenated atom rows;
+ * d_conf_off[nconf+1] gives each configuration's rows, d_conf_of[nrows] the
+ * configuration of each row.  d_js holds global row indices (>= nrows for an
+ * empty slot).  Replaces the reference's global padding of every
+ * configuration to (A_max, K_max) (motep_jax_train_import.py:219-295). */
+
+/* Per-configuration energies d_E[nconf], forces d_forces[nrows*3] and
+ * stress d_stress[nconf*6] (reference Voigt order, / V_c)
+ * (jax.py:29-98 semantics with the Newton-3 scatter of jax_backend.py:263-303). */
+int mtp_batch_forward(mtp_ctx *ctx, int nrows, int nslots, int nconf, const int *d_conf_off, const int *d_conf_of,
+                      const int *d_itypes, const int *d_js, const double *d_rijs, const int *d_jtypes,
+                      const double *d_volumes, double *d_E, double *d_forces, double *d_stress, void *stream);
+
+/* Hand-written parameter gradient of sum gE.E + sum gF.F + sum gS.S for
+ * upstream gradients d_gE[nconf], d_gF[nrows*3], d_gS[nconf*6], written to
+ * d_grad[S + Sp + Sp*Sp*R*N] in the order (moment_coeffs, species_coeffs,
+ * radial_coeffs) of MTPData.parameters (data.py:60-72).  Replaces JAX's
+ * value_and_grad of the training loss (motep_jax_train_function.py:191). */
+int mtp_batch_param_grad(mtp_ctx *ctx, int nrows, int nslots, int nconf, const int *d_conf_off, const int *d_conf_of,
+                         const int *d_itypes, const int *d_js, const double *d_rijs, const int *d_jtypes,
+                         const double *d_volumes, const double *d_gE, const double *d_gF, const double *d_gS,
+                         double *d_grad, void *stream);
+
 /* ---- host-only helpers (no GPU work; usable without a device) -------- */
 
 /* Parse, validate and plan a .mtp file without touching the GPU.  Returns

@@ -39,6 +39,8 @@ SIGNATURES = {
     "mtp_sync": (_I, [_P, _P]),
     "mtp_profile": (_I, [_P, _I]),
     "mtp_profile_read": (_I, [_P, _P, _P]),
+    "mtp_batch_forward": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mtp_batch_param_grad": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mtp_check_file": (_I, [ctypes.c_char_p, _I, _P, _P, _P, _P, _P, _P, _P]),
     "mtp_pack_full_list": (_LL, [_I, _P, _P, _P, _I, _P, _P, _P, _D, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P]),
     "mtp_stage_host": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -16,6 +16,7 @@
 #include "mtp_format.hpp"
 #include "mtp_kernels.hpp"
 #include "mtp_plan.hpp"
+#include "mtp_train.hpp"
 #include "nbr_pack.hpp"
 
 using namespace mtp;
@@ -62,6 +63,10 @@ struct mtp_ctx {
     int *h_it = nullptr, *h_js = nullptr, *h_jt = nullptr;
     double* h_F = nullptr;
     double* h_out = nullptr;
+    // training buffers (allocated at the first training call)
+    size_t train_lds = 0;
+    double *d_gxi = nullptr, *d_gc = nullptr, *d_part = nullptr, *d_xisum = nullptr, *d_tsum = nullptr;
+    int lds_max = 0;
     // live per-kernel timing (events recorded around each atom-kernel launch)
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -154,6 +159,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
         delete c;
         return MTP_ERR_HIP;
     }
+    c->lds_max = lds_max;
     if ((long long)c->lds_bytes > (long long)lds_max) {
         set_global_error("potential needs " + std::to_string(c->lds_bytes) + " bytes of LDS per atom, device has " +
                          std::to_string(lds_max));
@@ -502,6 +508,68 @@ long long mtp_pack_full_list(int nlocal, const double* x, const int* type, const
     PackOut o{pos, nbr, ntype, counts, atom_types};
     return pack_full_list(nlocal, xr.data(), type, tag, inum, ilist, numneigh, firstneigh, cutoff, max_atoms,
                           max_nbr, mi, o, t2d, n_truncated);
+}
+
+int mtp_batch_forward(mtp_ctx* ctx, int nrows, int nslots, int nconf, const int* d_conf_off, const int* d_conf_of,
+                      const int* d_itypes, const int* d_js, const double* d_rijs, const int* d_jtypes,
+                      const double* d_volumes, double* d_E, double* d_forces, double* d_stress, void* stream)
+{
+    (void)d_conf_of;
+    if (!ctx) return MTP_ERR_ARG;
+    if (nconf <= 0 || !d_conf_off || !d_volumes || !d_E || !d_stress) return ctx->fail(MTP_ERR_ARG, "bad batch");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    // one evaluation over all rows: per-atom energies, gradients, virials, forces
+    int rc = mtp_compute_padded_device(ctx, nrows, nslots, d_itypes, d_js, d_rijs, d_jtypes, 3, 1.0, nrows, nrows,
+                                       ctx->d_out, d_forces, ctx->d_Ei, s);
+    if (rc) return rc;
+    hipError_t e = launch_seg_totals(ctx->d_Ei, ctx->d_Wi, d_conf_off, d_volumes, nconf, d_E, d_stress, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "batch totals");
+    return MTP_OK;
+}
+
+int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const int* d_conf_off, const int* d_conf_of,
+                         const int* d_itypes, const int* d_js, const double* d_rijs, const int* d_jtypes,
+                         const double* d_volumes, const double* d_gE, const double* d_gF, const double* d_gS,
+                         double* d_grad, void* stream)
+{
+    (void)d_conf_off;
+    if (!ctx) return MTP_ERR_ARG;
+    if (nrows <= 0 || nrows > ctx->max_atoms)
+        return ctx->fail(MTP_ERR_CAPACITY, "batch rows " + std::to_string(nrows) + " outside 1..max_atoms");
+    if (nslots <= 0 || nslots > ctx->max_nbr) return ctx->fail(MTP_ERR_CAPACITY, "batch slots exceed max_neighbors");
+    if (nconf <= 0 || !d_conf_of || !d_itypes || !d_js || !d_rijs || !d_jtypes || !d_volumes || !d_gE || !d_gF ||
+        !d_gS || !d_grad)
+        return ctx->fail(MTP_ERR_ARG, "null batch array");
+    const HostPlan& p = ctx->plan;
+    const MtpFile& f = p.file;
+    if (!ctx->d_gxi) {
+        ctx->train_lds = train_lds_bytes(f.alpha_moments_count, ctx->max_nbr, f.radial_funcs_count,
+                                         f.radial_basis_size, p.nu_max, p.n_parts);
+        if ((long long)ctx->train_lds > (long long)ctx->lds_max)
+            return ctx->fail(MTP_ERR_UNSUPPORTED, "the parameter gradient needs " + std::to_string(ctx->train_lds) +
+                                                      " bytes of LDS per atom at this level; the device has " +
+                                                      std::to_string(ctx->lds_max));
+        hipError_t e = set_train_kernel_lds(ctx->train_lds);
+        const size_t A = (size_t)ctx->max_atoms;
+        const size_t ncols = 1 + (size_t)f.species_count * f.radial_funcs_count * f.radial_basis_size;
+        const size_t nchunk = (A + 1023) / 1024;
+        const size_t part = nchunk * std::max((size_t)f.S(), (size_t)f.species_count * ncols);
+        ctx->d_gxi = ctx->alloc<double>(A * (size_t)std::max(1, f.S()), e);
+        ctx->d_gc = ctx->alloc<double>(A * ncols, e);
+        ctx->d_part = ctx->alloc<double>(part, e);
+        ctx->d_xisum = ctx->alloc<double>((size_t)std::max(1, f.S()), e);
+        ctx->d_tsum = ctx->alloc<double>((size_t)f.species_count * ncols, e);
+        if (e != hipSuccess) return ctx->hip_fail(e, "training buffers");
+    }
+    TrainArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.nconf = nconf;
+    A.rij = d_rijs; A.itypes = d_itypes; A.jtypes = d_jtypes; A.js = d_js; A.conf_of = d_conf_of;
+    A.volumes = d_volumes; A.gE = d_gE; A.gF = d_gF; A.gS = d_gS;
+    A.gxi_part = ctx->d_gxi; A.gc_part = ctx->d_gc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipError_t e = launch_train_grad(ctx->dev, A, ctx->train_lds, ctx->d_part, ctx->d_xisum, ctx->d_tsum, d_grad, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "parameter-gradient launch");
+    return MTP_OK;
 }
 
 }  // extern "C"

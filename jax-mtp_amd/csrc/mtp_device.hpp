@@ -1,0 +1,139 @@
+// Device helpers shared by the evaluation and training kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mtp_plan.hpp"
+
+namespace mtp {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int wrap_species(int t, int Sp)
+{
+    if (t < 0) t += Sp;
+    if (t < 0) t = 0;
+    if (t >= Sp) t = Sp - 1;
+    return t;
+}
+
+// Radial basis of one slot (jax_backend.py:83-95, 108-160).  Writes f[mu]
+// and df[mu] = d f_mu / d r for mu < R.  Real is double or float.
+template <typename Real>
+__device__ __forceinline__ void radial_slot(const DevPlan& P, double r, int ti, int tk, double* f, double* df)
+{
+    const Real rmin = (Real)P.min_dist, rmax = (Real)P.max_dist;
+    const Real rr = (Real)r;
+    const Real range_inv = (Real)1 / (rmax - rmin);
+    const Real x = ((Real)2 * rr - (rmin + rmax)) * range_inv;
+    const Real dxdr = (Real)2 * range_inv;
+    const Real s = (rmax - rr) * (rmax - rr);
+    const Real ds = (Real)-2 * (rmax - rr);
+    const double* c = P.radial_coeffs + (size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R * P.N;
+    Real af[kMaxR], ad[kMaxR];
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m) {
+        af[m] = m < P.R ? (Real)c[m * P.N] : (Real)0;
+        ad[m] = 0;
+    }
+    if (P.N > 1) {
+#pragma unroll
+        for (int m = 0; m < kMaxR; ++m)
+            if (m < P.R) { af[m] += (Real)c[m * P.N + 1] * x; ad[m] += (Real)c[m * P.N + 1]; }
+        Real t2 = 1, t1 = x, d2 = 0, d1 = 1;
+        for (int n = 2; n < P.N; ++n) {
+            const Real t0 = (Real)2 * x * t1 - t2;
+            const Real d0 = (Real)2 * t1 + (Real)2 * x * d1 - d2;
+#pragma unroll
+            for (int m = 0; m < kMaxR; ++m)
+                if (m < P.R) { af[m] += (Real)c[m * P.N + n] * t0; ad[m] += (Real)c[m * P.N + n] * d0; }
+            t2 = t1; t1 = t0; d2 = d1; d1 = d0;
+        }
+    }
+    const Real sc = (Real)P.scaling;
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m) {
+        if (m < P.R) {
+            f[m] = (double)(sc * s * af[m]);
+            df[m] = (double)(sc * (ds * af[m] + s * ad[m] * dxdr));
+        }
+    }
+}
+
+// FP64 radial basis of one slot with the per-coefficient factors used by the
+// parameter gradient: a[n*stride] = d f_mu / d c_n = scaling * s * T_n and
+// b[n*stride] = d f'_mu / d c_n = scaling * (s' T_n + s T_n' dx/dr).
+__device__ __forceinline__ void radial_slot_coef(const DevPlan& P, double r, int ti, int tk, double* f, double* df,
+                                                 double* a, double* b, int stride)
+{
+    const double rmin = P.min_dist, rmax = P.max_dist;
+    const double range_inv = 1.0 / (rmax - rmin);
+    const double x = (2.0 * r - (rmin + rmax)) * range_inv;
+    const double dxdr = 2.0 * range_inv;
+    const double s = (rmax - r) * (rmax - r);
+    const double ds = -2.0 * (rmax - r);
+    const double sc = P.scaling;
+    const double* c = P.radial_coeffs + (size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R * P.N;
+    double af[kMaxR], ad[kMaxR];
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m) { af[m] = 0.0; ad[m] = 0.0; }
+    double t2 = 0.0, t1 = 0.0, d2 = 0.0, d1 = 0.0;
+    for (int n = 0; n < P.N; ++n) {
+        double t0, d0;
+        if (n == 0) { t0 = 1.0; d0 = 0.0; }
+        else if (n == 1) { t0 = x; d0 = 1.0; }
+        else { t0 = 2.0 * x * t1 - t2; d0 = 2.0 * t1 + 2.0 * x * d1 - d2; }
+        a[n * stride] = sc * s * t0;
+        b[n * stride] = sc * (ds * t0 + s * d0 * dxdr);
+#pragma unroll
+        for (int m = 0; m < kMaxR; ++m)
+            if (m < P.R) { af[m] += c[m * P.N + n] * t0; ad[m] += c[m * P.N + n] * d0; }
+        t2 = t1; t1 = t0; d2 = d1; d1 = d0;
+    }
+#pragma unroll
+    for (int m = 0; m < kMaxR; ++m)
+        if (m < P.R) { f[m] = sc * s * af[m]; df[m] = sc * (ds * af[m] + s * ad[m] * dxdr); }
+}
+
+// Canonical index of monomial (a, b, NU-a-b) inside a degree-NU block.
+__host__ __device__ constexpr int mono_idx(int NU, int a, int b) { return (NU - a) * (NU - a + 1) / 2 + (NU - a - b); }
+
+// H = sum_alpha d[alpha] grad_u u^alpha over one (mu, NU) block, written as a
+// sum over the degree NU-1 monomials (Euler: u . H = NU * sum d P).
+template <int NU>
+__device__ __forceinline__ void grad_block(const double* __restrict__ d, const double* px, const double* py,
+                                           const double* pz, double& Hx, double& Hy, double& Hz)
+{
+#pragma unroll
+    for (int a = NU - 1; a >= 0; --a) {
+#pragma unroll
+        for (int b = NU - 1 - a; b >= 0; --b) {
+            const int c = NU - 1 - a - b;
+            const double P = px[a] * py[b] * pz[c];
+            Hx += (double)(a + 1) * d[mono_idx(NU, a + 1, b)] * P;
+            Hy += (double)(b + 1) * d[mono_idx(NU, a, b + 1)] * P;
+            Hz += (double)(c + 1) * d[mono_idx(NU, a, b)] * P;
+        }
+    }
+}
+
+__device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, const double* px, const double* py,
+                                                    const double* pz, double& Hx, double& Hy, double& Hz)
+{
+    switch (nu) {
+        case 1: grad_block<1>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 2: grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 3: grad_block<3>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 4: grad_block<4>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 5: grad_block<5>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 6: grad_block<6>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 7: grad_block<7>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 8: grad_block<8>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 9: grad_block<9>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 10: grad_block<10>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 11: grad_block<11>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 12: grad_block<12>(d, px, py, pz, Hx, Hy, Hz); break;
+        default: break;
+    }
+}
+
+}  // namespace mtp

@@ -23,9 +23,10 @@ struct PackOut {
     int* atom_types; // [max_atoms]
 };
 
-// MinImage: callable (double& dx, double& dy, double& dz).
-template <typename MinImage>
-long long pack_full_list(int nlocal, const double* const* x, const int* type, const int* tag, int inum,
+// MinImage: callable (double& dx, double& dy, double& dz).  TagT is LAMMPS's
+// tagint (int, or int64_t in LAMMPS_BIGBIG builds).
+template <typename MinImage, typename TagT>
+long long pack_full_list(int nlocal, const double* const* x, const int* type, const TagT* tag, int inum,
                          const int* ilist, const int* numneigh, const int* const* firstneigh, double cutoff,
                          int max_atoms, int max_nbr, MinImage&& min_image, const PackOut& o,
                          std::vector<int>& tag_to_dense, long long* n_truncated)
@@ -34,8 +35,8 @@ long long pack_full_list(int nlocal, const double* const* x, const int* type, co
     const double cutsq = cutoff * cutoff;
     for (int i = 0; i < max_atoms; ++i) { o.atom_types[i] = -1; o.counts[i] = 0; }
     for (int i = 0; i < nlocal; ++i) o.atom_types[i] = type[i] - 1;
-    int max_tag = 0;
-    for (int i = 0; i < nlocal; ++i) max_tag = tag[i] > max_tag ? tag[i] : max_tag;
+    long long max_tag = 0;
+    for (int i = 0; i < nlocal; ++i) max_tag = (long long)tag[i] > max_tag ? (long long)tag[i] : max_tag;
     tag_to_dense.assign((size_t)max_tag + 1, -1);
     for (int i = 0; i < nlocal; ++i)
         if (tag[i] >= 0 && tag[i] <= max_tag) tag_to_dense[tag[i]] = i;
@@ -58,7 +59,7 @@ long long pack_full_list(int nlocal, const double* const* x, const int* type, co
                 if (cnt >= max_nbr) { ++dropped; continue; }
                 prow[3 * cnt] = dx; prow[3 * cnt + 1] = dy; prow[3 * cnt + 2] = dz;
                 int dense = nlocal;
-                const int jt = tag[j];
+                const long long jt = (long long)tag[j];
                 if (jt >= 0 && jt <= max_tag && tag_to_dense[jt] != -1) dense = tag_to_dense[jt];
                 nrow[cnt] = dense;
                 trow[cnt] = dense != nlocal ? type[j] - 1 : -1;

@@ -1,0 +1,257 @@
+"""Training path: packed batches of uneven configurations, the weighted
+energy/force/stress loss and its parameter gradient on the GPU.
+
+Reference semantics (motep_jax_train_function.py:67-100, 154-191):
+    L = w_E sum_c (E_c - E_c^ref)^2 + w_F sum_{c,a} |F_a - F_a^ref|^2
+        + w_S sum_c |sigma_c - sigma_c^ref|^2,   (w_E, w_F, w_S) = (1, 0.01, 0.001)
+summed over configurations without per-atom normalisation, differentiated
+with respect to {moment_coeffs, species_coeffs, radial_coeffs} (scaling is
+not trained).  The reference pads every configuration to a global
+(A_max, K_max) (motep_jax_train_import.py:219-295); here configurations are
+concatenated and described by a segment-offset table, so one launch covers
+the whole batch without padded atoms.
+
+Gradients come from a hand-written second backward (the C ABI's
+``mtp_batch_param_grad``): with the upstream gradients (gE per
+configuration, gF per atom, gS per configuration) it evaluates
+    J = sum_i [ gE_c(i) E_i + sum_k g_ik . Delta_ik ],
+    Delta_ik = gF_i - gF_{j_ik} + Lambda_c r_ik,  Lambda = (M + M^T) / (2 V_c)
+(M holds gS in the reference's Voigt order), whose parameter gradient equals
+that of the loss: a forward tangent of the moments along Delta followed by a
+reverse pass over moments and tangents.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from .neighbors import padded_neighbors
+
+LOSS_WEIGHTS = (1.0, 0.01, 0.001)
+
+
+@dataclass
+class PackedBatch:
+    conf_off: np.ndarray   # [C+1] atom offsets
+    itypes: np.ndarray     # [N]
+    js: np.ndarray         # [N, K] global row index; N marks an empty slot
+    rij: np.ndarray        # [N, K, 3]
+    jtypes: np.ndarray     # [N, K]
+    volumes: np.ndarray    # [C]
+    conf_of: np.ndarray    # [N]
+
+    @property
+    def nconf(self) -> int:
+        return len(self.volumes)
+
+    @property
+    def natoms(self) -> int:
+        return len(self.itypes)
+
+    @property
+    def nslots(self) -> int:
+        return self.js.shape[1]
+
+    def to_device(self):
+        import torch
+
+        dev = torch.device("cuda")
+        return {
+            "conf_off": torch.as_tensor(self.conf_off, dtype=torch.int32, device=dev),
+            "conf_of": torch.as_tensor(self.conf_of, dtype=torch.int32, device=dev),
+            "itypes": torch.as_tensor(self.itypes, dtype=torch.int32, device=dev),
+            "js": torch.as_tensor(self.js, dtype=torch.int32, device=dev),
+            "rij": torch.as_tensor(self.rij, dtype=torch.float64, device=dev),
+            "jtypes": torch.as_tensor(self.jtypes, dtype=torch.int32, device=dev),
+            "volumes": torch.as_tensor(self.volumes, dtype=torch.float64, device=dev),
+        }
+
+
+def pack_configs(configs, cutoff: float, max_nbr: int) -> PackedBatch:
+    """configs: iterable of (positions[n,3], types[n], cell[3]) (orthogonal periodic cells)."""
+    rows_it, rows_js, rows_r, rows_jt, vols, offs = [], [], [], [], [], [0]
+    for pos, types, cell in configs:
+        nb = padded_neighbors(pos, types, cell, cutoff, max_nbr)
+        n = len(pos)
+        rows_it.append(nb["itypes"]); rows_r.append(nb["rij"]); rows_jt.append(nb["jtypes"])
+        rows_js.append(nb["js"].astype(np.int64) + offs[-1])  # local sentinel n -> fixed below
+        vols.append(float(np.prod(cell)))
+        offs.append(offs[-1] + n)
+    N = offs[-1]
+    js = np.concatenate(rows_js)
+    jt = np.concatenate(rows_jt)
+    js[jt < 0] = N
+    conf_off = np.array(offs, dtype=np.int32)
+    conf_of = np.repeat(np.arange(len(vols), dtype=np.int32), np.diff(conf_off))
+    return PackedBatch(conf_off, np.concatenate(rows_it).astype(np.int32), js.astype(np.int32),
+                       np.concatenate(rows_r), jt.astype(np.int32), np.array(vols), conf_of)
+
+
+def random_cells(nconf: int, rng, natoms_range=(8, 256), density_range=(0.06, 0.09), species: int = 2,
+                 kinds=("fcc", "bcc", "random"), min_sep: float = 1.9):
+    """Seeded FCC / BCC / random-dense periodic cells of uneven atom counts."""
+    out = []
+    for c in range(nconf):
+        kind = kinds[c % len(kinds)]
+        n_target = int(rng.integers(natoms_range[0], natoms_range[1] + 1))
+        rho = float(rng.uniform(*density_range))
+        if kind in ("fcc", "bcc"):
+            per = 4 if kind == "fcc" else 2
+            m = max(1, int(round((n_target / per) ** (1.0 / 3.0))))
+            n = per * m ** 3
+            a = (per / rho) ** (1.0 / 3.0)
+            base = (np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) if kind == "fcc"
+                    else np.array([[0, 0, 0], [0.5, 0.5, 0.5]]))
+            g = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+            pos = (g[:, None] + base[None]).reshape(-1, 3) * a
+            pos = pos + rng.normal(0, 0.05 * a, pos.shape)
+            cell = np.full(3, m * a)
+        else:
+            n = n_target
+            L = (n / rho) ** (1.0 / 3.0)
+            cell = np.full(3, L)
+            pts = []
+            tries = 0
+            while len(pts) < n and tries < 200000:
+                tries += 1
+                p = rng.uniform(0, L, 3)
+                if pts:
+                    d = np.asarray(pts) - p
+                    d -= L * np.round(d / L)
+                    if np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep:
+                        continue
+                pts.append(p)
+            pos = np.array(pts)
+            n = len(pos)
+        types = rng.integers(0, species, n).astype(np.int32)
+        out.append((pos, types, cell))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# GPU calls
+# ---------------------------------------------------------------------------
+def batch_forward(pot, dbatch, stream=None):
+    """Per-configuration energies [C], forces [N,3] and stress [C,6] (reference Voigt order)."""
+    import torch
+
+    L = capi.lib()
+    N, K = dbatch["js"].shape
+    C = dbatch["volumes"].shape[0]
+    E = torch.empty(C, dtype=torch.float64, device="cuda")
+    F = torch.empty((N, 3), dtype=torch.float64, device="cuda")
+    S = torch.empty((C, 6), dtype=torch.float64, device="cuda")
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    capi.check(L.mtp_batch_forward(pot.ctx, N, K, C, dbatch["conf_off"].data_ptr(), dbatch["conf_of"].data_ptr(),
+                                   dbatch["itypes"].data_ptr(), dbatch["js"].data_ptr(), dbatch["rij"].data_ptr(),
+                                   dbatch["jtypes"].data_ptr(), dbatch["volumes"].data_ptr(), E.data_ptr(),
+                                   F.data_ptr(), S.data_ptr(), s), pot.ctx)
+    return E, F, S
+
+
+def batch_param_grad(pot, dbatch, gE, gF, gS, stream=None):
+    """d/dtheta of sum gE.E + sum gF.F + sum gS.S, theta = (moment, species, radial) coefficients."""
+    import torch
+
+    L = capi.lib()
+    N, K = dbatch["js"].shape
+    C = dbatch["volumes"].shape[0]
+    p = pot.params
+    n_theta = len(p.moment_coeffs) + p.species_count + int(np.prod(p.radial_coeffs.shape))
+    grad = torch.empty(n_theta, dtype=torch.float64, device="cuda")
+    gE = gE.contiguous().to(torch.float64)
+    gF = gF.contiguous().to(torch.float64)
+    gS = gS.contiguous().to(torch.float64)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    capi.check(L.mtp_batch_param_grad(pot.ctx, N, K, C, dbatch["conf_off"].data_ptr(), dbatch["conf_of"].data_ptr(),
+                                      dbatch["itypes"].data_ptr(), dbatch["js"].data_ptr(), dbatch["rij"].data_ptr(),
+                                      dbatch["jtypes"].data_ptr(), dbatch["volumes"].data_ptr(), gE.data_ptr(),
+                                      gF.data_ptr(), gS.data_ptr(), grad.data_ptr(), s), pot.ctx)
+    return grad
+
+
+def theta_of(params) -> np.ndarray:
+    return np.concatenate([np.asarray(params.moment_coeffs, float), np.asarray(params.species_coeffs, float),
+                           np.asarray(params.radial_coeffs, float).ravel()])
+
+
+def _split_theta(params, theta):
+    S = len(params.moment_coeffs)
+    sp = params.species_count
+    return theta[:S], theta[S:S + sp], theta[S + sp:]
+
+
+def make_function():
+    import torch
+
+    class MTPBatchFunction(torch.autograd.Function):
+        """(theta) -> (E[C], F[N,3], S[C,6]) with the hand-written parameter backward."""
+
+        @staticmethod
+        def forward(ctx, theta, pot, dbatch):
+            th = theta.detach().cpu().numpy()
+            m, s, r = _split_theta(pot.params, th)
+            pot.set_params(m, s, r)
+            E, F, S = batch_forward(pot, dbatch)
+            ctx.pot = pot
+            ctx.dbatch = dbatch
+            return E, F, S
+
+        @staticmethod
+        def backward(ctx, gE, gF, gS):
+            import torch as _t
+
+            C = ctx.dbatch["volumes"].shape[0]
+            N = ctx.dbatch["js"].shape[0]
+            if gE is None:
+                gE = _t.zeros(C, dtype=_t.float64, device="cuda")
+            if gF is None:
+                gF = _t.zeros((N, 3), dtype=_t.float64, device="cuda")
+            if gS is None:
+                gS = _t.zeros((C, 6), dtype=_t.float64, device="cuda")
+            return batch_param_grad(ctx.pot, ctx.dbatch, gE, gF, gS), None, None
+
+    return MTPBatchFunction
+
+
+def loss_fn(E, F, S, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
+    """motep_jax_train_function.py:67-74."""
+    wE, wF, wS = weights
+    return wE * ((E - E_ref) ** 2).sum() + wF * ((F - F_ref) ** 2).sum() + wS * ((S - S_ref) ** 2).sum()
+
+
+class Trainer:
+    """Full-batch training with torch.optim.LBFGS over the hand-written gradient.
+
+    Replaces the reference's optax L-BFGS loop (motep_jax_train_function.py:178-213); the optimizer itself is
+    PyTorch's.  The step (forward + hand-written backward) is a fixed launch sequence on one stream.
+    """
+
+    def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0):
+        import torch
+
+        self.pot = pot
+        self.dbatch = batch.to_device()
+        self.E_ref = torch.as_tensor(E_ref, dtype=torch.float64, device="cuda")
+        self.F_ref = torch.as_tensor(F_ref, dtype=torch.float64, device="cuda")
+        self.S_ref = torch.as_tensor(S_ref, dtype=torch.float64, device="cuda")
+        self.weights = weights
+        self.fn = make_function()
+        self.theta = torch.tensor(theta_of(pot.params), dtype=torch.float64, device="cuda", requires_grad=True)
+        self.opt = torch.optim.LBFGS([self.theta], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
+
+    def loss(self):
+        E, F, S = self.fn.apply(self.theta, self.pot, self.dbatch)
+        return loss_fn(E, F, S, self.E_ref, self.F_ref, self.S_ref, self.weights)
+
+    def step(self):
+        def closure():
+            self.opt.zero_grad()
+            L = self.loss()
+            L.backward()
+            return L
+
+        return float(self.opt.step(closure))

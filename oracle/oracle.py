@@ -431,6 +431,83 @@ def remap_coeffs(orc: Oracle) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# Training: packed-batch outputs and the exact parameter gradient
+# ---------------------------------------------------------------------------
+def batch_outputs(orc: Oracle, itypes, js, rij, jtypes, conf_off, volumes, nthreads=0):
+    """Per-configuration E[C], forces[N,3], stress[C,6] (reference Voigt), per-slot g."""
+    Ei, g = orc.compact(itypes, rij, jtypes, True, nthreads)
+    N = len(itypes)
+    F = Oracle.forces(g, js, N)
+    C = len(volumes)
+    E = np.array([Ei[conf_off[c]:conf_off[c + 1]].sum() for c in range(C)])
+    S = np.zeros((C, 6))
+    for c in range(C):
+        sl = slice(conf_off[c], conf_off[c + 1])
+        S[c], _ = Oracle.stress(rij[sl], g[sl], volumes[c])
+    return E, F, S, g
+
+
+def _alpha_delta(itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS):
+    """alpha_i and Delta_ik of J = sum alpha E + sum g . Delta (the loss's linearisation)."""
+    N = len(itypes)
+    conf_of = np.repeat(np.arange(len(volumes)), np.diff(conf_off))
+    alpha = np.asarray(gE)[conf_of]
+    gF = np.asarray(gF).reshape(N, 3)
+    valid_j = js < N
+    Delta = gF[:, None, :] - np.where(valid_j[..., None], gF[np.minimum(js, N - 1)], 0.0)
+    for c in range(len(volumes)):
+        s = np.asarray(gS)[c]
+        M = np.array([[s[0], s[5], s[4]], [0.0, s[1], s[3]], [0.0, 0.0, s[2]]])
+        Lam = (M + M.T) / (2.0 * volumes[c])
+        sl = slice(conf_off[c], conf_off[c + 1])
+        Delta[sl] += rij[sl] @ Lam.T
+    return alpha, Delta
+
+
+def param_grad_exact(orc: Oracle, itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS, h=0.5, nthreads=0):
+    """d/dtheta of sum gE.E + sum gF.F + sum gS.S, theta = (moment, species, radial) coefficients.
+
+    J(theta) = sum_i alpha_i E_i + sum_ik g_ik . Delta_ik has the same parameter gradient.  Along any single
+    coordinate J is a polynomial of degree <= 4 (every basis function is a product of at most four basic
+    moments, each linear in the coefficients), so the 5-point central stencil is exact up to rounding for any
+    step h; tests check the degree bound itself.
+    """
+    alpha, Delta = _alpha_delta(itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS)
+    k = orc._keep
+
+    def J():
+        Ei, g = orc.compact(itypes, rij, jtypes, True, nthreads)
+        return float(alpha @ Ei + np.sum(g * Delta))
+
+    grads = []
+    for arr in (k["moment"], k["species"], k["radial"].reshape(-1)):
+        for q in range(arr.size):
+            x0 = arr[q]
+            vals = []
+            for t in (-2, -1, 1, 2):
+                arr[q] = x0 + t * h
+                vals.append(J())
+            arr[q] = x0
+            grads.append((vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h))
+    return np.array(grads)
+
+
+def J_along(orc: Oracle, itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS, which, q, ts, nthreads=0):
+    """J at theta + t e_q for each t (degree checks)."""
+    alpha, Delta = _alpha_delta(itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS)
+    arr = {"moment": orc._keep["moment"], "species": orc._keep["species"],
+           "radial": orc._keep["radial"].reshape(-1)}[which]
+    x0 = arr[q]
+    out = []
+    for t in ts:
+        arr[q] = x0 + t
+        Ei, g = orc.compact(itypes, rij, jtypes, True, nthreads)
+        out.append(float(alpha @ Ei + np.sum(g * Delta)))
+    arr[q] = x0
+    return np.array(out)
+
+
+# ---------------------------------------------------------------------------
 # Neighbor packing restatement (pair_jax_mtp.cpp, zero_overhead.cpp)
 # ---------------------------------------------------------------------------
 def pack_full_list(x, type_, tag, ilist, neigh, nlocal, cutoff, prd, periodic, max_atoms, max_nbr):

@@ -1,0 +1,136 @@
+"""Training path: packed batches, the loss, and the hand-written parameter gradient.
+
+CPU part pins the exact parameter-gradient oracle (5-point stencil of the loss
+linearisation, exact because the linearisation is a polynomial of degree <= 4
+along each coordinate) against plain finite differences of the loss itself.
+GPU part compares the HIP batch forward and parameter backward with it
+(norm-wise relative error <= 1e-10) and runs a few optimizer steps.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_for
+
+
+def _batch(jm, nconf=4, seed=0, level=16, species=2, natoms=(8, 40)):
+    rng = np.random.default_rng(seed)
+    cells = jm.training.random_cells(nconf, rng, natoms_range=natoms, species=species)
+    batch = jm.training.pack_configs(cells, cutoff=5.0, max_nbr=96)
+    params = jm.configs.random_params(level, species, 8, 1.6, 5.0, rng)
+    return batch, params, rng
+
+
+def _oracle_outputs(O, orc, b):
+    return O.batch_outputs(orc, b.itypes, b.js, b.rij, b.jtypes, b.conf_off, b.volumes)
+
+
+def test_pack_configs_layout(jm):
+    b, p, rng = _batch(jm, nconf=5, seed=1)
+    n = np.diff(b.conf_off)
+    assert b.natoms == n.sum() and len(b.volumes) == 5
+    assert np.all(b.conf_of == np.repeat(np.arange(5), n))
+    # js point inside the own configuration or to the empty-slot marker N
+    for c in range(5):
+        rows = slice(b.conf_off[c], b.conf_off[c + 1])
+        j = b.js[rows][b.jtypes[rows] >= 0]
+        assert np.all((j >= b.conf_off[c]) & (j < b.conf_off[c + 1]))
+    assert np.all(b.js[b.jtypes < 0] == b.natoms)
+
+
+def test_linearisation_degree_at_most_4(jm, O):
+    b, p, rng = _batch(jm, nconf=2, seed=2, level=12)
+    orc = oracle_for(O, p)
+    C, N = len(b.volumes), b.natoms
+    gE, gF, gS = rng.normal(size=C), rng.normal(size=(N, 3)), rng.normal(size=(C, 6))
+    ts = np.arange(-3, 4) * 0.3
+    for which, q in [("radial", 5), ("radial", 17), ("moment", 3)]:
+        v = O.J_along(orc, b.itypes, b.js, b.rij, b.jtypes, b.conf_off, b.volumes, gE, gF, gS, which, q, ts)
+        d5 = np.diff(v, 5)
+        assert np.max(np.abs(d5)) <= 1e-9 * np.max(np.abs(v)), (which, q, d5)
+
+
+def test_exact_gradient_matches_loss_finite_differences(jm, O):
+    b, p, rng = _batch(jm, nconf=3, seed=3, level=10)
+    orc = oracle_for(O, p)
+    E, F, S, g = _oracle_outputs(O, orc, b)
+    E_ref = E + rng.normal(0, 0.1, E.shape)
+    F_ref = F + rng.normal(0, 0.1, F.shape)
+    S_ref = S + rng.normal(0, 0.01, S.shape)
+    wE, wF, wS = jm.training.LOSS_WEIGHTS
+
+    def loss():
+        E, F, S, _ = _oracle_outputs(O, orc, b)
+        return wE * np.sum((E - E_ref) ** 2) + wF * np.sum((F - F_ref) ** 2) + wS * np.sum((S - S_ref) ** 2)
+
+    gE, gF, gS = 2 * wE * (E - E_ref), 2 * wF * (F - F_ref), 2 * wS * (S - S_ref)
+    grad = O.param_grad_exact(orc, b.itypes, b.js, b.rij, b.jtypes, b.conf_off, b.volumes, gE, gF, gS)
+    nS, sp = len(p.moment_coeffs), p.species_count
+    assert grad.shape == (nS + sp + p.radial_coeffs.size,)
+    arrays = [orc._keep["moment"], orc._keep["species"], orc._keep["radial"].reshape(-1)]
+    offsets = [0, nS, nS + sp]
+    # the loss is a degree-8 polynomial in the radial coefficients with large energies, so a plain
+    # central difference needs a 4th-order stencil to get below 1e-6
+    h = 1e-6
+    for a, off in zip(arrays, offsets):
+        for q in list(range(0, a.size, max(1, a.size // 6)))[:6]:
+            x0 = a[q]
+            v = []
+            for t in (-2, -1, 1, 2):
+                a[q] = x0 + t * h
+                v.append(loss())
+            a[q] = x0
+            fd = (v[0] - 8 * v[1] + 8 * v[2] - v[3]) / (12 * h)
+            assert grad[off + q] == pytest.approx(fd, rel=1e-6, abs=1e-7 * np.max(np.abs(grad)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level,species", [(10, 1), (16, 2), (20, 2)])
+def test_gpu_batch_forward_and_param_grad(jm, O, level, species):
+    import torch
+
+    b, p, rng = _batch(jm, nconf=5, seed=level, level=level, species=species, natoms=(8, 48))
+    print(f"[case] training level={level} species={species} natoms={b.natoms}", flush=True)
+    orc = oracle_for(O, p)
+    E, F, S, g = _oracle_outputs(O, orc, b)
+    pot = jm.MTPPotential(p, max_atoms=b.natoms, max_nbr=b.nslots)
+    db = b.to_device()
+    Eg, Fg, Sg = jm.training.batch_forward(pot, db)
+    torch.cuda.synchronize()
+    rel = lambda a, r: float(np.max(np.abs(a - r)) / np.max(np.abs(r)))
+    assert rel(Eg.cpu().numpy(), E) <= 1e-10
+    assert rel(Fg.cpu().numpy(), F) <= 1e-10
+    assert rel(Sg.cpu().numpy(), S) <= 1e-10
+    C, N = len(b.volumes), b.natoms
+    gE, gF, gS = rng.normal(size=C), rng.normal(size=(N, 3)), rng.normal(size=(C, 6))
+    ref = O.param_grad_exact(orc, b.itypes, b.js, b.rij, b.jtypes, b.conf_off, b.volumes, gE, gF, gS)
+    dev = lambda x: torch.as_tensor(x, dtype=torch.float64, device="cuda")
+    grad = jm.training.batch_param_grad(pot, db, dev(gE), dev(gF), dev(gS)).cpu().numpy()
+    nS, sp = len(p.moment_coeffs), p.species_count
+    for name, sl in [("moment", slice(0, nS)), ("species", slice(nS, nS + sp)), ("radial", slice(nS + sp, None))]:
+        err = rel(grad[sl], ref[sl])
+        print(f"  grad {name}: rel err {err:.3e}", flush=True)
+        assert err <= 1e-10, name
+    pot.close()
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_loss(jm):
+    """Labels from a seeded higher-level teacher; a few L-BFGS steps on the hand-written gradient."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    cells = jm.training.random_cells(8, rng, natoms_range=(8, 32), species=2)
+    b = jm.training.pack_configs(cells, cutoff=5.0, max_nbr=96)
+    teacher = jm.configs.random_params(12, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    tp = jm.MTPPotential(teacher, max_atoms=b.natoms, max_nbr=b.nslots)
+    E_ref, F_ref, S_ref = [t.clone() for t in jm.training.batch_forward(tp, b.to_device())]
+    tp.close()
+    student = jm.configs.random_params(10, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    pot = jm.MTPPotential(student, max_atoms=b.natoms, max_nbr=b.nslots)
+    tr = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref)
+    l0 = float(tr.loss())
+    losses = [tr.step() for _ in range(8)]
+    l1 = float(tr.loss())
+    print(f"loss {l0:.6e} -> {l1:.6e}", flush=True)
+    assert l1 < 0.5 * l0
+    pot.close()

@@ -55,12 +55,28 @@ class MTPError(RuntimeError):
         self.code = code
 
 
+def _load_torch_hip_runtime() -> None:
+    """Load PyTorch (and with it its HIP runtime) before the library.
+
+    PyTorch-ROCm ships its own libamdhip64 (SONAME libamdhip64.so.7).  If the
+    library were loaded first, the dynamic linker would bind it to the system
+    ROCm copy and the process would end up with two HIP runtimes, after which
+    PyTorch no longer sees the GPU.  Loading PyTorch first makes the library's
+    libamdhip64.so.7 dependency resolve to the runtime PyTorch already uses.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python jax-mtp_amd/build.py` "
                                "(the HIP extension is required; there is no fallback)")
+        _load_torch_hip_runtime()
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -12,6 +12,13 @@ import pytest
 from conftest import ROOT
 
 
+def test_single_hip_runtime_in_process(jm):
+    """The library binds to PyTorch's HIP runtime; two runtimes in one process hide the GPU from PyTorch."""
+    jm.capi.lib()
+    paths = {line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line}
+    assert len(paths) == 1, paths
+
+
 def test_library_exports_every_declared_symbol(jm):
     header = (ROOT / "include" / "mtp_hip.h").read_text()
     declared = set(re.findall(r"\b(mtp_[a-z_0-9]+)\s*\(", header))

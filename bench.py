@@ -124,6 +124,8 @@ def main():
     A, K = js.shape
     pot = jm.MTPPotential(w.params, max_atoms=A, max_nbr=K)
     info = pot.info()
+    best, tune_ms = pot.autotune(it, js, rij, jt, reps=3)
+    log(f"[bench] autotune: variant {best} ({pot.VARIANTS[best]}); ms per variant {tune_ms}")
     L = jm.capi.lib()
     out = torch.empty(14, dtype=torch.float64, device=dev)
     F = torch.empty((A, 3), dtype=torch.float64, device=dev)
@@ -195,6 +197,8 @@ def main():
                      "kernel": "atom_kernel<double>", "kernel_ms": r["atom_kernel_ms"],
                      "algorithmic_flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(nb["counts"][: w.natoms])},
+        "autotune": {"chosen": best, "kernel": pot.VARIANTS[best],
+                     "ms_per_step_by_variant": {pot.VARIANTS[v]: tune_ms[v] for v in range(3) if tune_ms[v] >= 0}},
         "radial_fp32": {"value": w.natoms / (results["radial_fp32"]["ms_per_step"] * 1e-3),
                         "ms_per_step": results["radial_fp32"]["ms_per_step"],
                         "atom_kernel_ms": results["radial_fp32"]["atom_kernel_ms"]},

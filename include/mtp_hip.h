@@ -99,6 +99,18 @@ const double *mtp_last_slot_gradients(const mtp_ctx *ctx);
 /* Wait for work enqueued on stream (NULL = the context's stream). */
 int mtp_sync(mtp_ctx *ctx, void *stream);
 
+/* Atom-kernel variants (no reference counterpart; XLA chose its own
+ * fusion): 0 = one thread per target moment, 256 threads per atom;
+ * 1 / 2 = balanced work items with 256 / 512 threads per atom.
+ * mtp_open picks the widest variant whose LDS fits (env MTP_KERNEL_VARIANT
+ * overrides).  mtp_autotune times every available variant on the given
+ * device arrays (reps launches each, after one warm-up), selects the
+ * fastest for ctx and reports the per-variant ms (-1 = unavailable). */
+int mtp_set_variant(mtp_ctx *ctx, int variant);
+int mtp_get_variant(const mtp_ctx *ctx, int *variant, long long *lds_bytes);
+int mtp_autotune(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js, const double *d_rijs,
+                 const int *d_jtypes, int reps, int *best, double *ms_per_variant);
+
 /* Live per-kernel timing (no reference counterpart; the reference only
  * timed whole calls, compile_cuda.py:252-275).  While enabled, HIP events
  * are recorded on the launch stream around every atom-kernel launch;

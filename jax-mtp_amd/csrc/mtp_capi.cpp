@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -44,7 +45,10 @@ T* dalloc(size_t n, hipError_t& e)
 
 struct mtp_ctx {
     HostPlan plan;
-    DevPlan dev{};
+    DevPlan dev{};                        // variant 0 (and the base of the others)
+    DevPlan vdev[kNumVariants]{};         // per-variant device plans
+    size_t vlds[kNumVariants] = {0, 0, 0};  // LDS bytes per variant, 0 = unavailable
+    int variant = kVariantV1;
     int max_atoms = 0, max_nbr = 0, debug_level = 0;
     double cutoff = 0.0;
     int radial_fp32 = 0;
@@ -124,6 +128,7 @@ int upload_params(mtp_ctx* c)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "parameter upload");
     c->dev.scaling = f.scaling;
+    for (auto& v : c->vdev) v.scaling = f.scaling;
     return MTP_OK;
 }
 
@@ -166,7 +171,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
         delete c;
         return MTP_ERR_UNSUPPORTED;
     }
-    e = set_atom_kernel_lds(c->lds_bytes);
+    e = set_atom_kernel_lds(kVariantV1, c->lds_bytes);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);  // default priority
     // plan buffers
     auto* mono = c->alloc<I4>(p.mono.size(), e);
@@ -236,6 +241,82 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.n_parts = p.n_parts;
     for (int q = 0; q < 9; ++q) d.part_beg[q] = p.part_beg[q];
     d.part_blk = part_blk;
+
+    // --- kernel variants ------------------------------------------------
+    c->vdev[kVariantV1] = d;
+    c->vlds[kVariantV1] = c->lds_bytes;
+    {
+        HostPlan p8;
+        std::string e8;
+        build_plan(f, 8, p8, e8);  // same validated file: cannot fail
+        I2* part_blk8 = c->alloc<I2>(p8.part_blk.size(), e);
+        if (e == hipSuccess) e = upload(part_blk8, p8.part_blk, c->stream);
+        for (int v : {kVariantV2x256, kVariantV2x512}) {
+            const int nt = variant_threads(v);
+            const HostPlan& pv = v == kVariantV2x512 ? p8 : p;
+            ItemPlan ip;
+            build_items(pv, nt, ip);
+            DevPlan dv = d;
+            if (v == kVariantV2x512) {
+                dv.n_parts = p8.n_parts;
+                for (int q = 0; q < 9; ++q) dv.part_beg[q] = p8.part_beg[q];
+                dv.part_blk = part_blk8;
+            }
+            auto pack = [&](const std::vector<LayerItems>& layers, const std::vector<U2>& table, int* it_beg,
+                            int* cb_beg, int* tbl_base, int* bo_beg, const I4** d_items, const I4** d_comb,
+                            const U2** d_tbl, const int** d_bo) {
+                std::vector<I4> items, comb;
+                std::vector<int> bo;
+                for (int q = 0; q < kMaxDepth + 2; ++q) { it_beg[q] = 0; cb_beg[q] = 0; if (bo_beg) bo_beg[q] = 0; }
+                for (size_t L = 0; L < layers.size(); ++L) {
+                    it_beg[L] = (int)items.size();
+                    cb_beg[L] = (int)comb.size();
+                    if (bo_beg) bo_beg[L] = (int)bo.size();
+                    tbl_base[L] = layers[L].table_base;
+                    items.insert(items.end(), layers[L].items.begin(), layers[L].items.end());
+                    comb.insert(comb.end(), layers[L].comb.begin(), layers[L].comb.end());
+                    bo.insert(bo.end(), layers[L].base_only.begin(), layers[L].base_only.end());
+                }
+                for (size_t L = layers.size(); L < kMaxDepth + 2; ++L) {
+                    it_beg[L] = (int)items.size();
+                    cb_beg[L] = (int)comb.size();
+                    if (bo_beg) bo_beg[L] = (int)bo.size();
+                }
+                I4* di = c->alloc<I4>(std::max<size_t>(1, items.size()), e);
+                I4* dc = c->alloc<I4>(std::max<size_t>(1, comb.size()), e);
+                U2* dt = c->alloc<U2>(std::max<size_t>(1, table.size()), e);
+                int* db = c->alloc<int>(std::max<size_t>(1, bo.size()), e);
+                if (e == hipSuccess) e = upload(di, items, c->stream);
+                if (e == hipSuccess) e = upload(dc, comb, c->stream);
+                if (e == hipSuccess) e = upload(dt, table, c->stream);
+                if (e == hipSuccess) e = upload(db, bo, c->stream);
+                *d_items = di; *d_comb = dc; *d_tbl = dt;
+                if (d_bo) *d_bo = db;
+            };
+            pack(ip.fwd, ip.fwd_table, dv.fwd_it_beg, dv.fwd_cb_beg, dv.fwd_tbl_base, nullptr, &dv.fwd_items,
+                 &dv.fwd_comb, &dv.fwd_tbl, nullptr);
+            pack(ip.rev, ip.rev_table, dv.rev_it_beg, dv.rev_cb_beg, dv.rev_tbl_base, dv.rev_bo_beg, &dv.rev_items,
+                 &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo);
+            dv.max_partials = ip.max_partials;
+            const size_t lds = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
+                                               dv.n_parts, nt, ip.max_partials);
+            if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
+                c->vdev[v] = dv;
+                c->vlds[v] = lds;
+            }
+        }
+        if (e != hipSuccess) {
+            set_global_error(std::string("device allocation/upload failed: ") + hipGetErrorString(e));
+            delete c;
+            return MTP_ERR_HIP;
+        }
+        // default: the widest balanced variant that fits; MTP_KERNEL_VARIANT overrides
+        c->variant = c->vlds[kVariantV2x512] ? kVariantV2x512 : (c->vlds[kVariantV2x256] ? kVariantV2x256 : kVariantV1);
+        if (const char* ev = std::getenv("MTP_KERNEL_VARIANT")) {
+            const int v = std::atoi(ev);
+            if (v >= 0 && v < kNumVariants && c->vlds[v]) c->variant = v;
+        }
+    }
     int rc = upload_params(c);
     if (rc) {
         set_global_error(c->err);
@@ -351,8 +432,74 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
         ee = ctx->ev_pool[ctx->ev_used].second;
         ++ctx->ev_used;
     }
-    hipError_t e = launch_forces(ctx->dev, A, F, ctx->lds_bytes, ctx->radial_fp32 != 0, s, eb, ee);
+    hipError_t e = launch_forces(ctx->vdev[ctx->variant], A, F, ctx->vlds[ctx->variant], ctx->radial_fp32 != 0, s,
+                                 eb, ee, ctx->variant);
     if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
+    return MTP_OK;
+}
+
+int mtp_set_variant(mtp_ctx* ctx, int variant)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (variant < 0 || variant >= kNumVariants || !ctx->vlds[variant])
+        return ctx->fail(MTP_ERR_UNSUPPORTED, "kernel variant " + std::to_string(variant) + " is not available");
+    ctx->variant = variant;
+    return MTP_OK;
+}
+
+int mtp_get_variant(const mtp_ctx* ctx, int* variant, long long* lds_bytes)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (variant) *variant = ctx->variant;
+    if (lds_bytes) *lds_bytes = (long long)ctx->vlds[ctx->variant];
+    return MTP_OK;
+}
+
+int mtp_autotune(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const int* d_js, const double* d_rijs,
+                 const int* d_jtypes, int reps, int* best, double* ms_per_variant)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (reps < 1) reps = 1;
+    if (nrows <= 0 || nrows > ctx->max_atoms || nslots <= 0 || nslots > ctx->max_nbr)
+        return ctx->fail(MTP_ERR_CAPACITY, "autotune shape outside the context's capacity");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return ctx->fail(MTP_ERR_HIP, "event creation failed");
+    AtomArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.rij = d_rijs; A.itypes = d_itypes; A.jtypes = d_jtypes; A.js = d_js;
+    A.Ei = ctx->d_Ei; A.g = ctx->d_g; A.Wi = ctx->d_Wi;
+    ForceArgs F{};
+    F.n_e = nrows; F.n_f = nrows; F.cell_rank = 3; F.volume = 1.0;
+    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = ctx->d_F; F.out = ctx->d_out;
+    int chosen = ctx->variant;
+    double best_ms = 1e300;
+    for (int v = 0; v < kNumVariants; ++v) {
+        double ms_v = -1.0;
+        if (ctx->vlds[v]) {
+            hipError_t e = launch_forces(ctx->vdev[v], A, F, ctx->vlds[v], ctx->radial_fp32 != 0, ctx->stream,
+                                         nullptr, nullptr, v);
+            if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+            for (int r = 0; r < reps && e == hipSuccess; ++r)
+                e = launch_forces(ctx->vdev[v], A, F, ctx->vlds[v], ctx->radial_fp32 != 0, ctx->stream, nullptr,
+                                  nullptr, v);
+            if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                hipEventDestroy(e0);
+                hipEventDestroy(e1);
+                return ctx->hip_fail(e, "autotune");
+            }
+            ms_v = ms / reps;
+            if (ms_v < best_ms) { best_ms = ms_v; chosen = v; }
+        }
+        if (ms_per_variant) ms_per_variant[v] = ms_v;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->variant = chosen;
+    if (best) *best = chosen;
     return MTP_OK;
 }
 

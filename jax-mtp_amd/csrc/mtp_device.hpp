@@ -113,12 +113,26 @@ __device__ __forceinline__ void grad_block(const double* __restrict__ d, const d
             Hy += (double)(b + 1) * d[mono_idx(NU, a, b + 1)] * P;
             Hz += (double)(c + 1) * d[mono_idx(NU, a, b)] * P;
         }
+        // one row of monomials at a time: keeps the LDS loads of the fully
+        // unrolled block from being hoisted together (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-__device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, const double* px, const double* py,
-                                                    const double* pz, double& Hx, double& Hy, double& Hz)
+// Called once per (mu, nu) block inside a loop over blocks.  The monomial
+// products only depend on the slot, so the compiler would hoist all of them
+// (every order 1..12) out of that loop and spill; the empty asm marks the
+// power tables as rewritten each time, which keeps the products per block.
+__device__ __forceinline__ void opaque_powers(double* px, double* py, double* pz)
 {
+#pragma unroll
+    for (int p = 0; p <= kMaxNu; ++p) asm volatile("" : "+v"(px[p]), "+v"(py[p]), "+v"(pz[p]));
+}
+
+__device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, double* px, double* py, double* pz,
+                                                    double& Hx, double& Hy, double& Hz)
+{
+    opaque_powers(px, py, pz);
     switch (nu) {
         case 1: grad_block<1>(d, px, py, pz, Hx, Hy, Hz); break;
         case 2: grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;

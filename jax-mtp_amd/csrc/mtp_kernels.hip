@@ -260,6 +260,275 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 }
 
 // ---------------------------------------------------------------------------
+// atom_kernel_v2: NT threads per atom and balanced contraction layers.
+//
+// Each layer is a list of host-planned work items of about (entries / NT)
+// entries, one per thread: runs of whole target moments, or chunks of one
+// long target whose partial sums are combined afterwards in a fixed order
+// (deterministic, no atomics).  Item tables are transposed so that the
+// threads of a wave read consecutive 8-byte entries (coalesced), and each
+// entry packs both operand indices and the multiplier.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int K, int R, int nu_max, int n_parts, int nt,
+                                                    int max_partials)
+{
+    AtomLds2 L;
+    int mlen = amc;
+    if (mlen < n_parts * K * 3) mlen = n_parts * K * 3;
+    int dlen = amc;
+    if (dlen < K * 3 * (nu_max + 1)) dlen = K * 3 * (nu_max + 1);
+    L.m = 0;
+    L.dm = L.m + mlen;
+    L.part = L.dm + dlen;
+    L.f = L.part + (max_partials > 0 ? max_partials : 1);
+    L.fp = L.f + R * K;
+    L.vx = L.fp + R * K;
+    L.vy = L.vx + K;
+    L.vz = L.vy + K;
+    L.r = L.vz + K;
+    L.red = L.r + K;
+    L.ints = L.red + nt;
+    L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
+    return L;
+}
+
+size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials)
+{
+    return sizeof(double) *
+           (size_t)atom_lds2_layout(amc, nslots, R, nu_max, n_parts, nt, max_partials).total_doubles;
+}
+
+template <int NT, bool REV>
+__device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* m, double* dm, double* part,
+                                                  int tid)
+{
+    const int i0 = REV ? P.rev_it_beg[L] : P.fwd_it_beg[L];
+    const int ni = (REV ? P.rev_it_beg[L + 1] : P.fwd_it_beg[L + 1]) - i0;
+    const I4* items = (REV ? P.rev_items : P.fwd_items) + i0;
+    const U2* tbl = (REV ? P.rev_tbl : P.fwd_tbl) + (REV ? P.rev_tbl_base[L] : P.fwd_tbl_base[L]);
+    const int* mom = REV ? P.rev_x : P.fwd_tgt;
+    double* dst = REV ? dm : m;
+    const double* second = REV ? dm : m;
+    for (int it = tid; it < ni; it += NT) {
+        const I4 w = items[it];
+        int mi = w.y;
+        int x = mom[mi];
+        double acc = (REV && w.z < 0) ? P.xi_of[x] : 0.0;
+        const U2* col = tbl + it;
+        for (int j = 0; j < w.x; ++j) {
+            const U2 q = col[(size_t)j * ni];
+            const float mf = __uint_as_float(q.y);
+            acc += (double)fabsf(mf) * m[q.x & 0xffffu] * second[q.x >> 16];
+            if (mf < 0.0f) {
+                if (w.z >= 0) part[w.z] = acc;
+                else dst[x] = acc;
+                if (j + 1 < w.x) {
+                    ++mi;
+                    x = mom[mi];
+                    acc = (REV && w.z < 0) ? P.xi_of[x] : 0.0;
+                }
+            }
+        }
+    }
+    if (REV)
+        for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
+            const int x = mom[P.rev_bo[t]];
+            dm[x] = P.xi_of[x];
+        }
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    if (c1 > c0) {
+        __syncthreads();
+        const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+        for (int c = c0 + tid; c < c1; c += NT) {
+            const I4 cb = comb[c];
+            const int x = mom[cb.x];
+            double acc = REV ? P.xi_of[x] : 0.0;
+            for (int p = cb.y; p < cb.z; ++p) acc += part[p];
+            dst[x] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename RadialReal, int NT>
+__global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    const int row = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int K = A.nslots;
+    constexpr int NW = NT / kWave;
+    const AtomLds2 L = atom_lds2_layout(P.amc, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials);
+    double* m = lds + L.m;
+    double* dm = lds + L.dm;
+    double* part = lds + L.part;
+    double* f = lds + L.f;
+    double* fp = lds + L.fp;
+    double* vx = lds + L.vx;
+    double* vy = lds + L.vy;
+    double* vz = lds + L.vz;
+    double* rr = lds + L.r;
+    double* red = lds + L.red;
+    int* ints = reinterpret_cast<int*>(lds + L.ints);
+    int* slot_of = ints;
+    int* cidx = ints + K;
+    int* wave_cnt = ints + 2 * K;
+    int* nvp = wave_cnt + NW;
+
+    const int ti = A.itypes[row];
+    const double* rij_row = A.rij + (size_t)row * K * 3;
+    const int* jt_row = A.jtypes + (size_t)row * K;
+
+    // ---- A: radial basis, compaction of valid slots in slot order (K <= NT)
+    {
+        const int k = tid;
+        double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        bool valid = false;
+        if (k < K) {
+            v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+        }
+        const uint64_t mask = __ballot(valid);
+        const int lane = tid & (kWave - 1), wave = tid / kWave;
+        if (lane == 0) wave_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+        const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+        if (k < K) cidx[k] = valid ? pos : -1;
+        if (valid) {
+            double fv[kMaxR], dfv[kMaxR];
+            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
+            vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
+            slot_of[pos] = k;
+        }
+        if (tid == 0) {
+            int tot = 0;
+            for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
+            *nvp = tot;
+        }
+        __syncthreads();
+    }
+    const int nv = *nvp;
+    for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;  // moments that are never targets stay 0
+
+    // ---- B: basic moments (power table in the dm region)
+    const int np1 = P.nu_max + 1;
+    double* pw = dm;
+    for (int t = tid; t < nv * 3; t += NT) {
+        const int k = t / 3, q = t - 3 * k;
+        const double inv = 1.0 / rr[k];
+        const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
+        double* row_pw = pw + (size_t)t * np1;
+        double v = 1.0;
+        row_pw[0] = 1.0;
+        for (int p = 1; p < np1; ++p) { v *= u; row_pw[p] = v; }
+    }
+    __syncthreads();
+    for (int t = tid; t < P.n_mono; t += NT) {
+        const I4 mo = P.mono[t];
+        const int nu = mo.w & 0xff, alpha = mo.w >> 8;
+        const int mumax = P.mu_max_of_nu[nu];
+        double acc[kMaxR];
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+        for (int k = 0; k < nv; ++k) {
+            const double* pk = pw + (size_t)k * 3 * np1;
+            const double Pm = pk[mo.x] * pk[np1 + mo.y] * pk[2 * np1 + mo.z];
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (mu <= mumax) acc[mu] += f[mu * K + k] * Pm;
+        }
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu)
+            if (mu <= mumax) m[P.blk_off[mu][nu] + alpha] = acc[mu];
+    }
+    __syncthreads();
+
+    // ---- C: contraction layers (depth 1 .. n_depth)
+    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, m, dm, part, tid);
+
+    // ---- D: site energy
+    {
+        double acc = 0.0;
+        for (int s = tid; s < P.S; s += NT) acc += P.moment_coeffs[s] * m[P.mapping[s]];
+        red[tid] = acc;
+        __syncthreads();
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
+    }
+
+    // ---- E: reverse layers, deepest first
+    for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, m, dm, part, tid);
+
+    // ---- F: per-slot gradient, thread = (slot, part)
+    double* gpart = m;
+    {
+        const int kk = tid & (kWave - 1), prt = tid / kWave;
+        for (int k = kk; k < nv && prt < P.n_parts; k += kWave) {
+            const double ir = 1.0 / rr[k];
+            const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
+            double px[kMaxNu + 1], py[kMaxNu + 1], pz[kMaxNu + 1];
+            px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+            for (int p = 1; p <= kMaxNu; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
+            double sdf = 0.0, sfn = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
+            const int b1 = P.part_beg[prt + 1];
+            for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
+                const I2 bl = P.part_blk[bi];
+                const int mu = bl.x, nu = bl.y;
+                const double* d = dm + P.blk_off[mu][nu];
+                const double fm = f[mu * K + k], fpm = fp[mu * K + k];
+                if (nu == 0) {
+                    sdf += fpm * d[0];
+                } else {
+                    double Hx = 0.0, Hy = 0.0, Hz = 0.0;
+                    grad_block_dispatch(nu, d, px, py, pz, Hx, Hy, Hz);
+                    const double uH = ux * Hx + uy * Hy + uz * Hz;
+                    sdf += fpm * uH * (1.0 / (double)nu);
+                    sfn += fm * uH;
+                    Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
+                }
+            }
+            double* gp = gpart + ((size_t)prt * K + k) * 3;
+            gp[0] = sdf * ux + (Gx - sfn * ux) * ir;
+            gp[1] = sdf * uy + (Gy - sfn * uy) * ir;
+            gp[2] = sdf * uz + (Gz - sfn * uz) * ir;
+        }
+    }
+    __syncthreads();
+    double* gk = dm;
+    for (int t = tid; t < nv * 3; t += NT) {
+        double s = 0.0;
+        for (int prt = 0; prt < P.n_parts; ++prt) s += gpart[(size_t)prt * K * 3 + t];
+        gk[t] = s;
+    }
+    __syncthreads();
+    double* g_row = A.g + (size_t)row * K * 3;
+    for (int t = tid; t < K * 3; t += NT) {
+        const int k = t / 3, q = t - 3 * k;
+        const int c = cidx[k];
+        g_row[t] = c >= 0 ? gk[3 * c + q] : 0.0;
+    }
+    // ---- G: per-atom virial
+    if (tid < 9) {
+        const int a = tid / 3, b = tid - 3 * (tid / 3);
+        const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
+        double s = 0.0;
+        for (int k = 0; k < nv; ++k) s += va[k] * gk[3 * k + b];
+        A.Wi[(size_t)row * 9 + tid] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Newton-3 force gather (jax_backend.py:263-303) without FP64 atomics:
 // a reverse-neighbor CSR is built per call; each atom sums its incoming
 // slots in increasing slot order, so results are bitwise reproducible.
@@ -391,17 +660,35 @@ __global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const do
 }
 
 // ---------------------------------------------------------------------------
-hipError_t set_atom_kernel_lds(size_t bytes)
+hipError_t set_atom_kernel_lds(int variant, size_t bytes)
 {
-    hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)atom_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes);
+    const int b = (int)bytes;
+    switch (variant) {
+        case kVariantV2x256: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 256>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 256>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
+        case kVariantV2x512: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 512>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 512>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
+        default: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
+    }
 }
 
 hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
-                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end)
+                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end, int variant)
 {
     if (A.nrows <= 0) {
         hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, 0, 0, F.cell_rank, F.volume,
@@ -409,10 +696,21 @@ hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F
         return hipGetLastError();
     }
     if (ev_begin) hipEventRecord(ev_begin, stream);
-    if (radial_fp32)
-        hipLaunchKernelGGL(atom_kernel<float>, dim3(A.nrows), dim3(kBlock), lds_bytes, stream, P, A);
-    else
-        hipLaunchKernelGGL(atom_kernel<double>, dim3(A.nrows), dim3(kBlock), lds_bytes, stream, P, A);
+    const dim3 grid(A.nrows);
+    switch (variant) {
+        case kVariantV2x256:
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 256>), grid, dim3(256), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 256>), grid, dim3(256), lds_bytes, stream, P, A);
+            break;
+        case kVariantV2x512:
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 512>), grid, dim3(512), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 512>), grid, dim3(512), lds_bytes, stream, P, A);
+            break;
+        default:
+            if (radial_fp32) hipLaunchKernelGGL(atom_kernel<float>, grid, dim3(kBlock), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL(atom_kernel<double>, grid, dim3(kBlock), lds_bytes, stream, P, A);
+            break;
+    }
     if (ev_end) hipEventRecord(ev_end, stream);
     const int n = A.nrows;
     const long long slots = (long long)A.nrows * A.nslots;

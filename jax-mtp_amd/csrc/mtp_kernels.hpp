@@ -33,13 +33,27 @@ struct AtomLds {
     int m, dm, f, fp, vx, vy, vz, r, red, ints, total_doubles;
 };
 
+struct AtomLds2 {
+    int m, dm, part, f, fp, vx, vy, vz, r, red, ints, total_doubles;
+};
+
+// Atom-kernel variants (chosen per potential by the launch autotuner):
+// v1 = 256 threads, one thread per target moment; v2 = balanced work items
+// with 256 or 512 threads per atom.
+constexpr int kVariantV1 = 0;
+constexpr int kVariantV2x256 = 1;
+constexpr int kVariantV2x512 = 2;
+constexpr int kNumVariants = 3;
+constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : 256; }
+
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
-hipError_t set_atom_kernel_lds(size_t bytes);
+size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials);
+hipError_t set_atom_kernel_lds(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.
 hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
                          bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin = nullptr,
-                         hipEvent_t ev_end = nullptr);
+                         hipEvent_t ev_end = nullptr, int variant = kVariantV1);
 
 constexpr int kAtomBlock = 256;  // threads per atom workgroup; max slots per row
 

@@ -141,8 +141,115 @@ bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err)
     }
     for (int k = n_parts; k < 9; ++k) p.part_beg[k] = (int)p.part_blk.size();
 
+    // --- packed entries: (i1 | i2 << 16, mult as float bits) ---------------
+    p.fwd_pk.resize(p.fwd_pair.size());
+    for (size_t e = 0; e < p.fwd_pair.size(); ++e) {
+        float mf = (float)p.fwd_mult[e];
+        uint32_t bits;
+        std::memcpy(&bits, &mf, 4);
+        p.fwd_pk[e] = U2{p.fwd_pair[e], bits};
+    }
+    p.rev_pk.resize(p.rev_pair.size());
+    for (size_t e = 0; e < p.rev_pair.size(); ++e) {
+        float mf = (float)p.rev_mult[e];
+        uint32_t bits;
+        std::memcpy(&bits, &mf, 4);
+        p.rev_pk[e] = U2{p.rev_pair[e], bits};
+    }
+
     refresh_plan_params(p);
     return true;
+}
+
+// Split one layer (moment positions [l0, l1) with entry ranges beg[] into
+// the packed entry array pk) into work items of about `chunk` entries: runs of
+// whole moments, or one chunk of a long moment that writes a partial sum
+// (partials are combined afterwards in a fixed order).  Moments without
+// entries go to `base_only`.  The item tables are stored transposed, entry j
+// of item i at [j * n_items + i], so that the threads of a wave, one item
+// each, read consecutive addresses.  The last entry of each moment (or chunk)
+// carries a negative multiplier as an end marker.
+static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, int l0, int l1, int chunk,
+                        LayerItems& L, std::vector<U2>& table, int& npart)
+{
+    npart = 0;
+    std::vector<std::vector<U2>> lists;
+    auto flagged = [&](int e, bool last) {
+        U2 q = pk[e];
+        if (last) {
+            float mf;
+            std::memcpy(&mf, &q.y, 4);
+            mf = -mf;
+            std::memcpy(&q.y, &mf, 4);
+        }
+        return q;
+    };
+    int pos = l0;
+    while (pos < l1) {
+        const int n = beg[pos + 1] - beg[pos];
+        if (n == 0) { L.base_only.push_back(pos); ++pos; continue; }
+        if (n > chunk) {
+            const int k = (n + chunk - 1) / chunk;
+            const int c = (n + k - 1) / k;
+            const int p0 = npart;
+            for (int q = 0; q < k; ++q) {
+                const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
+                std::vector<U2> lst;
+                for (int e = e0; e < e1; ++e) lst.push_back(flagged(e, e == e1 - 1));
+                L.items.push_back(I4{(int)lst.size(), pos, npart++, 0});
+                lists.push_back(std::move(lst));
+            }
+            L.comb.push_back(I4{pos, p0, npart, 0});
+            ++pos;
+            continue;
+        }
+        const int start = pos;
+        int cost = 0;
+        std::vector<U2> lst;
+        while (pos < l1) {
+            const int m = beg[pos + 1] - beg[pos];
+            if (m == 0 || m > chunk || (pos > start && cost + m > chunk)) break;
+            for (int e = beg[pos]; e < beg[pos + 1]; ++e) lst.push_back(flagged(e, e == beg[pos + 1] - 1));
+            cost += m;
+            ++pos;
+        }
+        L.items.push_back(I4{(int)lst.size(), start, -1, 0});
+        lists.push_back(std::move(lst));
+    }
+    size_t maxlen = 0;
+    for (auto& l : lists) maxlen = std::max(maxlen, l.size());
+    L.table_base = (int)table.size();
+    const size_t ni = lists.size();
+    table.resize(table.size() + maxlen * ni, U2{0, 0});
+    for (size_t i = 0; i < ni; ++i)
+        for (size_t j = 0; j < lists[i].size(); ++j) table[L.table_base + j * ni + i] = lists[i][j];
+}
+
+void build_items(const HostPlan& p, int nt, ItemPlan& ip)
+{
+    ip = ItemPlan();
+    ip.nt = nt;
+    ip.max_partials = 0;
+    for (int d = 1; d <= p.n_depth; ++d) {
+        LayerItems L;
+        const int l0 = p.fwd_layer[d], l1 = p.fwd_layer[d + 1];
+        const int ent = p.fwd_beg[l1] - p.fwd_beg[l0];
+        const int chunk = std::max(4, (ent + nt - 1) / nt);
+        int np = 0;
+        split_layer(p.fwd_beg, p.fwd_pk, l0, l1, chunk, L, ip.fwd_table, np);
+        ip.max_partials = std::max(ip.max_partials, np);
+        ip.fwd.push_back(std::move(L));
+    }
+    for (int li = 0; li <= p.n_depth; ++li) {
+        LayerItems L;
+        const int l0 = p.rev_layer[li], l1 = p.rev_layer[li + 1];
+        const int ent = p.rev_beg[l1] - p.rev_beg[l0];
+        const int chunk = std::max(4, (ent + nt - 1) / nt);
+        int np = 0;
+        split_layer(p.rev_beg, p.rev_pk, l0, l1, chunk, L, ip.rev_table, np);
+        ip.max_partials = std::max(ip.max_partials, np);
+        ip.rev.push_back(std::move(L));
+    }
 }
 
 }  // namespace mtp

@@ -28,6 +28,7 @@ namespace mtp {
 
 struct I4 { int x, y, z, w; };
 struct I2 { int x, y; };
+struct U2 { uint32_t x, y; };
 
 constexpr int kMaxR = 8;      // radial functions (mu) supported
 constexpr int kMaxNu = 12;    // tensor order supported
@@ -65,6 +66,35 @@ struct DevPlan {
     int n_parts;
     int part_beg[9];                 // prefix into part_blk
     const I2* part_blk;            // (mu, nu)
+    // balanced work items for the contraction layers (see build_items).
+    // Layer L (forward: depth L+1; reverse: rev_layer index L) owns items
+    // [it_beg[L], it_beg[L+1]), combines [cb_beg[L], cb_beg[L+1]), base-only
+    // moments [bo_beg[L], bo_beg[L+1]) and a transposed table at tbl_base[L].
+    int fwd_it_beg[kMaxDepth + 2], fwd_cb_beg[kMaxDepth + 2], fwd_tbl_base[kMaxDepth + 1];
+    int rev_it_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2], rev_tbl_base[kMaxDepth + 1];
+    int rev_bo_beg[kMaxDepth + 2];
+    const I4* fwd_items;             // (n entries, first moment position, partial slot or -1, -)
+    const I4* fwd_comb;              // (moment position, p0, p1, -)
+    const U2* fwd_tbl;               // (i1 | i2 << 16, +-mult float bits; negative = last of a moment)
+    const I4* rev_items;
+    const I4* rev_comb;
+    const U2* rev_tbl;               // (other | target << 16, +-mult)
+    const int* rev_bo;               // moment positions without entries (dm = xi)
+    int max_partials;
+};
+
+// Work items of one contraction layer for a workgroup of nt threads (see
+// split_layer in mtp_plan.cpp).
+struct LayerItems {
+    std::vector<I4> items, comb;
+    std::vector<int> base_only;
+    int table_base = 0;
+};
+
+struct ItemPlan {
+    int nt = 0, max_partials = 0;
+    std::vector<LayerItems> fwd, rev;   // fwd[d-1] = depth d; rev[li] = rev_layer li
+    std::vector<U2> fwd_table, rev_table;
 };
 
 struct HostPlan {
@@ -84,9 +114,12 @@ struct HostPlan {
     std::vector<double> xi_of;
     std::vector<I2> part_blk;
     std::vector<int> part_beg;
+    std::vector<U2> fwd_pk, rev_pk;
     int blk_off[kMaxR][kMaxNu + 1];
     int mu_max_of_nu[kMaxNu + 1];
 };
+
+void build_items(const HostPlan& p, int nt, ItemPlan& ip);
 
 // Build the plan from a validated file.  n_parts splits the per-neighbor
 // gradient work.  Returns false with a message for unsupported layouts.

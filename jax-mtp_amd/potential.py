@@ -70,6 +70,30 @@ class MTPPotential:
         self.radial_fp32 = bool(radial_fp32)
         capi.check(capi.lib().mtp_set_precision(self.ctx, int(self.radial_fp32)), self.ctx)
 
+    VARIANTS = {0: "per-target, 256 threads", 1: "balanced items, 256 threads", 2: "balanced items, 512 threads"}
+
+    @property
+    def variant(self) -> int:
+        v = ctypes.c_int()
+        capi.check(capi.lib().mtp_get_variant(self.ctx, ctypes.byref(v), None), self.ctx)
+        return v.value
+
+    def set_variant(self, variant: int):
+        capi.check(capi.lib().mtp_set_variant(self.ctx, int(variant)), self.ctx)
+
+    def autotune(self, itypes, all_js, all_rijs, all_jtypes, reps: int = 3):
+        """Time every kernel variant on these (device) arrays and keep the fastest; returns (best, ms list)."""
+        import torch
+
+        dev = torch.device("cuda")
+        it, js, rij, jt = _i32(itypes, dev), _i32(all_js, dev), _f64(all_rijs, dev), _i32(all_jtypes, dev)
+        A, K = js.shape
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * 3)()
+        capi.check(capi.lib().mtp_autotune(self.ctx, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(),
+                                           jt.data_ptr(), int(reps), ctypes.byref(best), ms), self.ctx)
+        return best.value, list(ms)
+
     def info(self) -> dict:
         v = [ctypes.c_int() for _ in range(8)]
         capi.check(capi.lib().mtp_info(self.ctx, *[ctypes.byref(x) for x in v]), self.ctx)

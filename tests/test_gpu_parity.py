@@ -21,9 +21,11 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / den) if den > 0 else float(np.max(np.abs(a - b)))
 
 
-def _run_gpu(jm, w, nb, radial_fp32=False, n_e=None, n_f=None, rows=None):
+def _run_gpu(jm, w, nb, radial_fp32=False, n_e=None, n_f=None, rows=None, variant=None):
     A = nb["js"].shape[0]
     pot = jm.MTPPotential(w.params, max_atoms=A, max_nbr=nb["js"].shape[1], radial_fp32=radial_fp32)
+    if variant is not None:
+        pot.set_variant(variant)
     n_e = w.natoms if n_e is None else n_e
     n_f = w.natoms if n_f is None else n_f
     E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
@@ -54,12 +56,13 @@ def _check(gpu, ref, tol=TOL):
     assert _rel(gpu["s6"], ref["s6"]) <= tol
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("level", [2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24])
-def test_levels_random_cells(jm, O, level):
+def test_levels_random_cells(jm, O, level, variant):
     w = jm.configs.small_cluster(level, 3, 24, seed=100 + level, cell=8.0, cutoff=5.0, max_nbr=80)
     nb = w.neighbors()
-    print(f"[case] level={level} natoms={w.natoms} maxn={nb['counts'].max()}", flush=True)
-    _check(_run_gpu(jm, w, nb), _run_oracle(O, w, nb))
+    print(f"[case] level={level} variant={variant} natoms={w.natoms} maxn={nb['counts'].max()}", flush=True)
+    _check(_run_gpu(jm, w, nb, variant=variant), _run_oracle(O, w, nb))
 
 
 def test_fcc256_level10(jm, O):
@@ -87,15 +90,25 @@ def test_masks_and_padding_rows(jm, O):
     _check(_run_gpu(jm, w, nb, n_e=17, n_f=15), _run_oracle(O, w, nb, n_e=17, n_f=15))
 
 
-def test_fcc32k_level24_full_size(jm, O):
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_fcc32k_level24_full_size(jm, O, variant):
     """The 32,000-atom level-24 workload at full size against the oracle."""
     w = jm.configs.fcc32k_level24(seed=0)
     nb = w.neighbors()
-    print(f"[case] fcc32k level24 maxn={nb['counts'].max()}", flush=True)
-    gpu = _run_gpu(jm, w, nb)
-    ref = _run_oracle(O, w, nb)
+    print(f"[case] fcc32k level24 variant={variant} maxn={nb['counts'].max()}", flush=True)
+    gpu = _run_gpu(jm, w, nb, variant=variant)
+    ref = _fcc32k_reference(O, w, nb)
     _check(gpu, ref)
     assert np.abs(gpu["F"].sum(axis=0)).max() <= 1e-9 * np.abs(gpu["F"]).max()
+
+
+_REF_CACHE = {}
+
+
+def _fcc32k_reference(O, w, nb):
+    if "fcc32k" not in _REF_CACHE:
+        _REF_CACHE["fcc32k"] = _run_oracle(O, w, nb)
+    return _REF_CACHE["fcc32k"]
 
 
 def test_radial_fp32_variant_error(jm, O):

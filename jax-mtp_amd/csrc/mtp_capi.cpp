@@ -263,40 +263,45 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                 dv.part_blk = part_blk8;
             }
             auto pack = [&](const std::vector<LayerItems>& layers, const std::vector<U2>& table, int* it_beg,
-                            int* cb_beg, int* tbl_base, int* bo_beg, const I4** d_items, const I4** d_comb,
-                            const U2** d_tbl, const int** d_bo) {
+                            int* cb_beg, int* tbl_base, int* bo_beg, int* xa_beg, const I4** d_items,
+                            const I4** d_comb, const U2** d_tbl, const int** d_bo, const int** d_xa) {
                 std::vector<I4> items, comb;
-                std::vector<int> bo;
-                for (int q = 0; q < kMaxDepth + 2; ++q) { it_beg[q] = 0; cb_beg[q] = 0; if (bo_beg) bo_beg[q] = 0; }
-                for (size_t L = 0; L < layers.size(); ++L) {
+                std::vector<int> bo, xa;
+                for (int q = 0; q < kMaxDepth + 2; ++q) {
+                    it_beg[q] = 0; cb_beg[q] = 0;
+                    if (bo_beg) bo_beg[q] = 0;
+                    if (xa_beg) xa_beg[q] = 0;
+                }
+                for (size_t L = 0; L < kMaxDepth + 2; ++L) {
                     it_beg[L] = (int)items.size();
                     cb_beg[L] = (int)comb.size();
                     if (bo_beg) bo_beg[L] = (int)bo.size();
+                    if (xa_beg) xa_beg[L] = (int)xa.size();
+                    if (L >= layers.size()) continue;
                     tbl_base[L] = layers[L].table_base;
                     items.insert(items.end(), layers[L].items.begin(), layers[L].items.end());
                     comb.insert(comb.end(), layers[L].comb.begin(), layers[L].comb.end());
                     bo.insert(bo.end(), layers[L].base_only.begin(), layers[L].base_only.end());
-                }
-                for (size_t L = layers.size(); L < kMaxDepth + 2; ++L) {
-                    it_beg[L] = (int)items.size();
-                    cb_beg[L] = (int)comb.size();
-                    if (bo_beg) bo_beg[L] = (int)bo.size();
+                    xa.insert(xa.end(), layers[L].xi_add.begin(), layers[L].xi_add.end());
                 }
                 I4* di = c->alloc<I4>(std::max<size_t>(1, items.size()), e);
                 I4* dc = c->alloc<I4>(std::max<size_t>(1, comb.size()), e);
                 U2* dt = c->alloc<U2>(std::max<size_t>(1, table.size()), e);
                 int* db = c->alloc<int>(std::max<size_t>(1, bo.size()), e);
+                int* dx = c->alloc<int>(std::max<size_t>(1, xa.size()), e);
                 if (e == hipSuccess) e = upload(di, items, c->stream);
                 if (e == hipSuccess) e = upload(dc, comb, c->stream);
                 if (e == hipSuccess) e = upload(dt, table, c->stream);
                 if (e == hipSuccess) e = upload(db, bo, c->stream);
+                if (e == hipSuccess) e = upload(dx, xa, c->stream);
                 *d_items = di; *d_comb = dc; *d_tbl = dt;
                 if (d_bo) *d_bo = db;
+                if (d_xa) *d_xa = dx;
             };
-            pack(ip.fwd, ip.fwd_table, dv.fwd_it_beg, dv.fwd_cb_beg, dv.fwd_tbl_base, nullptr, &dv.fwd_items,
-                 &dv.fwd_comb, &dv.fwd_tbl, nullptr);
-            pack(ip.rev, ip.rev_table, dv.rev_it_beg, dv.rev_cb_beg, dv.rev_tbl_base, dv.rev_bo_beg, &dv.rev_items,
-                 &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo);
+            pack(ip.fwd, ip.fwd_table, dv.fwd_it_beg, dv.fwd_cb_beg, dv.fwd_tbl_base, nullptr, nullptr,
+                 &dv.fwd_items, &dv.fwd_comb, &dv.fwd_tbl, nullptr, nullptr);
+            pack(ip.rev, ip.rev_table, dv.rev_it_beg, dv.rev_cb_beg, dv.rev_tbl_base, dv.rev_bo_beg, dv.rev_xa_beg,
+                 &dv.rev_items, &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo, &dv.rev_xa);
             dv.max_partials = ip.max_partials;
             const size_t lds = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
                                                dv.n_parts, nt, ip.max_partials);

@@ -298,6 +298,15 @@ size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int 
            (size_t)atom_lds2_layout(amc, nslots, R, nu_max, n_parts, nt, max_partials).total_doubles;
 }
 
+// One contraction layer over its work items.  Each thread walks its item's
+// entries in batches of kUB, with the table loads of the next batch in
+// flight while the current batch is processed (the table is L2-resident and
+// would otherwise cost one full memory latency per entry).  Loads of a batch
+// are issued before its stores: a layer's stores never alias its operands
+// (targets have a higher depth than operands in the forward pass, a lower
+// depth than the targets they read in the reverse pass).
+constexpr int kUB = 8;
+
 template <int NT, bool REV>
 __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* m, double* dm, double* part,
                                                   int tid)
@@ -306,49 +315,69 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
     const int ni = (REV ? P.rev_it_beg[L + 1] : P.fwd_it_beg[L + 1]) - i0;
     const I4* items = (REV ? P.rev_items : P.fwd_items) + i0;
     const U2* tbl = (REV ? P.rev_tbl : P.fwd_tbl) + (REV ? P.rev_tbl_base[L] : P.fwd_tbl_base[L]);
-    const int* mom = REV ? P.rev_x : P.fwd_tgt;
     double* dst = REV ? dm : m;
     const double* second = REV ? dm : m;
     for (int it = tid; it < ni; it += NT) {
         const I4 w = items[it];
-        int mi = w.y;
-        int x = mom[mi];
-        double acc = (REV && w.z < 0) ? P.xi_of[x] : 0.0;
+        const int len = w.x;
+        double* pdst = w.z >= 0 ? part + w.z : dst - 1;  // dest + 1 is encoded in the entries
+        const bool chunk = w.z >= 0;
         const U2* col = tbl + it;
-        for (int j = 0; j < w.x; ++j) {
-            const U2 q = col[(size_t)j * ni];
-            const float mf = __uint_as_float(q.y);
-            acc += (double)fabsf(mf) * m[q.x & 0xffffu] * second[q.x >> 16];
-            if (mf < 0.0f) {
-                if (w.z >= 0) part[w.z] = acc;
-                else dst[x] = acc;
-                if (j + 1 < w.x) {
-                    ++mi;
-                    x = mom[mi];
-                    acc = (REV && w.z < 0) ? P.xi_of[x] : 0.0;
+        U2 nxt[kUB];
+#pragma unroll
+        for (int u = 0; u < kUB; ++u) nxt[u] = u < len ? col[(size_t)u * ni] : U2{0u, 0u};
+        double acc = 0.0;
+        for (int j0 = 0; j0 < len; j0 += kUB) {
+            U2 q[kUB];
+#pragma unroll
+            for (int u = 0; u < kUB; ++u) q[u] = nxt[u];
+#pragma unroll
+            for (int u = 0; u < kUB; ++u) {
+                const int j = j0 + kUB + u;
+                nxt[u] = j < len ? col[(size_t)j * ni] : U2{0u, 0u};
+            }
+            double a[kUB], b[kUB];
+#pragma unroll
+            for (int u = 0; u < kUB; ++u) {
+                a[u] = m[q[u].x & 0xffffu];
+                b[u] = second[q[u].x >> 16];
+            }
+#pragma unroll
+            for (int u = 0; u < kUB; ++u) {
+                if (j0 + u < len) {
+                    acc += (double)(q[u].y & 0xffffu) * a[u] * b[u];
+                    const uint32_t d = q[u].y >> 16;
+                    if (d) {
+                        pdst[chunk ? 0 : d] = acc;
+                        acc = 0.0;
+                    }
                 }
             }
         }
     }
-    if (REV)
+    __syncthreads();
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    const int* mom = REV ? P.rev_x : P.fwd_tgt;
+    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    for (int c = c0 + tid; c < c1; c += NT) {
+        const I4 cb = comb[c];
+        const int x = mom[cb.x];
+        double acc = REV ? P.xi_of[x] : 0.0;
+        for (int p = cb.y; p < cb.z; ++p) acc += part[p];
+        dst[x] = acc;
+    }
+    if (REV) {
         for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
             const int x = mom[P.rev_bo[t]];
             dm[x] = P.xi_of[x];
         }
-    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
-    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
-    if (c1 > c0) {
-        __syncthreads();
-        const I4* comb = REV ? P.rev_comb : P.fwd_comb;
-        for (int c = c0 + tid; c < c1; c += NT) {
-            const I4 cb = comb[c];
-            const int x = mom[cb.x];
-            double acc = REV ? P.xi_of[x] : 0.0;
-            for (int p = cb.y; p < cb.z; ++p) acc += part[p];
-            dst[x] = acc;
+        for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
+            const int x = P.rev_xa[t];
+            dm[x] += P.xi_of[x];
         }
     }
-    __syncthreads();
+    if (c1 > c0 || REV) __syncthreads();
 }
 
 template <typename RadialReal, int NT>
@@ -437,12 +466,18 @@ __global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A
         double acc[kMaxR];
 #pragma unroll
         for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+        const double* px = pw + mo.x;
+        const double* py = pw + np1 + mo.y;
+        const double* pz = pw + 2 * np1 + mo.z;
         for (int k = 0; k < nv; ++k) {
-            const double* pk = pw + (size_t)k * 3 * np1;
-            const double Pm = pk[mo.x] * pk[np1 + mo.y] * pk[2 * np1 + mo.z];
+            const int o = k * 3 * np1;
+            const double Pm = px[o] * py[o] * pz[o];
 #pragma unroll
             for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu <= mumax) acc[mu] += f[mu * K + k] * Pm;
+                if (mu < P.R) {  // uniform
+                    const double v = f[mu * K + k] * Pm;
+                    acc[mu] += mu <= mumax ? v : 0.0;  // per-thread mask as a select
+                }
         }
 #pragma unroll
         for (int mu = 0; mu < kMaxR; ++mu)

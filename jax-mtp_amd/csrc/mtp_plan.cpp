@@ -162,26 +162,26 @@ bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err)
 }
 
 // Split one layer (moment positions [l0, l1) with entry ranges beg[] into
-// the packed entry array pk) into work items of about `chunk` entries: runs of
-// whole moments, or one chunk of a long moment that writes a partial sum
-// (partials are combined afterwards in a fixed order).  Moments without
-// entries go to `base_only`.  The item tables are stored transposed, entry j
-// of item i at [j * n_items + i], so that the threads of a wave, one item
-// each, read consecutive addresses.  The last entry of each moment (or chunk)
-// carries a negative multiplier as an end marker.
-static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, int l0, int l1, int chunk,
-                        LayerItems& L, std::vector<U2>& table, int& npart)
+// the packed entry array pk, moment ids mom[]) into work items of about
+// `chunk` entries: runs of whole moments, or one chunk of a long moment that
+// writes a partial sum (partials are combined afterwards in a fixed order).
+// Moments without entries go to `base_only`; moments of run items that carry
+// a coefficient (`mapped`) go to `xi_add`.  The item tables are transposed,
+// entry j of item i at [j * n_items + i], so that the threads of a wave (one
+// item each) read consecutive addresses.  Entry word y holds the multiplier
+// in its low 16 bits; the last entry of a moment (or chunk) also holds the
+// destination moment + 1 (1 for a chunk) in its high 16 bits.
+static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, const std::vector<int>& mom,
+                        const std::vector<char>& mapped, int l0, int l1, int chunk, LayerItems& L,
+                        std::vector<U2>& table, int& npart)
 {
     npart = 0;
     std::vector<std::vector<U2>> lists;
-    auto flagged = [&](int e, bool last) {
+    auto entry = [&](int e, int dest_plus1) {
         U2 q = pk[e];
-        if (last) {
-            float mf;
-            std::memcpy(&mf, &q.y, 4);
-            mf = -mf;
-            std::memcpy(&q.y, &mf, 4);
-        }
+        float mf;
+        std::memcpy(&mf, &q.y, 4);
+        q.y = (uint32_t)mf | ((uint32_t)dest_plus1 << 16);
         return q;
     };
     int pos = l0;
@@ -195,7 +195,7 @@ static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, 
             for (int q = 0; q < k; ++q) {
                 const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
                 std::vector<U2> lst;
-                for (int e = e0; e < e1; ++e) lst.push_back(flagged(e, e == e1 - 1));
+                for (int e = e0; e < e1; ++e) lst.push_back(entry(e, e == e1 - 1 ? 1 : 0));
                 L.items.push_back(I4{(int)lst.size(), pos, npart++, 0});
                 lists.push_back(std::move(lst));
             }
@@ -209,7 +209,9 @@ static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, 
         while (pos < l1) {
             const int m = beg[pos + 1] - beg[pos];
             if (m == 0 || m > chunk || (pos > start && cost + m > chunk)) break;
-            for (int e = beg[pos]; e < beg[pos + 1]; ++e) lst.push_back(flagged(e, e == beg[pos + 1] - 1));
+            for (int e = beg[pos]; e < beg[pos + 1]; ++e)
+                lst.push_back(entry(e, e == beg[pos + 1] - 1 ? mom[pos] + 1 : 0));
+            if (mapped[mom[pos]]) L.xi_add.push_back(mom[pos]);
             cost += m;
             ++pos;
         }
@@ -230,13 +232,16 @@ void build_items(const HostPlan& p, int nt, ItemPlan& ip)
     ip = ItemPlan();
     ip.nt = nt;
     ip.max_partials = 0;
+    const int amc = p.file.alpha_moments_count;
+    std::vector<char> mapped(amc, 0), no_map(amc, 0);
+    for (int x : p.file.alpha_moment_mapping) mapped[x] = 1;
     for (int d = 1; d <= p.n_depth; ++d) {
         LayerItems L;
         const int l0 = p.fwd_layer[d], l1 = p.fwd_layer[d + 1];
         const int ent = p.fwd_beg[l1] - p.fwd_beg[l0];
         const int chunk = std::max(4, (ent + nt - 1) / nt);
         int np = 0;
-        split_layer(p.fwd_beg, p.fwd_pk, l0, l1, chunk, L, ip.fwd_table, np);
+        split_layer(p.fwd_beg, p.fwd_pk, p.fwd_tgt, no_map, l0, l1, chunk, L, ip.fwd_table, np);
         ip.max_partials = std::max(ip.max_partials, np);
         ip.fwd.push_back(std::move(L));
     }
@@ -246,7 +251,7 @@ void build_items(const HostPlan& p, int nt, ItemPlan& ip)
         const int ent = p.rev_beg[l1] - p.rev_beg[l0];
         const int chunk = std::max(4, (ent + nt - 1) / nt);
         int np = 0;
-        split_layer(p.rev_beg, p.rev_pk, l0, l1, chunk, L, ip.rev_table, np);
+        split_layer(p.rev_beg, p.rev_pk, p.rev_x, mapped, l0, l1, chunk, L, ip.rev_table, np);
         ip.max_partials = std::max(ip.max_partials, np);
         ip.rev.push_back(std::move(L));
     }

@@ -72,14 +72,15 @@ struct DevPlan {
     // moments [bo_beg[L], bo_beg[L+1]) and a transposed table at tbl_base[L].
     int fwd_it_beg[kMaxDepth + 2], fwd_cb_beg[kMaxDepth + 2], fwd_tbl_base[kMaxDepth + 1];
     int rev_it_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2], rev_tbl_base[kMaxDepth + 1];
-    int rev_bo_beg[kMaxDepth + 2];
+    int rev_bo_beg[kMaxDepth + 2], rev_xa_beg[kMaxDepth + 2];
     const I4* fwd_items;             // (n entries, first moment position, partial slot or -1, -)
     const I4* fwd_comb;              // (moment position, p0, p1, -)
-    const U2* fwd_tbl;               // (i1 | i2 << 16, +-mult float bits; negative = last of a moment)
+    const U2* fwd_tbl;               // (i1 | i2 << 16, mult | (dest + 1) << 16 on the last entry of a moment)
     const I4* rev_items;
     const I4* rev_comb;
-    const U2* rev_tbl;               // (other | target << 16, +-mult)
+    const U2* rev_tbl;               // (other | target << 16, mult | (dest + 1) << 16)
     const int* rev_bo;               // moment positions without entries (dm = xi)
+    const int* rev_xa;               // moments of run items whose dm gets + xi after the sums
     int max_partials;
 };
 
@@ -87,7 +88,8 @@ struct DevPlan {
 // split_layer in mtp_plan.cpp).
 struct LayerItems {
     std::vector<I4> items, comb;
-    std::vector<int> base_only;
+    std::vector<int> base_only;   // positions
+    std::vector<int> xi_add;      // moment ids
     int table_base = 0;
 };
 

@@ -223,6 +223,39 @@ def loss_fn(E, F, S, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
     return wE * ((E - E_ref) ** 2).sum() + wF * ((F - F_ref) ** 2).sum() + wS * ((S - S_ref) ** 2).sum()
 
 
+def lls_init(pot, dbatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
+    """Weighted linear least squares for moment_coeffs at fixed radial and species coefficients.
+
+    Mirrors solve_lls_for_basis (motep_jax_train_import.py:605-683): every output (E, F, stress) is linear in
+    the moment coefficients, so the design matrix column s is the batch output with coefficient e_s minus the
+    output with all coefficients zero (computed here by the GPU forward kernels), and the weights are sqrt(w).
+    Returns the new coefficients (also loaded into ``pot``).
+    """
+    import torch
+
+    p = pot.params
+    S = len(p.moment_coeffs)
+    wE, wF, wS = (float(np.sqrt(w)) for w in weights)
+
+    def flat(E, F, St):
+        return torch.cat([wE * E.reshape(-1), wF * F.reshape(-1), wS * St.reshape(-1)])
+
+    pot.set_params(moment_coeffs=np.zeros(S))
+    off = flat(*batch_forward(pot, dbatch))
+    cols = []
+    for s in range(S):
+        e = np.zeros(S)
+        e[s] = 1.0
+        pot.set_params(moment_coeffs=e)
+        cols.append(flat(*batch_forward(pot, dbatch)) - off)
+    X = torch.stack(cols, dim=1)
+    y = flat(torch.as_tensor(E_ref, device="cuda"), torch.as_tensor(F_ref, device="cuda"),
+             torch.as_tensor(S_ref, device="cuda")) - off
+    sol = torch.linalg.lstsq(X.cpu(), y.cpu().unsqueeze(1)).solution.squeeze(1).numpy()
+    pot.set_params(moment_coeffs=sol)
+    return sol
+
+
 class Trainer:
     """Full-batch training with torch.optim.LBFGS over the hand-written gradient.
 
@@ -230,7 +263,8 @@ class Trainer:
     PyTorch's.  The step (forward + hand-written backward) is a fixed launch sequence on one stream.
     """
 
-    def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0):
+    def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0,
+                 lls: bool = True):
         import torch
 
         self.pot = pot
@@ -240,6 +274,8 @@ class Trainer:
         self.S_ref = torch.as_tensor(S_ref, dtype=torch.float64, device="cuda")
         self.weights = weights
         self.fn = make_function()
+        if lls:  # motep_jax_train_function.py:160-172: LLS for the moment coefficients first
+            lls_init(pot, self.dbatch, self.E_ref, self.F_ref, self.S_ref, weights)
         self.theta = torch.tensor(theta_of(pot.params), dtype=torch.float64, device="cuda", requires_grad=True)
         self.opt = torch.optim.LBFGS([self.theta], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
 

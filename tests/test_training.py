@@ -127,10 +127,13 @@ def test_gpu_training_reduces_loss(jm):
     tp.close()
     student = jm.configs.random_params(10, 2, 8, 1.6, 5.0, rng, sigma=0.05)
     pot = jm.MTPPotential(student, max_atoms=b.natoms, max_nbr=b.nslots)
-    tr = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref)
-    l0 = float(tr.loss())
+    tr0 = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=False)
+    l_init = float(tr0.loss().detach())
+    tr = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=True)
+    l0 = float(tr.loss().detach())
     losses = [tr.step() for _ in range(8)]
-    l1 = float(tr.loss())
-    print(f"loss {l0:.6e} -> {l1:.6e}", flush=True)
-    assert l1 < 0.5 * l0
+    l1 = float(tr.loss().detach())
+    print(f"loss {l_init:.6e} -> LLS {l0:.6e} -> L-BFGS {l1:.6e}", flush=True)
+    assert l0 < l_init
+    assert l1 < l0
     pot.close()

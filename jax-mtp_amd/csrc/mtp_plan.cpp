@@ -227,6 +227,24 @@ static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, 
         for (size_t j = 0; j < lists[i].size(); ++j) table[L.table_base + j * ni + i] = lists[i][j];
 }
 
+// Smallest chunk (>= ceil(entries / nt)) whose split yields at most nt items,
+// so that every thread of the workgroup gets one item per layer.
+static void split_layer_fit(const std::vector<int>& beg, const std::vector<U2>& pk, const std::vector<int>& mom,
+                            const std::vector<char>& mapped, int l0, int l1, int nt, LayerItems& L,
+                            std::vector<U2>& table, int& npart)
+{
+    const int ent = beg[l1] - beg[l0];
+    int chunk = std::max(4, (ent + nt - 1) / nt);
+    const size_t base = table.size();
+    for (;;) {
+        L = LayerItems();
+        table.resize(base);
+        split_layer(beg, pk, mom, mapped, l0, l1, chunk, L, table, npart);
+        if ((int)L.items.size() <= nt || chunk >= ent) return;
+        chunk += std::max(1, chunk / 32);
+    }
+}
+
 void build_items(const HostPlan& p, int nt, ItemPlan& ip)
 {
     ip = ItemPlan();
@@ -237,21 +255,17 @@ void build_items(const HostPlan& p, int nt, ItemPlan& ip)
     for (int x : p.file.alpha_moment_mapping) mapped[x] = 1;
     for (int d = 1; d <= p.n_depth; ++d) {
         LayerItems L;
-        const int l0 = p.fwd_layer[d], l1 = p.fwd_layer[d + 1];
-        const int ent = p.fwd_beg[l1] - p.fwd_beg[l0];
-        const int chunk = std::max(4, (ent + nt - 1) / nt);
         int np = 0;
-        split_layer(p.fwd_beg, p.fwd_pk, p.fwd_tgt, no_map, l0, l1, chunk, L, ip.fwd_table, np);
+        split_layer_fit(p.fwd_beg, p.fwd_pk, p.fwd_tgt, no_map, p.fwd_layer[d], p.fwd_layer[d + 1], nt, L,
+                        ip.fwd_table, np);
         ip.max_partials = std::max(ip.max_partials, np);
         ip.fwd.push_back(std::move(L));
     }
     for (int li = 0; li <= p.n_depth; ++li) {
         LayerItems L;
-        const int l0 = p.rev_layer[li], l1 = p.rev_layer[li + 1];
-        const int ent = p.rev_beg[l1] - p.rev_beg[l0];
-        const int chunk = std::max(4, (ent + nt - 1) / nt);
         int np = 0;
-        split_layer(p.rev_beg, p.rev_pk, p.rev_x, mapped, l0, l1, chunk, L, ip.rev_table, np);
+        split_layer_fit(p.rev_beg, p.rev_pk, p.rev_x, mapped, p.rev_layer[li], p.rev_layer[li + 1], nt, L,
+                        ip.rev_table, np);
         ip.max_partials = std::max(ip.max_partials, np);
         ip.rev.push_back(std::move(L));
     }

@@ -256,11 +256,69 @@ def lls_init(pot, dbatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
     return sol
 
 
-class Trainer:
-    """Full-batch training with torch.optim.LBFGS over the hand-written gradient.
+class LBFGS:
+    """Limited-memory BFGS with Armijo backtracking on a flat parameter tensor.
 
-    Replaces the reference's optax L-BFGS loop (motep_jax_train_function.py:178-213); the optimizer itself is
-    PyTorch's.  The step (forward + hand-written backward) is a fixed launch sequence on one stream.
+    The MTP loss is a high-degree polynomial in the radial coefficients with gradients of 1e6-1e8 at random
+    initial parameters; a backtracking line search that only asks for sufficient decrease is robust there
+    (torch.optim.LBFGS's strong-Wolfe search can return a zero step on such curvature).
+    """
+
+    def __init__(self, fun, history: int = 20, c1: float = 1e-4, max_backtrack: int = 60):
+        self.fun = fun          # theta -> (loss float, grad tensor)
+        self.m = history
+        self.c1 = c1
+        self.max_backtrack = max_backtrack
+        self.S, self.Y = [], []
+
+    def _direction(self, g):
+        q = g.clone()
+        alphas = []
+        for s, y in zip(reversed(self.S), reversed(self.Y)):
+            rho = 1.0 / float(y @ s)
+            a = rho * float(s @ q)
+            q -= a * y
+            alphas.append((rho, a))
+        if self.S:
+            s, y = self.S[-1], self.Y[-1]
+            q *= float(s @ y) / float(y @ y)
+        else:
+            q /= max(1.0, float(g.abs().max()))
+        for (s, y), (rho, a) in zip(zip(self.S, self.Y), reversed(alphas)):
+            b = rho * float(y @ q)
+            q += (a - b) * s
+        return -q
+
+    def step(self, theta):
+        f, g = self.fun(theta)
+        d = self._direction(g)
+        gd = float(g @ d)
+        if gd >= 0:  # not a descent direction: restart from steepest descent
+            self.S, self.Y = [], []
+            d = -g / max(1.0, float(g.abs().max()))
+            gd = float(g @ d)
+        t = 1.0
+        for _ in range(self.max_backtrack):
+            f_new, g_new = self.fun(theta + t * d)
+            if f_new <= f + self.c1 * t * gd:
+                s = t * d
+                y = g_new - g
+                if float(s @ y) > 1e-30:
+                    self.S.append(s)
+                    self.Y.append(y)
+                    if len(self.S) > self.m:
+                        self.S.pop(0)
+                        self.Y.pop(0)
+                return theta + s, f_new
+            t *= 0.5
+        return theta, f
+
+
+class Trainer:
+    """Full-batch training: LLS start, then L-BFGS (Armijo backtracking) over the hand-written gradient.
+
+    Replaces the reference's LLS + optax L-BFGS loop (motep_jax_train_function.py:160-213).  Each loss/gradient
+    evaluation is the batch forward plus the hand-written parameter backward, through torch.autograd.
     """
 
     def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0,
@@ -277,17 +335,26 @@ class Trainer:
         if lls:  # motep_jax_train_function.py:160-172: LLS for the moment coefficients first
             lls_init(pot, self.dbatch, self.E_ref, self.F_ref, self.S_ref, weights)
         self.theta = torch.tensor(theta_of(pot.params), dtype=torch.float64, device="cuda", requires_grad=True)
-        self.opt = torch.optim.LBFGS([self.theta], lr=lr, max_iter=1, history_size=20, line_search_fn="strong_wolfe")
+        self.opt = LBFGS(self.value_and_grad)
 
-    def loss(self):
-        E, F, S = self.fn.apply(self.theta, self.pot, self.dbatch)
+    def loss(self, theta=None):
+        th = self.theta if theta is None else theta
+        E, F, S = self.fn.apply(th, self.pot, self.dbatch)
         return loss_fn(E, F, S, self.E_ref, self.F_ref, self.S_ref, self.weights)
 
-    def step(self):
-        def closure():
-            self.opt.zero_grad()
-            L = self.loss()
-            L.backward()
-            return L
+    def value_and_grad(self, theta):
+        import torch
 
-        return float(self.opt.step(closure))
+        th = theta.detach().clone().requires_grad_(True)
+        L = self.loss(th)
+        (g,) = torch.autograd.grad(L, th)
+        return float(L.detach()), g.detach()
+
+    def step(self) -> float:
+        """One L-BFGS iteration; returns the loss after it."""
+        import torch
+
+        new, f = self.opt.step(self.theta.detach())
+        with torch.no_grad():
+            self.theta.copy_(new)
+        return f

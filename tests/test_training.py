@@ -132,6 +132,7 @@ def test_gpu_training_reduces_loss(jm):
     tr = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=True)
     l0 = float(tr.loss().detach())
     losses = [tr.step() for _ in range(8)]
+    print("L-BFGS losses", losses, flush=True)
     l1 = float(tr.loss().detach())
     print(f"loss {l_init:.6e} -> LLS {l0:.6e} -> L-BFGS {l1:.6e}", flush=True)
     assert l0 < l_init

@@ -143,6 +143,79 @@ int mtp_batch_param_grad(mtp_ctx *ctx, int nrows, int nslots, int nconf, const i
                          const double *d_volumes, const double *d_gE, const double *d_gF, const double *d_gS,
                          double *d_grad, void *stream);
 
+/* ---- basis values, per-row coefficients, basis Jacobians -------------
+ * No counterpart in the reference (it has no learned head).  They expose the
+ * pieces of the atom kernel the distilled head is built from.  All arrays
+ * are device arrays; rows and slots as in mtp_compute_padded_device. */
+
+/* Basis values b[row][s] = m[alpha_moment_mapping[s]] (the scalar moments
+ * that E_i = species_coeff + moment_coeffs . b_i contracts; forward pass
+ * only). */
+int mtp_descriptor_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const double *d_rijs,
+                          const int *d_jtypes, double *d_basis, void *stream);
+
+/* Slot gradients d_g[row][slot][3] = d(xi_row . b_row)/d r_row,slot and
+ * per-row virials d_Wi[row][9] (may be NULL) for per-row coefficient vectors
+ * xi_row = d_xi + row * xi_stride (xi_stride 0: one vector for all rows). */
+int mtp_slot_grad_xi_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const double *d_rijs,
+                            const int *d_jtypes, const double *d_xi, int xi_stride, double *d_g, double *d_Wi,
+                            void *stream);
+
+/* Dense basis Jacobian d_J[row][s][slot][3] = d b_row,s / d r_row,slot. */
+int mtp_basis_jacobian_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const double *d_rijs,
+                              const int *d_jtypes, double *d_J, void *stream);
+
+/* d_g[row][slot] = sum_s d_xi[row][s] d_J[row][s][slot] */
+int mtp_jacobian_apply_device(int nrows, int nslots, int S, const double *d_J, const double *d_xi, double *d_g,
+                              void *stream);
+
+/* d_v[row][s] = [row < n_f] sum_slot d_J[row][s][slot] . (gF[row] - [j < n_f] gF[j]), j = d_js[row][slot]:
+ * the adjoint of the Newton-3 force gather applied to force gradients gF. */
+int mtp_jacobian_tapply_device(int nrows, int nslots, int S, int n_f, const double *d_J, const int *d_js,
+                               const double *d_gF, double *d_v, void *stream);
+
+/* Newton-3 force gather of slot gradients d_g (jax_backend.py:263-303) with
+ * the context's scratch. */
+int mtp_gather_forces_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_js, const double *d_g, int n_f,
+                             double *d_forces, void *stream);
+
+/* ---- distilled head ---------------------------------------------------
+ * No counterpart in the reference.  An MLP over the basis values of a
+ * low-level potential `desc` (level 12: 29 values) stands in for a
+ * high-level potential `exact`, which re-evaluates every atom whose
+ * normalised basis values leave the training box [lo, hi] (per species).
+ * Network: z = [(b - mean) * inv_std, onehot(type)] (S + Sp <= 32),
+ * 64 SiLU, 64 SiLU, E_i = e_shift[type] + e_scale * out.  Parameters theta
+ * (mtp_head_theta_size() doubles): W1[32][64] b1[64] W2[64][64] b2[64]
+ * w3[64] b3, weights stored (in, out). */
+typedef struct mtp_head mtp_head;
+
+int mtp_head_theta_size(void);
+/* exact may be NULL (no fallback); lo/hi [Sp][S] may both be NULL (no box).
+ * Both potentials must stay open while the head is. */
+int mtp_head_open(mtp_ctx *desc, mtp_ctx *exact, const double *theta, const double *mean, const double *inv_std,
+                  const double *lo, const double *hi, const double *e_shift, double e_scale, mtp_head **out);
+void mtp_head_close(mtp_head *h);
+const char *mtp_head_last_error(const mtp_head *h);
+/* theta from host (on_device 0, synchronous) or device memory (stream-ordered). */
+int mtp_head_set_theta(mtp_head *h, const double *theta, int on_device, void *stream);
+
+/* Same outputs as mtp_compute_padded_device (d_out[14]: E, stress6,
+ * virial6, status), plus d_flags[nrows] (1 = out of box; may be NULL) and
+ * d_n_fallback (device int: rows re-evaluated exactly; may be NULL). */
+int mtp_head_compute_padded_device(mtp_head *h, int nrows, int nslots, const int *d_itypes, const int *d_js,
+                                   const double *d_rijs, const int *d_jtypes, int cell_rank, double volume, int n_e,
+                                   int n_f, double *d_out, double *d_forces, double *d_eatom, int *d_flags,
+                                   int *d_n_fallback, void *stream);
+
+/* Network alone on given basis values: d_E[row], d_xi[row][s] = dE_row/db_row,s. */
+int mtp_head_eval_device(mtp_head *h, int nrows, const int *d_itypes, const double *d_basis, double *d_E,
+                         double *d_xi, int *d_flags, void *stream);
+/* d_grad[theta] = d/dtheta sum_row [alpha_row E_row + v_row . dE_row/db_row]
+ * (the head's training gradient: alpha = dL/dE, v = J^T dL/dg). */
+int mtp_head_param_grad_device(mtp_head *h, int nrows, const int *d_itypes, const double *d_basis,
+                               const double *d_alpha, const double *d_v, double *d_grad, void *stream);
+
 /* ---- host-only helpers (no GPU work; usable without a device) -------- */
 
 /* Parse, validate and plan a .mtp file without touching the GPU.  Returns

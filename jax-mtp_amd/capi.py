@@ -44,6 +44,20 @@ SIGNATURES = {
     "mtp_profile_read": (_I, [_P, _P, _P]),
     "mtp_batch_forward": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mtp_batch_param_grad": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mtp_descriptor_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _P]),
+    "mtp_slot_grad_xi_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "mtp_basis_jacobian_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _P]),
+    "mtp_jacobian_apply_device": (_I, [_I, _I, _I, _P, _P, _P, _P]),
+    "mtp_jacobian_tapply_device": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
+    "mtp_gather_forces_device": (_I, [_P, _I, _I, _P, _P, _I, _P, _P]),
+    "mtp_head_theta_size": (_I, []),
+    "mtp_head_open": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, ctypes.POINTER(_P)]),
+    "mtp_head_close": (None, [_P]),
+    "mtp_head_last_error": (ctypes.c_char_p, [_P]),
+    "mtp_head_set_theta": (_I, [_P, _P, _I, _P]),
+    "mtp_head_compute_padded_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _I, _D, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "mtp_head_eval_device": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "mtp_head_param_grad_device": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "mtp_check_file": (_I, [ctypes.c_char_p, _I, _P, _P, _P, _P, _P, _P, _P]),
     "mtp_pack_full_list": (_LL, [_I, _P, _P, _P, _I, _P, _P, _P, _D, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P]),
     "mtp_stage_host": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -89,9 +103,12 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
-def check(rc: int, ctx=None) -> int:
+def check(rc: int, ctx=None, head=None) -> int:
     if rc < 0:
         L = lib()
-        msg = L.mtp_last_error(ctx) if ctx else L.mtp_global_error()
+        if head:
+            msg = L.mtp_head_last_error(head)
+        else:
+            msg = L.mtp_last_error(ctx) if ctx else L.mtp_global_error()
         raise MTPError(rc, (msg or b"").decode())
     return rc

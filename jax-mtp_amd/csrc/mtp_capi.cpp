@@ -15,6 +15,7 @@
 
 #include "../../include/mtp_hip.h"
 #include "mtp_format.hpp"
+#include "mtp_head.hpp"
 #include "mtp_kernels.hpp"
 #include "mtp_plan.hpp"
 #include "mtp_train.hpp"
@@ -48,6 +49,9 @@ struct mtp_ctx {
     DevPlan dev{};                        // variant 0 (and the base of the others)
     DevPlan vdev[kNumVariants]{};         // per-variant device plans
     size_t vlds[kNumVariants] = {0, 0, 0};  // LDS bytes per variant, 0 = unavailable
+    size_t vlds_xi[kNumVariants] = {0, 0, 0};  // the same with per-row coefficient vectors
+    int n_cu = 0;
+    double* d_eye = nullptr;              // [S][S] identity (basis Jacobians), allocated on first use
     int variant = kVariantV1;
     int max_atoms = 0, max_nbr = 0, debug_level = 0;
     double cutoff = 0.0;
@@ -165,6 +169,8 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
         return MTP_ERR_HIP;
     }
     c->lds_max = lds_max;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || c->n_cu <= 0)
+        c->n_cu = 256;
     if ((long long)c->lds_bytes > (long long)lds_max) {
         set_global_error("potential needs " + std::to_string(c->lds_bytes) + " bytes of LDS per atom, device has " +
                          std::to_string(lds_max));
@@ -185,6 +191,9 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     auto* rev_mult = c->alloc<double>(p.rev_mult.size(), e);
     auto* part_blk = c->alloc<I2>(p.part_blk.size(), e);
     auto* mapping = c->alloc<int>(f.alpha_moment_mapping.size(), e);
+    auto* map_x = c->alloc<int>(std::max<size_t>(1, p.map_x.size()), e);
+    auto* map_beg = c->alloc<int>(p.map_beg.size(), e);
+    auto* map_s = c->alloc<int>(std::max<size_t>(1, p.map_s.size()), e);
     c->d_radial = c->alloc<double>(f.radial_coeffs.size(), e);
     c->d_species = c->alloc<double>(f.species_coeffs.size(), e);
     c->d_moment = c->alloc<double>(f.moment_coeffs.size() ? f.moment_coeffs.size() : 1, e);
@@ -220,6 +229,9 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     if (e == hipSuccess) e = upload(rev_mult, p.rev_mult, c->stream);
     if (e == hipSuccess) e = upload(part_blk, p.part_blk, c->stream);
     if (e == hipSuccess) e = upload(mapping, f.alpha_moment_mapping, c->stream);
+    if (e == hipSuccess) e = upload(map_x, p.map_x, c->stream);
+    if (e == hipSuccess) e = upload(map_beg, p.map_beg, c->stream);
+    if (e == hipSuccess) e = upload(map_s, p.map_s, c->stream);
     if (e != hipSuccess) {
         set_global_error(std::string("device allocation/upload failed: ") + hipGetErrorString(e));
         delete c;
@@ -241,6 +253,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.n_parts = p.n_parts;
     for (int q = 0; q < 9; ++q) d.part_beg[q] = p.part_beg[q];
     d.part_blk = part_blk;
+    d.n_mapped = (int)p.map_x.size(); d.map_x = map_x; d.map_beg = map_beg; d.map_s = map_s;
 
     // --- kernel variants ------------------------------------------------
     c->vdev[kVariantV1] = d;
@@ -308,6 +321,10 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
                 c->vdev[v] = dv;
                 c->vlds[v] = lds;
+                const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
+                                                   dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
+                if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
+                    c->vlds_xi[v] = lxi;
             }
         }
         if (e != hipSuccess) {
@@ -721,6 +738,348 @@ int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const i
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     hipError_t e = launch_train_grad(ctx->dev, A, ctx->train_lds, ctx->d_part, ctx->d_xisum, ctx->d_tsum, d_grad, s);
     if (e != hipSuccess) return ctx->hip_fail(e, "parameter-gradient launch");
+    return MTP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Basis values, per-row coefficient vectors, basis Jacobians and the
+// distilled head.  None of these has a counterpart in the reference (it has
+// no learned head); they serve the head's inference and training.
+// ---------------------------------------------------------------------------
+namespace {
+
+int mode_variant(const mtp_ctx* c, bool with_xi)
+{
+    const size_t* l = with_xi ? c->vlds_xi : c->vlds;
+    if (c->variant != kVariantV1 && l[c->variant]) return c->variant;
+    if (l[kVariantV2x256]) return kVariantV2x256;
+    if (l[kVariantV2x512]) return kVariantV2x512;
+    return -1;
+}
+
+int check_rows(mtp_ctx* c, int nrows, int nslots)
+{
+    if (nrows < 0 || nrows > c->max_atoms)
+        return c->fail(MTP_ERR_CAPACITY, "rows " + std::to_string(nrows) + " outside 0..max_atoms " +
+                                             std::to_string(c->max_atoms));
+    if (nslots <= 0 || nslots > c->max_nbr)
+        return c->fail(MTP_ERR_CAPACITY, "slots " + std::to_string(nslots) + " outside 1..max_neighbors " +
+                                             std::to_string(c->max_nbr));
+    return MTP_OK;
+}
+
+AtomArgs row_args(int nrows, int nslots, const int* it, const double* rij, const int* jt)
+{
+    AtomArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.rij = rij; A.itypes = it; A.jtypes = jt;
+    return A;
+}
+
+int slot_grad_xi(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs, const int* d_jtypes,
+                 const double* d_xi, int xi_stride, double* d_g, double* d_Wi, hipStream_t s)
+{
+    const int v = mode_variant(ctx, true);
+    if (v < 0) return ctx->fail(MTP_ERR_UNSUPPORTED, "per-row coefficients need a balanced-item kernel variant");
+    AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+    A.xi_atom = d_xi; A.xi_stride = xi_stride; A.g = d_g; A.Wi = d_Wi;
+    hipError_t e = launch_atoms(ctx->vdev[v], A, ctx->vlds_xi[v], ctx->radial_fp32, s, v, nrows);
+    if (e != hipSuccess) return ctx->hip_fail(e, "slot-gradient launch");
+    return MTP_OK;
+}
+
+}  // namespace
+
+struct mtp_head {
+    mtp_ctx* desc = nullptr;
+    mtp_ctx* exact = nullptr;
+    HeadNet net{};
+    std::string err;
+    std::vector<void*> dptrs;
+    double *d_theta = nullptr, *d_wt = nullptr, *d_basis = nullptr, *d_xi = nullptr, *d_E = nullptr;
+    int *d_rows = nullptr, *d_count = nullptr;
+    double* d_part = nullptr;
+
+    int fail(int code, const std::string& msg)
+    {
+        err = msg;
+        if (desc && desc->debug_level > 0) std::fprintf(stderr, "[mtp_hip] %s\n", msg.c_str());
+        return code;
+    }
+    template <typename T>
+    T* alloc(size_t n, hipError_t& e)
+    {
+        T* p = dalloc<T>(n, e);
+        if (p) dptrs.push_back(p);
+        return p;
+    }
+    ~mtp_head()
+    {
+        if (desc && desc->stream) hipStreamSynchronize(desc->stream);
+        for (void* p : dptrs) hipFree(p);
+    }
+};
+
+extern "C" {
+
+int mtp_descriptor_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs,
+                          const int* d_jtypes, double* d_basis, void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    int rc = check_rows(ctx, nrows, nslots);
+    if (rc || nrows == 0) return rc;
+    if (!d_itypes || !d_rijs || !d_jtypes || !d_basis) return ctx->fail(MTP_ERR_ARG, "null descriptor array");
+    const int v = mode_variant(ctx, false);
+    if (v < 0) return ctx->fail(MTP_ERR_UNSUPPORTED, "basis values need a balanced-item kernel variant");
+    AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+    A.basis = d_basis; A.fwd_only = 1;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipError_t e = launch_atoms(ctx->vdev[v], A, ctx->vlds[v], ctx->radial_fp32, s, v, nrows);
+    if (e != hipSuccess) return ctx->hip_fail(e, "descriptor launch");
+    return MTP_OK;
+}
+
+int mtp_slot_grad_xi_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs,
+                            const int* d_jtypes, const double* d_xi, int xi_stride, double* d_g, double* d_Wi,
+                            void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    int rc = check_rows(ctx, nrows, nslots);
+    if (rc || nrows == 0) return rc;
+    if (!d_itypes || !d_rijs || !d_jtypes || !d_xi || !d_g || xi_stride < 0)
+        return ctx->fail(MTP_ERR_ARG, "null slot-gradient array or negative stride");
+    return slot_grad_xi(ctx, nrows, nslots, d_itypes, d_rijs, d_jtypes, d_xi, xi_stride, d_g, d_Wi,
+                        stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int mtp_basis_jacobian_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs,
+                              const int* d_jtypes, double* d_J, void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    int rc = check_rows(ctx, nrows, nslots);
+    if (rc || nrows == 0) return rc;
+    if (!d_itypes || !d_rijs || !d_jtypes || !d_J) return ctx->fail(MTP_ERR_ARG, "null Jacobian array");
+    const int S = ctx->plan.file.S();
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipError_t e = hipSuccess;
+    if (!ctx->d_eye) {
+        std::vector<double> eye((size_t)S * S, 0.0);
+        for (int q = 0; q < S; ++q) eye[(size_t)q * S + q] = 1.0;
+        ctx->d_eye = ctx->alloc<double>(eye.size(), e);
+        if (e == hipSuccess) e = upload(ctx->d_eye, eye, s);
+        if (e != hipSuccess) return ctx->hip_fail(e, "identity upload");
+    }
+    const size_t row = (size_t)nslots * 3 * sizeof(double);
+    for (int q = 0; q < S; ++q) {
+        rc = slot_grad_xi(ctx, nrows, nslots, d_itypes, d_rijs, d_jtypes, ctx->d_eye + (size_t)q * S, 0, ctx->d_g,
+                          nullptr, s);
+        if (rc) return rc;
+        e = hipMemcpy2DAsync(d_J + (size_t)q * nslots * 3, row * S, ctx->d_g, row, row, nrows,
+                             hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return ctx->hip_fail(e, "Jacobian copy");
+    }
+    return MTP_OK;
+}
+
+int mtp_jacobian_apply_device(int nrows, int nslots, int S, const double* d_J, const double* d_xi, double* d_g,
+                              void* stream)
+{
+    if (nrows < 0 || nslots <= 0 || S <= 0 || (nrows > 0 && (!d_J || !d_xi || !d_g))) return MTP_ERR_ARG;
+    return jacobian_apply(nrows, nslots, S, d_J, d_xi, d_g, (hipStream_t)stream) == hipSuccess ? MTP_OK : MTP_ERR_HIP;
+}
+
+int mtp_jacobian_tapply_device(int nrows, int nslots, int S, int n_f, const double* d_J, const int* d_js,
+                               const double* d_gF, double* d_v, void* stream)
+{
+    if (nrows < 0 || nslots <= 0 || S <= 0 || (nrows > 0 && (!d_J || !d_js || !d_gF || !d_v))) return MTP_ERR_ARG;
+    if ((size_t)nslots * 3 * sizeof(double) > 64 * 1024) return MTP_ERR_UNSUPPORTED;
+    return jacobian_tapply(nrows, nslots, S, n_f, d_J, d_js, d_gF, d_v, (hipStream_t)stream) == hipSuccess
+               ? MTP_OK
+               : MTP_ERR_HIP;
+}
+
+int mtp_gather_forces_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_js, const double* d_g, int n_f,
+                             double* d_forces, void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    int rc = check_rows(ctx, nrows, nslots);
+    if (rc || nrows == 0) return rc;
+    if (!d_js || !d_g || !d_forces) return ctx->fail(MTP_ERR_ARG, "null gather array");
+    AtomArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.js = d_js; A.g = const_cast<double*>(d_g);
+    ForceArgs F{};
+    F.n_f = n_f; F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces;
+    hipError_t e = launch_gather(A, F, stream ? (hipStream_t)stream : ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "force gather");
+    return MTP_OK;
+}
+
+const char* mtp_head_last_error(const mtp_head* h) { return h ? h->err.c_str() : mtp_global_error(); }
+
+int mtp_head_theta_size(void) { return kHeadTheta; }
+
+int mtp_head_open(mtp_ctx* desc, mtp_ctx* exact, const double* theta, const double* mean, const double* inv_std,
+                  const double* lo, const double* hi, const double* e_shift, double e_scale, mtp_head** out)
+{
+    if (!out) { set_global_error("mtp_head_open: out is NULL"); return MTP_ERR_ARG; }
+    *out = nullptr;
+    if (!desc || !theta || !mean || !inv_std || !e_shift || (!lo) != (!hi)) {
+        set_global_error("mtp_head_open: null descriptor potential or head array");
+        return MTP_ERR_ARG;
+    }
+    const MtpFile& f = desc->plan.file;
+    const int S = f.S(), Sp = f.species_count;
+    if (S + Sp > kHeadIn) {
+        set_global_error("head inputs (basis values + species) exceed " + std::to_string(kHeadIn));
+        return MTP_ERR_UNSUPPORTED;
+    }
+    if (mode_variant(desc, true) < 0 || mode_variant(desc, false) < 0) {
+        set_global_error("the descriptor potential has no balanced-item kernel variant");
+        return MTP_ERR_UNSUPPORTED;
+    }
+    if (exact && (exact->plan.file.species_count != Sp || mode_variant(exact, false) < 0)) {
+        set_global_error("the exact potential must have the descriptor's species and a balanced-item variant");
+        return MTP_ERR_ARG;
+    }
+    auto* h = new mtp_head();
+    h->desc = desc; h->exact = exact;
+    const size_t A = (size_t)desc->max_atoms;
+    hipError_t e = hipSuccess;
+    h->d_theta = h->alloc<double>(kHeadTheta, e);
+    h->d_wt = h->alloc<double>(kHeadHidden * kHeadIn + kHeadHidden * kHeadHidden, e);
+    double* d_mean = h->alloc<double>(S, e);
+    double* d_inv = h->alloc<double>(S, e);
+    double* d_lo = lo ? h->alloc<double>((size_t)Sp * S, e) : nullptr;
+    double* d_hi = hi ? h->alloc<double>((size_t)Sp * S, e) : nullptr;
+    double* d_shift = h->alloc<double>(Sp, e);
+    h->d_basis = h->alloc<double>(A * S, e);
+    h->d_xi = h->alloc<double>(A * S, e);
+    h->d_E = h->alloc<double>(A, e);
+    h->d_rows = h->alloc<int>(A, e);
+    h->d_count = h->alloc<int>(1, e);
+    hipStream_t s = desc->stream;
+    auto up = [&](double* d, const double* src, size_t n) {
+        if (e == hipSuccess && d) e = hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyHostToDevice, s);
+    };
+    up(h->d_theta, theta, kHeadTheta);
+    up(d_mean, mean, S);
+    up(d_inv, inv_std, S);
+    up(d_lo, lo, (size_t)Sp * S);
+    up(d_hi, hi, (size_t)Sp * S);
+    up(d_shift, e_shift, Sp);
+    HeadNet& n = h->net;
+    n.S = S; n.Sp = Sp; n.theta = h->d_theta; n.wt = h->d_wt; n.mean = d_mean; n.inv_std = d_inv;
+    n.lo = d_lo; n.hi = d_hi; n.e_shift = d_shift; n.e_scale = e_scale;
+    if (e == hipSuccess) e = head_prepare(n, h->d_wt, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        set_global_error(std::string("head allocation/upload failed: ") + hipGetErrorString(e));
+        delete h;
+        return MTP_ERR_HIP;
+    }
+    *out = h;
+    return MTP_OK;
+}
+
+void mtp_head_close(mtp_head* h) { delete h; }
+
+int mtp_head_set_theta(mtp_head* h, const double* theta, int on_device, void* stream)
+{
+    if (!h) return MTP_ERR_ARG;
+    if (!theta) return h->fail(MTP_ERR_ARG, "null theta");
+    hipStream_t s = stream ? (hipStream_t)stream : h->desc->stream;
+    hipError_t e = hipMemcpyAsync(h->d_theta, theta, kHeadTheta * sizeof(double),
+                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = head_prepare(h->net, h->d_wt, s);
+    if (e == hipSuccess && !on_device) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("theta upload: ") + hipGetErrorString(e));
+    return MTP_OK;
+}
+
+int mtp_head_eval_device(mtp_head* h, int nrows, const int* d_itypes, const double* d_basis, double* d_E,
+                         double* d_xi, int* d_flags, void* stream)
+{
+    if (!h) return MTP_ERR_ARG;
+    if (nrows < 0 || nrows > h->desc->max_atoms) return h->fail(MTP_ERR_CAPACITY, "rows outside 0..max_atoms");
+    if (nrows > 0 && (!d_itypes || !d_basis || !d_E || !d_xi)) return h->fail(MTP_ERR_ARG, "null head array");
+    hipError_t e = head_eval(h->net, nrows, d_itypes, d_basis, d_E, d_xi, d_flags, nullptr, nullptr,
+                             stream ? (hipStream_t)stream : h->desc->stream);
+    if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
+    return MTP_OK;
+}
+
+int mtp_head_param_grad_device(mtp_head* h, int nrows, const int* d_itypes, const double* d_basis,
+                               const double* d_alpha, const double* d_v, double* d_grad, void* stream)
+{
+    if (!h) return MTP_ERR_ARG;
+    if (nrows <= 0 || nrows > h->desc->max_atoms) return h->fail(MTP_ERR_CAPACITY, "rows outside 1..max_atoms");
+    if (!d_itypes || !d_basis || !d_alpha || !d_v || !d_grad) return h->fail(MTP_ERR_ARG, "null head array");
+    hipError_t e = hipSuccess;
+    if (!h->d_part) {
+        h->d_part = h->alloc<double>(head_grad_partials(h->desc->max_atoms), e);
+        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head partials: ") + hipGetErrorString(e));
+    }
+    e = head_param_grad(h->net, nrows, d_itypes, d_basis, d_alpha, d_v, h->d_part, d_grad,
+                        stream ? (hipStream_t)stream : h->desc->stream);
+    if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head gradient launch: ") + hipGetErrorString(e));
+    return MTP_OK;
+}
+
+int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int* d_itypes, const int* d_js,
+                                   const double* d_rijs, const int* d_jtypes, int cell_rank, double volume, int n_e,
+                                   int n_f, double* d_out, double* d_forces, double* d_eatom, int* d_flags,
+                                   int* d_n_fallback, void* stream)
+{
+    if (!h) return MTP_ERR_ARG;
+    mtp_ctx* D = h->desc;
+    mtp_ctx* X = h->exact;
+    int rc = check_rows(D, nrows, nslots);
+    if (rc) return h->fail(rc, D->err);
+    if (X && (nrows > X->max_atoms || nslots > X->max_nbr))
+        return h->fail(MTP_ERR_CAPACITY, "rows/slots exceed the exact potential's capacity");
+    if (!d_out || (nrows > 0 && (!d_itypes || !d_js || !d_rijs || !d_jtypes || !d_forces)))
+        return h->fail(MTP_ERR_ARG, "null input or output array");
+    if (n_e < 0 || n_f < 0) return h->fail(MTP_ERR_ARG, "negative natoms_energy / natoms_force");
+    hipStream_t s = stream ? (hipStream_t)stream : D->stream;
+    double* E = d_eatom ? d_eatom : h->d_E;
+    hipError_t e = hipSuccess;
+    AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+    A.js = d_js; A.Ei = E; A.g = D->d_g; A.Wi = D->d_Wi;
+    ForceArgs F{};
+    F.n_e = n_e; F.n_f = n_f; F.cell_rank = cell_rank; F.volume = volume;
+    F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out;
+    if (nrows > 0) {
+        // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
+        rc = mtp_descriptor_device(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_basis, s);
+        if (rc) return h->fail(rc, D->err);
+        e = head_zero_count(h->d_count, s);
+        if (e == hipSuccess)
+            e = head_eval(h->net, nrows, d_itypes, h->d_basis, E, h->d_xi, d_flags, X ? h->d_rows : nullptr,
+                          X ? h->d_count : nullptr, s);
+        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
+        // 3. slot gradients of the head energy through the descriptor
+        rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g, D->d_Wi, s);
+        if (rc) return h->fail(rc, D->err);
+        // 4. exact energies and slot gradients of the out-of-box rows
+        if (X) {
+            const int v = mode_variant(X, false);
+            AtomArgs B = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+            B.Ei = E; B.g = D->d_g; B.Wi = D->d_Wi; B.rows = h->d_rows; B.row_count = h->d_count;
+            const int grid = nrows < X->n_cu ? nrows : X->n_cu;
+            e = launch_atoms(X->vdev[v], B, X->vlds[v], false, s, v, grid);
+            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("fallback launch: ") + hipGetErrorString(e));
+        }
+        // 5. Newton-3 gather
+        e = launch_gather(A, F, s);
+        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("force gather: ") + hipGetErrorString(e));
+    }
+    e = launch_totals(A, F, s);
+    if (e == hipSuccess && d_n_fallback) {
+        if (X && nrows > 0) e = hipMemcpyAsync(d_n_fallback, h->d_count, sizeof(int), hipMemcpyDeviceToDevice, s);
+        else e = head_zero_count(d_n_fallback, s);
+    }
+    if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head totals: ") + hipGetErrorString(e));
     return MTP_OK;
 }
 

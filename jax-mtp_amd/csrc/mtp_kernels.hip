@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int K, int R, int nu_max, int n_parts, int nt,
-                                                    int max_partials)
+                                                    int max_partials, int xi_len)
 {
     AtomLds2 L;
     int mlen = amc;
@@ -287,16 +287,19 @@ __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int K, int R, int 
     L.vz = L.vy + K;
     L.r = L.vz + K;
     L.red = L.r + K;
-    L.ints = L.red + nt;
+    L.xl = L.red + nt;
+    L.ints = L.xl + xi_len;
     L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
     return L;
 }
 
-size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials)
+size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials, int xi_len)
 {
     return sizeof(double) *
-           (size_t)atom_lds2_layout(amc, nslots, R, nu_max, n_parts, nt, max_partials).total_doubles;
+           (size_t)atom_lds2_layout(amc, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
 }
+
+hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
 
 // One contraction layer over its work items.  Each thread walks its item's
 // entries in batches of kUB, with the table loads of the next batch in
@@ -309,7 +312,7 @@ constexpr int kUB = 8;
 
 template <int NT, bool REV>
 __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* m, double* dm, double* part,
-                                                  int tid)
+                                                  int tid, const double* xi)
 {
     const int i0 = REV ? P.rev_it_beg[L] : P.fwd_it_beg[L];
     const int ni = (REV ? P.rev_it_beg[L + 1] : P.fwd_it_beg[L + 1]) - i0;
@@ -363,32 +366,31 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
     for (int c = c0 + tid; c < c1; c += NT) {
         const I4 cb = comb[c];
         const int x = mom[cb.x];
-        double acc = REV ? P.xi_of[x] : 0.0;
+        double acc = REV ? xi[x] : 0.0;
         for (int p = cb.y; p < cb.z; ++p) acc += part[p];
         dst[x] = acc;
     }
     if (REV) {
         for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
             const int x = mom[P.rev_bo[t]];
-            dm[x] = P.xi_of[x];
+            dm[x] = xi[x];
         }
         for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
             const int x = P.rev_xa[t];
-            dm[x] += P.xi_of[x];
+            dm[x] += xi[x];
         }
     }
     if (c1 > c0 || REV) __syncthreads();
 }
 
 template <typename RadialReal, int NT>
-__global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A)
+__device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A, const int row, double* lds)
 {
-    extern __shared__ double lds[];
-    const int row = blockIdx.x;
     const int tid = threadIdx.x;
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
-    const AtomLds2 L = atom_lds2_layout(P.amc, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials);
+    const AtomLds2 L = atom_lds2_layout(P.amc, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
+                                        A.xi_atom ? P.amc : 0);
     double* m = lds + L.m;
     double* dm = lds + L.dm;
     double* part = lds + L.part;
@@ -486,23 +488,45 @@ __global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A
     __syncthreads();
 
     // ---- C: contraction layers (depth 1 .. n_depth)
-    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, m, dm, part, tid);
+    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, m, dm, part, tid, P.xi_of);
 
-    // ---- D: site energy
+    // ---- D: site energy (and the basis values m[mapping] when asked for)
+    const double* xrow = A.xi_atom ? A.xi_atom + (size_t)row * A.xi_stride : P.moment_coeffs;
     {
         double acc = 0.0;
-        for (int s = tid; s < P.S; s += NT) acc += P.moment_coeffs[s] * m[P.mapping[s]];
+        for (int s = tid; s < P.S; s += NT) {
+            const double b = m[P.mapping[s]];
+            acc += xrow[s] * b;
+            if (A.basis) A.basis[(size_t)row * P.S + s] = b;
+        }
         red[tid] = acc;
         __syncthreads();
         for (int w = NT / 2; w > 0; w >>= 1) {
             if (tid < w) red[tid] += red[tid + w];
             __syncthreads();
         }
-        if (tid == 0) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
+        if (tid == 0 && A.Ei) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
     }
+    if (A.fwd_only) return;
 
-    // ---- E: reverse layers, deepest first
-    for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, m, dm, part, tid);
+    // ---- E: reverse layers, deepest first.  dE/dm starts from xi_of (sum of
+    // the coefficients mapped to each moment), or from this row's own
+    // coefficient vector summed the same way (distilled-head forces, basis
+    // Jacobians).
+    const double* xi = P.xi_of;
+    if (A.xi_atom) {
+        double* xl = lds + L.xl;
+        for (int x = tid; x < P.amc; x += NT) xl[x] = 0.0;
+        __syncthreads();
+        for (int q = tid; q < P.n_mapped; q += NT) {
+            double v = 0.0;
+            for (int t = P.map_beg[q]; t < P.map_beg[q + 1]; ++t) v += xrow[P.map_s[t]];
+            xl[P.map_x[q]] = v;
+        }
+        __syncthreads();
+        xi = xl;
+    }
+    for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, m, dm, part, tid, xi);
 
     // ---- F: per-slot gradient, thread = (slot, part)
     double* gpart = m;
@@ -554,12 +578,35 @@ __global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A
         g_row[t] = c >= 0 ? gk[3 * c + q] : 0.0;
     }
     // ---- G: per-atom virial
-    if (tid < 9) {
+    if (tid < 9 && A.Wi) {
         const int a = tid / 3, b = tid - 3 * (tid / 3);
         const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
         double s = 0.0;
         for (int k = 0; k < nv; ++k) s += va[k] * gk[3 * k + b];
         A.Wi[(size_t)row * 9 + tid] = s;
+    }
+}
+
+// One workgroup per row.
+template <typename RadialReal, int NT>
+__global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    atom_row_v2<RadialReal, NT>(P, A, blockIdx.x, lds);
+}
+
+// The rows of a list, walked by a fixed grid (the list length is read on the
+// device, so the list can come from a device-side compaction without a host
+// round trip).  A separate kernel: the loop costs the per-row kernel ~40
+// VGPRs and spills.
+template <int NT>
+__global__ void __launch_bounds__(NT) atom_kernel_v2_list(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    const int n = *A.row_count;
+    for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+        atom_row_v2<double, NT>(P, A, A.rows[idx], lds);
+        __syncthreads();
     }
 }
 
@@ -695,13 +742,28 @@ __global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const do
 }
 
 // ---------------------------------------------------------------------------
+// Raises (never lowers) the dynamic-LDS limit of a variant's kernels: the
+// limit belongs to the kernel function, which every open potential shares.
 hipError_t set_atom_kernel_lds(int variant, size_t bytes)
+{
+    static size_t cur[kNumVariants] = {0, 0, 0};
+    if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
+    if (bytes <= cur[variant]) return hipSuccess;
+    hipError_t e = set_atom_kernel_lds_exact(variant, bytes);
+    if (e == hipSuccess) cur[variant] = bytes;
+    return e;
+}
+
+hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes)
 {
     const int b = (int)bytes;
     switch (variant) {
         case kVariantV2x256: {
             hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 256>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel_v2_list<256>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
             if (e != hipSuccess) return e;
             return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 256>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
@@ -709,6 +771,9 @@ hipError_t set_atom_kernel_lds(int variant, size_t bytes)
         case kVariantV2x512: {
             hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 512>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel_v2_list<512>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
             if (e != hipSuccess) return e;
             return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 512>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
@@ -722,32 +787,43 @@ hipError_t set_atom_kernel_lds(int variant, size_t bytes)
     }
 }
 
-hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
-                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end, int variant)
+hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, bool radial_fp32, hipStream_t stream,
+                        int variant, int grid)
 {
-    if (A.nrows <= 0) {
-        hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, 0, 0, F.cell_rank, F.volume,
-                           F.out);
+    const bool modes = A.xi_atom || A.basis || A.fwd_only || A.rows || A.row_count || !A.Ei;
+    if (grid <= 0) return hipSuccess;
+    if (A.rows || A.row_count) {
+        if (!A.rows || !A.row_count || radial_fp32) return hipErrorInvalidValue;
+        if (variant == kVariantV2x256)
+            hipLaunchKernelGGL(atom_kernel_v2_list<256>, dim3(grid), dim3(256), lds_bytes, stream, P, A);
+        else if (variant == kVariantV2x512)
+            hipLaunchKernelGGL(atom_kernel_v2_list<512>, dim3(grid), dim3(512), lds_bytes, stream, P, A);
+        else
+            return hipErrorInvalidValue;
         return hipGetLastError();
     }
-    if (ev_begin) hipEventRecord(ev_begin, stream);
-    const dim3 grid(A.nrows);
     switch (variant) {
         case kVariantV2x256:
-            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 256>), grid, dim3(256), lds_bytes, stream, P, A);
-            else hipLaunchKernelGGL((atom_kernel_v2<double, 256>), grid, dim3(256), lds_bytes, stream, P, A);
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 256>), dim3(grid), dim3(256), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 256>), dim3(grid), dim3(256), lds_bytes, stream, P, A);
             break;
         case kVariantV2x512:
-            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 512>), grid, dim3(512), lds_bytes, stream, P, A);
-            else hipLaunchKernelGGL((atom_kernel_v2<double, 512>), grid, dim3(512), lds_bytes, stream, P, A);
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 512>), dim3(grid), dim3(512), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 512>), dim3(grid), dim3(512), lds_bytes, stream, P, A);
             break;
         default:
-            if (radial_fp32) hipLaunchKernelGGL(atom_kernel<float>, grid, dim3(kBlock), lds_bytes, stream, P, A);
-            else hipLaunchKernelGGL(atom_kernel<double>, grid, dim3(kBlock), lds_bytes, stream, P, A);
+            if (modes) return hipErrorInvalidValue;  // the per-target kernel has no optional modes
+            if (radial_fp32) hipLaunchKernelGGL(atom_kernel<float>, dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL(atom_kernel<double>, dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
             break;
     }
-    if (ev_end) hipEventRecord(ev_end, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stream)
+{
     const int n = A.nrows;
+    if (n <= 0) return hipSuccess;
     const long long slots = (long long)A.nrows * A.nslots;
     int gs = (int)((slots + 255) / 256);
     if (gs > 65536) gs = 65536;
@@ -758,9 +834,27 @@ hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F
                        F.cnt + n, F.rev);
     hipLaunchKernelGGL(force_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
                        F.off, F.rev, F.F);
-    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, A.nrows, F.n_e, F.cell_rank,
-                       F.volume, F.out);
     return hipGetLastError();
+}
+
+hipError_t launch_totals(const AtomArgs& A, const ForceArgs& F, hipStream_t stream)
+{
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, A.nrows > 0 ? A.nrows : 0,
+                       A.nrows > 0 ? F.n_e : 0, F.cell_rank, F.volume, F.out);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
+                         bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin, hipEvent_t ev_end, int variant)
+{
+    if (A.nrows <= 0) return launch_totals(A, F, stream);
+    if (ev_begin) hipEventRecord(ev_begin, stream);
+    hipError_t e = launch_atoms(P, A, lds_bytes, radial_fp32, stream, variant, A.nrows);
+    if (e != hipSuccess) return e;
+    if (ev_end) hipEventRecord(ev_end, stream);
+    e = launch_gather(A, F, stream);
+    if (e != hipSuccess) return e;
+    return launch_totals(A, F, stream);
 }
 
 }  // namespace mtp

@@ -17,6 +17,14 @@ struct AtomArgs {
     double* Ei;            // [A]      site energies
     double* g;             // [A][K][3] dE_i/dr_ik
     double* Wi;            // [A][9]   per-atom sum_k r_ik (x) g_ik
+    // Optional modes of the balanced-item kernels (zero / null for a plain
+    // force evaluation):
+    const double* xi_atom; // [rows][xi_stride] per-row moment coefficients (file order), replacing xi
+    int xi_stride;         // 0: the same vector for every row
+    double* basis;         // [A][S] basis values m[mapping], written when not null
+    int fwd_only;          // stop after the basis and the energy (g, Wi untouched)
+    const int* rows;       // rows to evaluate (null: 0 .. nrows-1)
+    const int* row_count;  // device-side length of `rows` (null: nrows)
 };
 
 struct ForceArgs {
@@ -34,7 +42,7 @@ struct AtomLds {
 };
 
 struct AtomLds2 {
-    int m, dm, part, f, fp, vx, vy, vz, r, red, ints, total_doubles;
+    int m, dm, part, f, fp, vx, vy, vz, r, red, xl, ints, total_doubles;
 };
 
 // Atom-kernel variants (chosen per potential by the launch autotuner):
@@ -47,10 +55,16 @@ constexpr int kNumVariants = 3;
 constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : 256; }
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
-size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials);
+size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials, int xi_len = 0);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.
+// The atom kernel alone over `grid` workgroups (rows, or the persistent
+// grid that walks A.rows); the Newton-3 gather of A.g into F.F; the totals.
+hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, bool radial_fp32, hipStream_t stream,
+                        int variant, int grid);
+hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stream);
+hipError_t launch_totals(const AtomArgs& A, const ForceArgs& F, hipStream_t stream);
 hipError_t launch_forces(const DevPlan& P, const AtomArgs& A, const ForceArgs& F, size_t lds_bytes,
                          bool radial_fp32, hipStream_t stream, hipEvent_t ev_begin = nullptr,
                          hipEvent_t ev_end = nullptr, int variant = kVariantV1);

@@ -25,6 +25,19 @@ bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err)
     if (f.radial_basis_size > kMaxN) { err = "radial_basis_size above " + std::to_string(kMaxN) + " is not supported"; return false; }
     if (amc > 65535) { err = "alpha_moments_count above 65535 is not supported"; return false; }
 
+    // --- inverse of the scalar mapping (per-row coefficient vectors) -------
+    {
+        std::vector<std::vector<int>> by_x(amc);
+        for (int s = 0; s < f.S(); ++s) by_x[f.alpha_moment_mapping[s]].push_back(s);
+        p.map_beg.push_back(0);
+        for (int x = 0; x < amc; ++x) {
+            if (by_x[x].empty()) continue;
+            p.map_x.push_back(x);
+            p.map_s.insert(p.map_s.end(), by_x[x].begin(), by_x[x].end());
+            p.map_beg.push_back((int)p.map_s.size());
+        }
+    }
+
     // --- basic moments as canonical (mu, nu) blocks -----------------------
     for (auto& row : p.blk_off) for (int& v : row) v = -1;
     for (int& v : p.mu_max_of_nu) v = -1;

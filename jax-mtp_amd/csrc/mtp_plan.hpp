@@ -82,6 +82,12 @@ struct DevPlan {
     const int* rev_bo;               // moment positions without entries (dm = xi)
     const int* rev_xa;               // moments of run items whose dm gets + xi after the sums
     int max_partials;
+    // inverse of alpha_moment_mapping: distinct mapped moments (increasing)
+    // and, per moment, the scalar indices s mapped to it (increasing)
+    int n_mapped;
+    const int* map_x;                // [n_mapped]
+    const int* map_beg;              // [n_mapped + 1]
+    const int* map_s;                // [S]
 };
 
 // Work items of one contraction layer for a workgroup of nt threads (see
@@ -117,6 +123,7 @@ struct HostPlan {
     std::vector<I2> part_blk;
     std::vector<int> part_beg;
     std::vector<U2> fwd_pk, rev_pk;
+    std::vector<int> map_x, map_beg, map_s;
     int blk_off[kMaxR][kMaxNu + 1];
     int mu_max_of_nu[kMaxNu + 1];
 };

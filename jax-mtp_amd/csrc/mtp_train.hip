@@ -395,9 +395,15 @@ __global__ void __launch_bounds__(256) seg_totals_kernel(const double* Ei, const
     }
 }
 
+// Raises (never lowers) the kernel's dynamic-LDS limit, shared by every open
+// potential.
 hipError_t set_train_kernel_lds(size_t bytes)
 {
-    return hipFuncSetAttribute((const void*)train_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    static size_t cur = 0;
+    if (bytes <= cur) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void*)train_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) cur = bytes;
+    return e;
 }
 
 hipError_t launch_train_grad(const DevPlan& P, const TrainArgs& A, size_t lds_bytes, double* scratch_partial,

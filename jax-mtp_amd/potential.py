@@ -162,6 +162,71 @@ class MTPPotential:
         return E, F, stress, dict(virial6=out[7:13], eatom=eatom, status=out[13], g=g)
 
     # ------------------------------------------------------------------
+    # Pieces of the atom kernel (device arrays in, torch tensors out); the
+    # distilled head and its training are built from these.
+    def _rows(self, itypes, all_rijs, all_jtypes):
+        import torch
+
+        dev = torch.device("cuda")
+        it, rij, jt = _i32(itypes, dev), _f64(all_rijs, dev), _i32(all_jtypes, dev)
+        A, K = jt.shape
+        if rij.shape != (A, K, 3) or it.shape != (A,):
+            raise ValueError("inconsistent padded shapes")
+        return it, rij, jt, A, K, torch.cuda.current_stream().cuda_stream
+
+    def descriptor(self, itypes, all_rijs, all_jtypes):
+        """Basis values b[A, S] (m[alpha_moment_mapping]) of every row."""
+        import torch
+
+        it, rij, jt, A, K, st = self._rows(itypes, all_rijs, all_jtypes)
+        S = len(self.params.moment_coeffs)
+        b = torch.empty((A, S), dtype=torch.float64, device=rij.device)
+        capi.check(capi.lib().mtp_descriptor_device(self.ctx, A, K, it.data_ptr(), rij.data_ptr(), jt.data_ptr(),
+                                                    b.data_ptr(), st), self.ctx)
+        return b
+
+    def slot_grad_xi(self, itypes, all_rijs, all_jtypes, xi, with_virial=False):
+        """Slot gradients g[A, K, 3] of xi_row . b_row for per-row xi [A, S] (or one xi [S])."""
+        import torch
+
+        it, rij, jt, A, K, st = self._rows(itypes, all_rijs, all_jtypes)
+        x = _f64(xi, rij.device)
+        S = len(self.params.moment_coeffs)
+        stride = 0 if x.dim() == 1 else S
+        if x.shape[-1] != S or (x.dim() == 2 and x.shape[0] != A):
+            raise ValueError("xi must be [S] or [A, S]")
+        g = torch.empty((A, K, 3), dtype=torch.float64, device=rij.device)
+        W = torch.empty((A, 9), dtype=torch.float64, device=rij.device) if with_virial else None
+        capi.check(capi.lib().mtp_slot_grad_xi_device(
+            self.ctx, A, K, it.data_ptr(), rij.data_ptr(), jt.data_ptr(), x.data_ptr(), stride, g.data_ptr(),
+            W.data_ptr() if W is not None else None, st), self.ctx)
+        return (g, W) if with_virial else g
+
+    def basis_jacobian(self, itypes, all_rijs, all_jtypes):
+        """Dense J[A, S, K, 3] = d b / d r."""
+        import torch
+
+        it, rij, jt, A, K, st = self._rows(itypes, all_rijs, all_jtypes)
+        S = len(self.params.moment_coeffs)
+        J = torch.empty((A, S, K, 3), dtype=torch.float64, device=rij.device)
+        capi.check(capi.lib().mtp_basis_jacobian_device(self.ctx, A, K, it.data_ptr(), rij.data_ptr(),
+                                                        jt.data_ptr(), J.data_ptr(), st), self.ctx)
+        return J
+
+    def gather_forces(self, all_js, g, natoms_force):
+        """Newton-3 gather (jax_backend.py:263-303) of slot gradients g[A, K, 3]."""
+        import torch
+
+        dev = torch.device("cuda")
+        js, gg = _i32(all_js, dev), _f64(g, dev)
+        A, K = js.shape
+        F = torch.empty((A, 3), dtype=torch.float64, device=dev)
+        capi.check(capi.lib().mtp_gather_forces_device(self.ctx, A, K, js.data_ptr(), gg.data_ptr(),
+                                                       int(natoms_force), F.data_ptr(),
+                                                       torch.cuda.current_stream().cuda_stream), self.ctx)
+        return F
+
+    # ------------------------------------------------------------------
     def compute_rows(self, natoms, rij_rows, types, nbr_rows, nbr_counts, nbr_type_rows, volume):
         """The pair style's call (zero_overhead.hpp:58-64) from numpy row arrays.
 

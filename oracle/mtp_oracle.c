@@ -149,8 +149,9 @@ int orc_compact_basis_rb(const orc_mtp *P, int nn, const double *u, const double
  * coefficient gradient dc[R][N] of this atom's species pair rows
  * (accumulated into dc_full[Sp][Sp][R][N] when given). */
 static void compact_atom(const orc_mtp *P, int K, const double *rij, int ti, const int *tk,
-                         double *Ei, double *g, double *basis, double *dc_full, double *work)
+                         double *Ei, double *g, double *basis, double *dc_full, double *work, const double *xi)
 {
+    if (!xi) xi = P->moment_coeffs;
     int R = P->R, N = P->N, amc = P->amc;
     int mp = max_pow_of(P);
     double *m = work;                 /* amc */
@@ -185,7 +186,7 @@ static void compact_atom(const orc_mtp *P, int K, const double *rij, int ti, con
     }
     double E = P->species_coeffs[wrap_species(ti, P->Sp)];
     for (int s = 0; s < P->S; ++s) {
-        E += P->moment_coeffs[s] * m[P->mapping[s]];
+        E += xi[s] * m[P->mapping[s]];
         if (basis) basis[s] = m[P->mapping[s]];
     }
     *Ei = E;
@@ -193,7 +194,7 @@ static void compact_atom(const orc_mtp *P, int K, const double *rij, int ti, con
 
     /* reverse pass over the contraction table, reverse file order */
     memset(dm, 0, sizeof(double) * amc);
-    for (int s = 0; s < P->S; ++s) dm[P->mapping[s]] += P->moment_coeffs[s];
+    for (int s = 0; s < P->S; ++s) dm[P->mapping[s]] += xi[s];
     for (int e = P->T - 1; e >= 0; --e) {
         const int *t = P->times + 4 * e;
         double w = t[2] * dm[t[3]];
@@ -264,7 +265,7 @@ int orc_compact_atoms(const orc_mtp *P, int A, int K, const int *itypes, const d
 #endif
             for (int i = 0; i < A; ++i)
                 compact_atom(P, K, rij + (size_t)i * K * 3, itypes[i], jtypes + (size_t)i * K, Ei + i,
-                             g ? g + (size_t)i * K * 3 : NULL, NULL, NULL, work);
+                             g ? g + (size_t)i * K * 3 : NULL, NULL, NULL, work, NULL);
             free(work);
         }
     }
@@ -281,9 +282,43 @@ int orc_compact_basis_and_dc(const orc_mtp *P, int A, int K, const int *itypes, 
     if (!work) return -1;
     for (int i = 0; i < A; ++i)
         compact_atom(P, K, rij + (size_t)i * K * 3, itypes[i], jtypes + (size_t)i * K, Ei + i, NULL,
-                     basis ? basis + (size_t)i * P->S : NULL, dc_full, work);
+                     basis ? basis + (size_t)i * P->S : NULL, dc_full, work, NULL);
     free(work);
     return 0;
+}
+
+/* Per-row coefficient vectors xi_row = xi + i * xi_stride in place of
+ * moment_coeffs: Ei = species_coeff + xi_row . b_i, g = its slot gradient,
+ * basis (may be NULL) = b_i.  Basis Jacobians and the distilled head's
+ * forces are built from this (the reference has no counterpart). */
+int orc_compact_atoms_xi(const orc_mtp *P, int A, int K, const int *itypes, const double *rij, const int *jtypes,
+                         const double *xi, int xi_stride, double *Ei, double *g, double *basis, int nthreads)
+{
+    if (P->R > 16 || P->N > 64) return -2;
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+    {
+        double *work = (double *)malloc(sizeof(double) * compact_work_doubles(P, K));
+        if (!work) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+            err = -1;
+        } else {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+            for (int i = 0; i < A; ++i)
+                compact_atom(P, K, rij + (size_t)i * K * 3, itypes[i], jtypes + (size_t)i * K, Ei + i,
+                             g ? g + (size_t)i * K * 3 : NULL, basis ? basis + (size_t)i * P->S : NULL, NULL, work,
+                             xi + (size_t)i * xi_stride);
+            free(work);
+        }
+    }
+    return err;
 }
 
 /* ------------------------------------------------------------------ */

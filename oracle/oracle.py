@@ -237,6 +237,7 @@ def lib():
         I = ctypes.c_int
         L.orc_compact_atoms.argtypes = [P, I, I, P, P, P, P, P, I]
         L.orc_compact_basis_and_dc.argtypes = [P, I, I, P, P, P, P, P, P]
+        L.orc_compact_atoms_xi.argtypes = [P, I, I, P, P, P, P, I, P, P, P, I]
         L.orc_compact_basis_rb.argtypes = [P, I, P, P, I, P]
         L.orc_tree_build.argtypes = [I, P, P, P, P, P, P, P, P, I, P]
         L.orc_tree_build.restype = P
@@ -343,6 +344,34 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_compact_atoms failed ({rc})")
         return Ei, g
+
+    def compact_xi(self, itypes, rij, jtypes, xi, grad=True, nthreads=0):
+        """Per-row coefficient vectors xi[A][S] (or one vector [S]) in place of
+        moment_coeffs: (Ei, g, basis)."""
+        itypes = _i32(itypes); rij = _f64(rij); jtypes = _i32(jtypes); xi = _f64(xi)
+        A, K = jtypes.shape
+        S = len(self._keep["mapping"])
+        stride = 0 if xi.ndim == 1 else S
+        assert xi.shape[-1] == S and (xi.ndim == 1 or xi.shape[0] == A)
+        Ei = np.zeros(A)
+        g = np.zeros((A, K, 3)) if grad else None
+        basis = np.zeros((A, S))
+        rc = lib().orc_compact_atoms_xi(ctypes.byref(self.c), A, K, _p(itypes), _p(rij), _p(jtypes), _p(xi), stride,
+                                        _p(Ei), _p(g) if grad else None, _p(basis), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_compact_atoms_xi failed ({rc})")
+        return Ei, g, basis
+
+    def basis_jacobian(self, itypes, rij, jtypes, nthreads=0):
+        """J[A][S][K][3] = d basis / d r (one per-row-coefficient pass per s)."""
+        A, K = np.asarray(jtypes).shape
+        S = len(self._keep["mapping"])
+        J = np.zeros((A, S, K, 3))
+        for s in range(S):
+            e = np.zeros(S)
+            e[s] = 1.0
+            J[:, s] = self.compact_xi(itypes, rij, jtypes, e, nthreads=nthreads)[1]
+        return J
 
     def basis_and_dc(self, itypes, rij, jtypes):
         """Per-atom basis [A,S] and d(sum_i E_i)/d radial_coeffs."""

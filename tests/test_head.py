@@ -116,8 +116,8 @@ def test_param_grad_central_differences():
         assert g[q] == pytest.approx(fd, rel=1e-7, abs=1e-9 * np.abs(g).max())
 
 
-def _level12_case(jm, seed=0, natoms=24):
-    w = jm.configs.small_cluster(12, SP, natoms, seed, cell=7.5, cutoff=5.0, max_nbr=64, radial_basis_size=8)
+def _level12_case(jm, seed=0, natoms=24, cell=7.5):
+    w = jm.configs.small_cluster(12, SP, natoms, seed, cell=cell, cutoff=5.0, max_nbr=64, radial_basis_size=8)
     nb = w.neighbors()
     return w, nb
 
@@ -180,7 +180,7 @@ def _rel(a, r):
 def test_gpu_descriptor_jacobian_gather(jm, O):
     import torch
 
-    w, nb = _level12_case(jm, seed=5, natoms=40)
+    w, nb = _level12_case(jm, seed=5, natoms=40, cell=8.2)
     print(f"[case] level-12 descriptor natoms={w.natoms}", flush=True)
     orc = oracle_for(O, w.params)
     pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=w.max_nbr)
@@ -233,7 +233,7 @@ def test_gpu_head_network_and_gradient(jm, O):
 def test_gpu_head_compute_with_fallback(jm, O, exact_level):
     import head_oracle as HO
 
-    w, nb = _level12_case(jm, seed=7, natoms=48)
+    w, nb = _level12_case(jm, seed=7, natoms=48, cell=8.6)
     print(f"[case] head compute natoms={w.natoms} exact level {exact_level}", flush=True)
     rng = np.random.default_rng(7)
     model = _model(rng, box=False)

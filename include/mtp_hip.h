@@ -101,11 +101,14 @@ int mtp_sync(mtp_ctx *ctx, void *stream);
 
 /* Atom-kernel variants (no reference counterpart; XLA chose its own
  * fusion): 0 = one thread per target moment, 256 threads per atom;
- * 1 / 2 = balanced work items with 256 / 512 threads per atom.
- * mtp_open picks the widest variant whose LDS fits (env MTP_KERNEL_VARIANT
- * overrides).  mtp_autotune times every available variant on the given
- * device arrays (reps launches each, after one warm-up), selects the
- * fastest for ctx and reports the per-variant ms (-1 = unavailable). */
+ * 1 / 2 / 3 = balanced work items with 256 / 512 / 64 threads per atom.
+ * mtp_open picks variant 3 for small contraction tables (<= 400 entries,
+ * levels up to 14) and otherwise the widest variant whose LDS fits (env
+ * MTP_KERNEL_VARIANT overrides).  mtp_autotune times every available variant
+ * on the given device arrays (reps launches each, after one warm-up),
+ * selects the fastest for ctx and reports the per-variant ms in
+ * ms_per_variant[MTP_NUM_VARIANTS] (-1 = unavailable). */
+#define MTP_NUM_VARIANTS 4
 int mtp_set_variant(mtp_ctx *ctx, int variant);
 int mtp_get_variant(const mtp_ctx *ctx, int *variant, long long *lds_bytes);
 int mtp_autotune(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js, const double *d_rijs,
@@ -185,7 +188,10 @@ int mtp_gather_forces_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_j
  * high-level potential `exact`, which re-evaluates every atom whose
  * normalised basis values leave the training box [lo, hi] (per species).
  * Network: z = [(b - mean) * inv_std, onehot(type)] (S + Sp <= 32),
- * 64 SiLU, 64 SiLU, E_i = e_shift[type] + e_scale * out.  Parameters theta
+ * 64 SiLU, 64 SiLU, E_i = e_shift[type] + e_scale * out; with residual != 0
+ * the descriptor potential's own energy species_coeff + moment_coeffs . b
+ * is added (the network corrects the low-level potential; its current
+ * coefficients are used).  Parameters theta
  * (mtp_head_theta_size() doubles): W1[32][64] b1[64] W2[64][64] b2[64]
  * w3[64] b3, weights stored (in, out). */
 typedef struct mtp_head mtp_head;
@@ -194,7 +200,8 @@ int mtp_head_theta_size(void);
 /* exact may be NULL (no fallback); lo/hi [Sp][S] may both be NULL (no box).
  * Both potentials must stay open while the head is. */
 int mtp_head_open(mtp_ctx *desc, mtp_ctx *exact, const double *theta, const double *mean, const double *inv_std,
-                  const double *lo, const double *hi, const double *e_shift, double e_scale, mtp_head **out);
+                  const double *lo, const double *hi, const double *e_shift, double e_scale, int residual,
+                  mtp_head **out);
 void mtp_head_close(mtp_head *h);
 const char *mtp_head_last_error(const mtp_head *h);
 /* theta from host (on_device 0, synchronous) or device memory (stream-ordered). */

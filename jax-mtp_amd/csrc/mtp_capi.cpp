@@ -48,8 +48,8 @@ struct mtp_ctx {
     HostPlan plan;
     DevPlan dev{};                        // variant 0 (and the base of the others)
     DevPlan vdev[kNumVariants]{};         // per-variant device plans
-    size_t vlds[kNumVariants] = {0, 0, 0};  // LDS bytes per variant, 0 = unavailable
-    size_t vlds_xi[kNumVariants] = {0, 0, 0};  // the same with per-row coefficient vectors
+    size_t vlds[kNumVariants] = {};          // LDS bytes per variant, 0 = unavailable
+    size_t vlds_xi[kNumVariants] = {};       // the same with per-row coefficient vectors
     int n_cu = 0;
     double* d_eye = nullptr;              // [S][S] identity (basis Jacobians), allocated on first use
     int variant = kVariantV1;
@@ -259,22 +259,24 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->vdev[kVariantV1] = d;
     c->vlds[kVariantV1] = c->lds_bytes;
     {
-        HostPlan p8;
-        std::string e8;
-        build_plan(f, 8, p8, e8);  // same validated file: cannot fail
-        I2* part_blk8 = c->alloc<I2>(p8.part_blk.size(), e);
-        if (e == hipSuccess) e = upload(part_blk8, p8.part_blk, c->stream);
-        for (int v : {kVariantV2x256, kVariantV2x512}) {
+        for (int v : {kVariantV2x256, kVariantV2x512, kVariantV2x64}) {
             const int nt = variant_threads(v);
-            const HostPlan& pv = v == kVariantV2x512 ? p8 : p;
-            ItemPlan ip;
-            build_items(pv, nt, ip);
+            HostPlan pp;
+            const HostPlan* pv = &p;
             DevPlan dv = d;
-            if (v == kVariantV2x512) {
-                dv.n_parts = p8.n_parts;
-                for (int q = 0; q < 9; ++q) dv.part_beg[q] = p8.part_beg[q];
-                dv.part_blk = part_blk8;
+            if (variant_parts(v) != p.n_parts) {  // another split of the gradient blocks
+                std::string ee;
+                build_plan(f, variant_parts(v), pp, ee);  // same validated file: cannot fail
+                pv = &pp;
+                I2* pb = c->alloc<I2>(pp.part_blk.size(), e);
+                if (e == hipSuccess) e = upload(pb, pp.part_blk, c->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                dv.n_parts = pp.n_parts;
+                for (int q = 0; q < 9; ++q) dv.part_beg[q] = pp.part_beg[q];
+                dv.part_blk = pb;
             }
+            ItemPlan ip;
+            build_items(*pv, nt, ip);
             auto pack = [&](const std::vector<LayerItems>& layers, const std::vector<U2>& table, int* it_beg,
                             int* cb_beg, int* tbl_base, int* bo_beg, int* xa_beg, const I4** d_items,
                             const I4** d_comb, const U2** d_tbl, const int** d_bo, const int** d_xa) {
@@ -315,6 +317,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                  &dv.fwd_items, &dv.fwd_comb, &dv.fwd_tbl, nullptr, nullptr);
             pack(ip.rev, ip.rev_table, dv.rev_it_beg, dv.rev_cb_beg, dv.rev_tbl_base, dv.rev_bo_beg, dv.rev_xa_beg,
                  &dv.rev_items, &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo, &dv.rev_xa);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host tables die with this iteration
             dv.max_partials = ip.max_partials;
             const size_t lds = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
                                                dv.n_parts, nt, ip.max_partials);
@@ -332,8 +335,10 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             delete c;
             return MTP_ERR_HIP;
         }
-        // default: the widest balanced variant that fits; MTP_KERNEL_VARIANT overrides
+        // default: one wavefront per atom for small contraction tables (levels <= 14), else the widest balanced
+        // variant that fits; mtp_autotune measures instead, MTP_KERNEL_VARIANT overrides
         c->variant = c->vlds[kVariantV2x512] ? kVariantV2x512 : (c->vlds[kVariantV2x256] ? kVariantV2x256 : kVariantV1);
+        if (f.T() <= 400 && c->vlds[kVariantV2x64]) c->variant = kVariantV2x64;
         if (const char* ev = std::getenv("MTP_KERNEL_VARIANT")) {
             const int v = std::atoi(ev);
             if (v >= 0 && v < kNumVariants && c->vlds[v]) c->variant = v;
@@ -920,7 +925,8 @@ const char* mtp_head_last_error(const mtp_head* h) { return h ? h->err.c_str() :
 int mtp_head_theta_size(void) { return kHeadTheta; }
 
 int mtp_head_open(mtp_ctx* desc, mtp_ctx* exact, const double* theta, const double* mean, const double* inv_std,
-                  const double* lo, const double* hi, const double* e_shift, double e_scale, mtp_head** out)
+                  const double* lo, const double* hi, const double* e_shift, double e_scale, int residual,
+                  mtp_head** out)
 {
     if (!out) { set_global_error("mtp_head_open: out is NULL"); return MTP_ERR_ARG; }
     *out = nullptr;
@@ -971,6 +977,8 @@ int mtp_head_open(mtp_ctx* desc, mtp_ctx* exact, const double* theta, const doub
     HeadNet& n = h->net;
     n.S = S; n.Sp = Sp; n.theta = h->d_theta; n.wt = h->d_wt; n.mean = d_mean; n.inv_std = d_inv;
     n.lo = d_lo; n.hi = d_hi; n.e_shift = d_shift; n.e_scale = e_scale;
+    n.lin = residual ? desc->d_moment : nullptr;
+    n.lin_sc = residual ? desc->d_species : nullptr;
     if (e == hipSuccess) e = head_prepare(n, h->d_wt, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {

@@ -140,10 +140,16 @@ __global__ void __launch_bounds__(1024) head_eval_kernel(const HeadNet Hn, int n
             dz += L.W1T[(half * (H / 2) + l) * IN + i] * (half ? d1 : d0);
         }
         dz += __shfl_xor(dz, IN);
-        if (lane < Hn.S) xi[(size_t)a * Hn.S + lane] = Hn.e_scale * dz * Hn.inv_std[lane];
+        double lin_e = 0.0;
+        if (Hn.lin) {  // residual form: + species energy + lin . b
+            const double bl = lane < Hn.S ? basis[(size_t)a * Hn.S + lane] : 0.0;
+            lin_e = wave_sum(lane < Hn.S ? Hn.lin[lane] * bl : 0.0) + Hn.lin_sc[ts];
+        }
+        if (lane < Hn.S)
+            xi[(size_t)a * Hn.S + lane] = Hn.e_scale * dz * Hn.inv_std[lane] + (Hn.lin ? Hn.lin[lane] : 0.0);
         const bool any_out = __ballot(out) != 0;
         if (lane == 0) {
-            E[a] = Hn.e_shift[ts] + Hn.e_scale * net;
+            E[a] = Hn.e_shift[ts] + Hn.e_scale * net + lin_e;
             if (flags) flags[a] = any_out ? 1 : 0;
             if (count && any_out && t >= 0) rows[atomicAdd(count, 1)] = a;
         }

@@ -8,6 +8,8 @@
 //   a1   = W1^T z + b1,  h1 = silu(a1)                        (64)
 //   a2   = W2^T h1 + b2, h2 = silu(a2)                        (64)
 //   E_i  = e_shift[t_i] + e_scale * (w3 . h2 + b3)
+//          [+ lin_sc[t_i] + lin . b_i   in residual form: the network then
+//           corrects the low-level potential's own linear energy]
 // Flat parameter vector theta (weights stored (in, out), row major):
 //   W1[32][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3
 // Rows of W1 beyond S + Sp multiply zero inputs.
@@ -30,6 +32,8 @@ struct HeadNet {
     const double* hi;       // [Sp][S]
     const double* e_shift;  // [Sp]
     double e_scale;
+    const double* lin;      // [S] linear coefficients (residual form) or null
+    const double* lin_sc;   // [Sp] species energies of the linear part
 };
 
 // theta -> transposed copies used by the backward passes.

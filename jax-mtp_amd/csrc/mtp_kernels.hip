@@ -411,8 +411,39 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const double* rij_row = A.rij + (size_t)row * K * 3;
     const int* jt_row = A.jtypes + (size_t)row * K;
 
-    // ---- A: radial basis, compaction of valid slots in slot order (K <= NT)
-    {
+    // ---- A: radial basis, compaction of valid slots in slot order
+    int nv = 0;
+    if constexpr (NT < kAtomBlock) {  // slots in chunks of NT (K may exceed NT)
+        for (int k0 = 0; k0 < K; k0 += NT) {
+            const int k = k0 + tid;
+            double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+            bool valid = false;
+            if (k < K) {
+                v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+                r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                valid = (jt_row[k] >= 0) && (r < P.max_dist);
+            }
+            const uint64_t mask = __ballot(valid);
+            const int lane = tid & (kWave - 1), wave = tid / kWave;
+            if (lane == 0) wave_cnt[wave] = __popcll(mask);
+            __syncthreads();
+            int off = nv;
+            for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+            const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+            if (k < K) cidx[k] = valid ? pos : -1;
+            if (valid) {
+                double fv[kMaxR], dfv[kMaxR];
+                radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
+#pragma unroll
+                for (int mu = 0; mu < kMaxR; ++mu)
+                    if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
+                vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
+                slot_of[pos] = k;
+            }
+            for (int w = 0; w < NW; ++w) nv += wave_cnt[w];
+            __syncthreads();
+        }
+    } else {  // K <= NT: one slot per thread
         const int k = tid;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
         bool valid = false;
@@ -444,8 +475,8 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             *nvp = tot;
         }
         __syncthreads();
+        nv = *nvp;
     }
-    const int nv = *nvp;
     for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;  // moments that are never targets stay 0
 
     // ---- B: basic moments (power table in the dm region)
@@ -746,7 +777,7 @@ __global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const do
 // limit belongs to the kernel function, which every open potential shares.
 hipError_t set_atom_kernel_lds(int variant, size_t bytes)
 {
-    static size_t cur[kNumVariants] = {0, 0, 0};
+    static size_t cur[kNumVariants] = {};
     if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
     if (bytes <= cur[variant]) return hipSuccess;
     hipError_t e = set_atom_kernel_lds_exact(variant, bytes);
@@ -778,6 +809,16 @@ hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes)
             return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 512>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
         }
+        case kVariantV2x64: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 64>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel_v2_list<64>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 64>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
         default: {
             hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
@@ -798,6 +839,8 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
             hipLaunchKernelGGL(atom_kernel_v2_list<256>, dim3(grid), dim3(256), lds_bytes, stream, P, A);
         else if (variant == kVariantV2x512)
             hipLaunchKernelGGL(atom_kernel_v2_list<512>, dim3(grid), dim3(512), lds_bytes, stream, P, A);
+        else if (variant == kVariantV2x64)
+            hipLaunchKernelGGL(atom_kernel_v2_list<64>, dim3(grid), dim3(64), lds_bytes, stream, P, A);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();
@@ -810,6 +853,10 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
         case kVariantV2x512:
             if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 512>), dim3(grid), dim3(512), lds_bytes, stream, P, A);
             else hipLaunchKernelGGL((atom_kernel_v2<double, 512>), dim3(grid), dim3(512), lds_bytes, stream, P, A);
+            break;
+        case kVariantV2x64:
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 64>), dim3(grid), dim3(64), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 64>), dim3(grid), dim3(64), lds_bytes, stream, P, A);
             break;
         default:
             if (modes) return hipErrorInvalidValue;  // the per-target kernel has no optional modes

@@ -47,12 +47,16 @@ struct AtomLds2 {
 
 // Atom-kernel variants (chosen per potential by the launch autotuner):
 // v1 = 256 threads, one thread per target moment; v2 = balanced work items
-// with 256 or 512 threads per atom.
+// with 256, 512 or 64 threads per atom (one wavefront: for low levels, where
+// a wide workgroup per atom leaves the CU waiting on one atom's barriers).
 constexpr int kVariantV1 = 0;
 constexpr int kVariantV2x256 = 1;
 constexpr int kVariantV2x512 = 2;
-constexpr int kNumVariants = 3;
-constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : 256; }
+constexpr int kVariantV2x64 = 3;
+constexpr int kNumVariants = 4;
+constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : (v == kVariantV2x64 ? 64 : 256); }
+// gradient parts (thread groups over the basic-moment blocks) per variant
+constexpr int variant_parts(int v) { return v == kVariantV2x512 ? 8 : (v == kVariantV2x64 ? 1 : 4); }
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
 size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials, int xi_len = 0);

@@ -1,0 +1,224 @@
+"""Distillation of a level-24 MTP into the head, and its evaluation.
+
+Workload (the distilled-head configuration): a level-24, 3-species MTP is
+the teacher; the head reads level-12 basis values (29 per atom) through a
+2x64 SiLU network; it is trained on 1e5 site environments of thermally
+displaced 3-species FCC snapshots at 300-1500 K and evaluated on a
+100,000-atom 3-species FCC cell, with exact level-24 re-evaluation of every
+atom outside the training box.  Reported: atom-steps/s of the head (with
+fallback) and of the plain level-24 evaluation, the head's force RMSE
+against the level-24 forces, and the same RMSE for the level-12 MTP alone
+(the method without the network).
+
+Synthetic teacher.  All parameters are seeded; there is no trained
+level-24 potential to start from.  Coefficients drawn N(0, 0.1) for every
+basis function (as in the parity workloads) give energies of 1e3-1e4 eV per
+atom and forces of 1e5 eV/A, which is no physical target for a force error
+in meV/A.  The teacher therefore normalises each scalar basis function by
+its RMS over 300 K snapshots and damps it with its MTP level (level of
+M_{mu,nu} = 2 + 4 mu + nu, levels add under contraction), a converging
+body-order expansion, then scales everything so that the RMS force at
+300 K is 0.3 eV/A.  The random-parameter teacher has no physical
+equilibrium, so snapshots are Gaussian thermal displacements of the
+lattice, sigma(T) = 0.05 A sqrt(T / 300 K) per component, rather than MD
+frames.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import configs, training
+from .head import DistilledHead, HeadTrainer
+from .mtp_io import MTPData, untrained
+from .potential import MTPPotential
+
+A_FCC = 3.6
+CUTOFF = 5.5
+MAX_NBR = 80
+FRACTIONS = (0.4, 0.3, 0.3)
+
+
+def basis_levels(p: MTPData) -> np.ndarray:
+    """MTP level of every alpha moment: 2 + 4 mu + nu for basic moments, sums for products."""
+    lev = np.zeros(p.alpha_moments_count, dtype=np.int64)
+    for b, (mu, a, bb, c) in enumerate(p.alpha_index_basic):
+        lev[b] = 2 + 4 * mu + a + bb + c
+    for i1, i2, _, i3 in p.alpha_index_times:
+        lev[i3] = lev[i1] + lev[i2]
+    return lev
+
+
+def thermal_snapshots(n_env: int, ncell=(4, 4, 4), T_range=(300.0, 1500.0), sigma300=0.05, seed=0):
+    """FCC snapshots (pos, types, cell, T) with at least n_env atoms in total."""
+    rng = np.random.default_rng(seed)
+    base, cell = configs.fcc(A_FCC, ncell)
+    out = []
+    total = 0
+    while total < n_env:
+        T = float(rng.uniform(*T_range))
+        pos = base + rng.normal(0.0, sigma300 * np.sqrt(T / 300.0), base.shape)
+        types = configs.species_by_fraction(len(base), FRACTIONS, rng)
+        out.append((pos, types, cell, T))
+        total += len(base)
+    return out
+
+
+def inference_cell(ncell=(25, 25, 40), T=900.0, sigma300=0.05, seed=1):
+    """The 100,000-atom 3-species FCC cell."""
+    rng = np.random.default_rng(seed)
+    pos, cell = configs.fcc(A_FCC, ncell)
+    pos = pos + rng.normal(0.0, sigma300 * np.sqrt(T / 300.0), pos.shape)
+    types = configs.species_by_fraction(len(pos), FRACTIONS, rng)
+    return configs.Workload(f"fcc{len(pos)}_head", None, pos, types, cell, CUTOFF, MAX_NBR)
+
+
+def teacher_params(seed=0, decay=0.25, f_rms=0.3, calib_cells=8) -> MTPData:
+    """Level-24, 3-species teacher with physical force scale (see module doc)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    p = configs.random_params(24, 3, 10, 2.0, CUTOFF, rng)
+    lev = basis_levels(p)[np.asarray(p.alpha_moment_mapping)]
+    raw = rng.normal(0.0, 1.0, len(lev)) * decay ** ((lev - 2) / 2.0)
+    snaps = thermal_snapshots(calib_cells * 256, T_range=(300.0, 300.0), seed=seed + 100)
+    batch = training.pack_configs([s[:3] for s in snaps], CUTOFF, MAX_NBR)
+    pot = MTPPotential(p, max_atoms=batch.natoms, max_nbr=MAX_NBR)
+    b = pot.descriptor(batch.itypes, batch.rij, batch.jtypes)
+    rms = torch.sqrt((b ** 2).mean(0)).cpu().numpy()
+    xi = np.where(rms > 0, raw / np.maximum(rms, 1e-300), 0.0)
+    pot.set_params(moment_coeffs=xi)
+    _, F, _ = pot.calc_energy_forces_stress_padded(batch.itypes, batch.js, batch.rij, batch.jtypes, 3, 1.0,
+                                                   batch.natoms, batch.natoms)
+    scale = f_rms / float(torch.sqrt((F ** 2).mean()))
+    pot.close()
+    p.moment_coeffs = xi * scale
+    p.potential_name = "synthetic_level24_teacher"
+    return p
+
+
+def label(pot: MTPPotential, batch):
+    """Site energies [N], forces [N, 3], per-configuration energies and stresses."""
+    _, F, _, extra = pot.calc_energy_forces_stress_padded(batch.itypes, batch.js, batch.rij, batch.jtypes, 3, 1.0,
+                                                          batch.natoms, batch.natoms, return_all=True)
+    E_c, F_b, S_c = training.batch_forward(pot, batch.to_device())
+    return extra["eatom"].clone(), F.clone(), E_c.clone(), S_c.clone()
+
+
+def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None):
+    """Level-12 MTP over the teacher's own low-level invariants.
+
+    The level-12 basis uses radial functions mu < 3 (level 2 + 4 mu <= 12);
+    they are taken from the teacher, so the head's inputs are exactly the
+    teacher's level-12 basis values.  Moment coefficients come from linear
+    least squares (then, if steps > 0, L-BFGS over all coefficients).  Alone,
+    this potential is the method without the network."""
+    rng = np.random.default_rng(seed)
+    p = configs.random_params(12, teacher.species_count, teacher.radial_basis_size, teacher.min_dist,
+                              teacher.max_dist, rng, scaling=teacher.scaling)
+    R = p.radial_funcs_count
+    p.radial_coeffs = np.array(teacher.radial_coeffs)[:, :, :R, :].copy()
+    p.species_coeffs = np.array(teacher.species_coeffs).copy()
+    p.potential_name = "level12_descriptor"
+    pot = MTPPotential(p, max_atoms=batch.natoms, max_nbr=MAX_NBR)
+    tr = training.Trainer(pot, batch, E_c, F, S_c, weights=(1.0, 1.0, 0.0))
+    hist = [float(tr.loss().detach())]
+    for _ in range(steps):
+        hist.append(tr.step())
+    th = tr.theta.detach().cpu().numpy()
+    S = len(p.moment_coeffs)
+    sp = p.species_count
+    pot.set_params(th[:S], th[S:S + sp], th[S + sp:])
+    if log:
+        log(f"[distill] level-12 fit loss {hist[0]:.4e} -> {hist[-1]:.4e}")
+    return pot, hist
+
+
+def _time(fn, reps=5, warmup=2):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        fn()
+    t1.record()
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / reps
+
+
+def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print):
+    import torch
+
+    t_start = time.time()
+    teacher = teacher_params(seed, decay=decay)
+    snaps = thermal_snapshots(n_env, seed=seed + 1)
+    batch = training.pack_configs([s[:3] for s in snaps], CUTOFF, MAX_NBR)
+    log(f"[distill] {len(snaps)} snapshots, {batch.natoms} environments, K={batch.nslots}")
+    tpot = MTPPotential(teacher, max_atoms=batch.natoms, max_nbr=MAX_NBR)
+    E_site, F, E_c, S_c = label(tpot, batch)
+    tpot.close()
+    desc, hist_d = fit_descriptor(teacher, batch, E_c, F, S_c, seed=seed, steps=desc_steps, log=log)
+    tr = HeadTrainer(desc, batch, E_site, F, weights=(1.0, 1.0), margin=0.05, seed=seed)
+    t0 = time.time()
+    hist = tr.fit(steps=head_steps, lr=3e-3, log_every=max(1, head_steps // 10))
+    torch.cuda.synchronize()
+    t_fit = time.time() - t0
+    e_rmse, f_rmse = tr.rmse()
+    log(f"[distill] head fit {t_fit:.1f} s: loss {hist[0][1]:.4e} -> {hist[-1][1]:.4e}; train RMSE E "
+        f"{e_rmse * 1e3:.2f} meV/atom, F {f_rmse * 1e3:.2f} meV/A")
+    model = tr.model
+    tr.head.close()
+    desc.close()
+    del tr
+    torch.cuda.empty_cache()
+
+    # inference on the large cell
+    w = inference_cell(infer_ncell, seed=seed + 2)
+    nb = w.neighbors()
+    dev = torch.device("cuda")
+    it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
+    js = torch.as_tensor(nb["js"], dtype=torch.int32, device=dev)
+    rij = torch.as_tensor(nb["rij"], dtype=torch.float64, device=dev)
+    jt = torch.as_tensor(nb["jtypes"], dtype=torch.int32, device=dev)
+    A = w.natoms
+    exact = MTPPotential(teacher, max_atoms=A, max_nbr=MAX_NBR)
+    exact.autotune(it, js, rij, jt)
+    head = DistilledHead(model, exact, max_atoms=A, max_nbr=MAX_NBR)
+    head.desc.autotune(it, js, rij, jt)
+    lin = head.desc  # the fitted level-12 MTP itself
+    Ex, Fx, _ = exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
+    Eh, Fh, _, extra = head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
+    El, Fl, _ = lin.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
+    rm = lambda a, b: float(torch.sqrt(((a - b) ** 2).mean()))
+    ms_exact = _time(lambda: exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False))
+    ms_head = _time(lambda: head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False))
+    res = {
+        "workload": w.name, "natoms": A, "train_environments": batch.natoms,
+        "atom_steps_per_s_head": A / (ms_head * 1e-3), "atom_steps_per_s_exact": A / (ms_exact * 1e-3),
+        "speedup": ms_exact / ms_head, "ms_head": ms_head, "ms_exact": ms_exact,
+        "n_fallback": extra["n_fallback"],
+        "force_rmse_head_meV_per_A": rm(Fh, Fx) * 1e3,
+        "force_rmse_level12_meV_per_A": rm(Fl, Fx) * 1e3,
+        "force_rms_exact_eV_per_A": float(torch.sqrt((Fx ** 2).mean())),
+        "energy_err_head_meV_per_atom": abs(float(Eh - Ex)) / A * 1e3,
+        "energy_err_level12_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
+        "train_force_rmse_meV_per_A": f_rmse * 1e3, "head_fit_s": t_fit,
+        "variant_exact": exact.variant, "variant_desc": head.desc.variant,
+        "total_s": time.time() - t_start,
+    }
+    head.close()
+    exact.close()
+    return res
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 100_000
+    print(json.dumps(run(n_env=n)))

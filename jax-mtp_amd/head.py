@@ -54,9 +54,10 @@ class HeadModel:
     hi: np.ndarray | None
     e_shift: np.ndarray         # [Sp]
     e_scale: float
+    residual: bool = False      # add the descriptor potential's own (linear) energy
 
     def save(self, path) -> None:
-        np.savez(path, theta=self.theta, mean=self.mean, inv_std=self.inv_std,
+        np.savez(path, theta=self.theta, mean=self.mean, inv_std=self.inv_std, residual=np.array(int(self.residual)),
                  lo=self.lo if self.lo is not None else np.zeros(0), hi=self.hi if self.hi is not None else np.zeros(0),
                  e_shift=self.e_shift, e_scale=np.array(self.e_scale), desc=np.array(format_mtp(self.desc)),
                  desc_radial=np.asarray(self.desc.radial_coeffs, dtype=np.float64),
@@ -74,7 +75,7 @@ class HeadModel:
         desc.moment_coeffs = d["desc_moment"].copy()
         return cls(desc, d["theta"].copy(), d["mean"].copy(), d["inv_std"].copy(),
                    lo.copy() if lo.size else None, hi.copy() if hi.size else None, d["e_shift"].copy(),
-                   float(d["e_scale"]))
+                   float(d["e_scale"]), bool(int(d["residual"])))
 
 
 def _ptr(a):
@@ -98,7 +99,7 @@ class DistilledHead:
             raise RuntimeError("head parameter layout mismatch")
         h = ctypes.c_void_p()
         capi.check(L.mtp_head_open(self.desc.ctx, exact.ctx if exact is not None else None, *[_ptr(a) for a in arrs],
-                                   float(model.e_scale), ctypes.byref(h)))
+                                   float(model.e_scale), int(model.residual), ctypes.byref(h)))
         self.h = h
 
     def close(self):
@@ -221,7 +222,7 @@ class HeadTrainer:
     forces F_ref [N, 3] on a packed training set (training.PackedBatch)."""
 
     def __init__(self, desc_pot: MTPPotential, batch, E_ref, F_ref, weights=(1.0, 1.0), margin: float = 0.0,
-                 seed: int = 0):
+                 seed: int = 0, residual: bool = True):
         import torch
 
         dev = torch.device("cuda")
@@ -252,14 +253,23 @@ class HeadTrainer:
             m = t == s
             if m.any():
                 lo[s], hi[s] = z[m].min(0), z[m].max(0)
-                e_shift[s] = Er[m].mean()
         span = hi - lo
         lo -= margin * span
         hi += margin * span
-        resid = Er - e_shift[np.clip(t, 0, Sp - 1)]
+        ts = np.where(t < 0, t + Sp, t).clip(0, Sp - 1)
+        if residual:  # the network fits what the descriptor potential's linear energy leaves
+            lin = np.asarray(desc_pot.params.moment_coeffs, dtype=np.float64)
+            Er = Er - (np.asarray(desc_pot.params.species_coeffs, dtype=np.float64)[ts] + b @ lin)
+        for s in range(Sp):
+            m = ts == s
+            e_shift[s] = Er[m].mean() if m.any() else 0.0
+        resid = Er - e_shift[ts]
         e_scale = float(resid.std()) if resid.std() > 0 else 1.0
         rng = np.random.default_rng(seed)
-        self.model = HeadModel(desc_pot.params, init_theta(self.S, Sp, rng), mean, inv_std, lo, hi, e_shift, e_scale)
+        theta = init_theta(self.S, Sp, rng)
+        if residual:  # start at the linear potential: small output layer
+            theta[-HIDDEN - 1:-1] *= 0.1
+        self.model = HeadModel(desc_pot.params, theta, mean, inv_std, lo, hi, e_shift, e_scale, residual)
         self.head = DistilledHead(self.model, None, self.N, self.K, desc_pot=desc_pot)
         self.theta = torch.tensor(self.model.theta, dtype=torch.float64, device=dev, requires_grad=True)
         self._fn = _function()

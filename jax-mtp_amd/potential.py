@@ -70,7 +70,8 @@ class MTPPotential:
         self.radial_fp32 = bool(radial_fp32)
         capi.check(capi.lib().mtp_set_precision(self.ctx, int(self.radial_fp32)), self.ctx)
 
-    VARIANTS = {0: "per-target, 256 threads", 1: "balanced items, 256 threads", 2: "balanced items, 512 threads"}
+    VARIANTS = {0: "per-target, 256 threads", 1: "balanced items, 256 threads", 2: "balanced items, 512 threads",
+                3: "balanced items, 64 threads"}
 
     @property
     def variant(self) -> int:
@@ -89,7 +90,7 @@ class MTPPotential:
         it, js, rij, jt = _i32(itypes, dev), _i32(all_js, dev), _f64(all_rijs, dev), _i32(all_jtypes, dev)
         A, K = js.shape
         best = ctypes.c_int()
-        ms = (ctypes.c_double * 3)()
+        ms = (ctypes.c_double * len(self.VARIANTS))()
         capi.check(capi.lib().mtp_autotune(self.ctx, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(),
                                            jt.data_ptr(), int(reps), ctypes.byref(best), ms), self.ctx)
         return best.value, list(ms)

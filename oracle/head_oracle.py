@@ -11,6 +11,7 @@ include/mtp_hip.h (mtp_head_*) and jax-mtp_amd/csrc/mtp_head.hpp:
   a1  = z W1 + b1,  h1 = silu(a1)             W1 [32, 64]
   a2  = h1 W2 + b2, h2 = silu(a2)             W2 [64, 64]
   E   = e_shift[type] + e_scale * (h2 . w3 + b3)
+        [+ lin_sc[type] + lin . b   in residual form]
 
 and its composition with the MTP oracle (mtp_oracle.c): basis values b of
 the descriptor potential, slot gradients of the head energy through the
@@ -88,6 +89,10 @@ def eval_head(model, basis, itypes):
     net, dz = net_and_grad(model.theta, z)
     E = model.e_shift[ts] + model.e_scale * net
     xi = model.e_scale * dz[:, :S] * model.inv_std
+    lin = getattr(model, "lin", None)
+    if lin is not None:
+        E = E + model.lin_sc[ts] + basis @ lin
+        xi = xi + lin
     if model.lo is None:
         flags = np.zeros(len(E), dtype=np.int32)
     else:

@@ -19,9 +19,14 @@ S12, SP = 29, 3
 class _M:
     """Minimal HeadModel-like holder for the oracle."""
 
-    def __init__(self, theta, mean, inv_std, lo, hi, e_shift, e_scale):
+    def __init__(self, theta, mean, inv_std, lo, hi, e_shift, e_scale, lin=None, lin_sc=None):
         self.theta, self.mean, self.inv_std, self.lo, self.hi = theta, mean, inv_std, lo, hi
-        self.e_shift, self.e_scale = e_shift, e_scale
+        self.e_shift, self.e_scale, self.lin, self.lin_sc = e_shift, e_scale, lin, lin_sc
+
+
+def _as_loaded(jm, p):
+    """The coefficients a potential opened from MTPData actually holds (its .mtp text)."""
+    return jm.parse_mtp_text(jm.format_mtp(p))
 
 
 def _model(rng, S=S12, Sp=SP, box=False):
@@ -201,15 +206,19 @@ def test_gpu_descriptor_jacobian_gather(jm, O):
 
 
 @pytest.mark.gpu
-def test_gpu_head_network_and_gradient(jm, O):
+@pytest.mark.parametrize("residual", [False, True])
+def test_gpu_head_network_and_gradient(jm, O, residual):
     import torch
     import head_oracle as HO
 
     rng = np.random.default_rng(6)
     desc = jm.configs.random_params(12, SP, 8, 2.0, 5.0, rng)
     model = _model(rng, box=True)
+    if residual:
+        dl = _as_loaded(jm, desc)
+        model.lin, model.lin_sc = np.asarray(dl.moment_coeffs), np.asarray(dl.species_coeffs)
     hm = jm.head.HeadModel(desc, model.theta, model.mean, model.inv_std, model.lo, model.hi, model.e_shift,
-                           model.e_scale)
+                           model.e_scale, residual)
     A = 3000
     head = jm.head.DistilledHead(hm, None, max_atoms=A, max_nbr=16)
     basis = rng.normal(0, 1, (A, S12)) + model.mean
@@ -251,8 +260,12 @@ def test_gpu_head_compute_with_fallback(jm, O, exact_level):
     E_r, F_r, S_r, eat_r, fl_r, _ = HO.head_compute(model, orc_d, orc_x, nb["itypes"], nb["js"], nb["rij"],
                                                      nb["jtypes"], 3, w.volume, n_e, n_f)
     assert 0 < fl_r.sum() < A
+    model.lin, model.lin_sc = orc_d._keep["moment"], orc_d._keep["species"]  # residual form
+    E_r, F_r, S_r, eat_r, fl_r, _ = HO.head_compute(model, orc_d, orc_x, nb["itypes"], nb["js"], nb["rij"],
+                                                     nb["jtypes"], 3, w.volume, n_e, n_f)
     exact = jm.MTPPotential(exact_p, max_atoms=A, max_nbr=K)
-    hm = jm.head.HeadModel(w.params, model.theta, model.mean, model.inv_std, lo, hi, model.e_shift, model.e_scale)
+    hm = jm.head.HeadModel(w.params, model.theta, model.mean, model.inv_std, lo, hi, model.e_shift, model.e_scale,
+                           True)
     head = jm.head.DistilledHead(hm, exact, max_atoms=A, max_nbr=K)
     E, F, St, extra = head.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
                                                             w.volume, n_e, n_f, return_all=True)
@@ -290,7 +303,10 @@ def test_gpu_head_training_gradient_and_fit(jm, O):
     g_gpu = tr.theta.grad.detach().cpu().numpy()
     # oracle: dL/dE, dL/dF -> v = J^T Delta -> parameter gradient
     orc_d = oracle_for(O, desc_p)
-    m = tr.model
+    tm = tr.model
+    assert tm.residual
+    m = _M(tm.theta, tm.mean, tm.inv_std, tm.lo, tm.hi, tm.e_shift, tm.e_scale, orc_d._keep["moment"],
+           orc_d._keep["species"])
     _, _, basis = orc_d.compact_xi(b.itypes, b.rij, b.jtypes, np.zeros(S12), grad=False)
     E, xi, _ = HO.eval_head(m, basis, b.itypes)
     _, g, _ = orc_d.compact_xi(b.itypes, b.rij, b.jtypes, xi)

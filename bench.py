@@ -6,7 +6,10 @@ A step is one full force evaluation of the padded neighbor arrays already
 resident in HBM: radial basis -> moments -> contraction -> energy ->
 adjoint -> per-slot gradients -> Newton-3 force gather -> virial.
 
-Prints ONE JSON line.  ``roofline`` is the atom kernel's achieved
+Prints ONE JSON line.  ``distilled_head`` adds the head configuration:
+distillation of the level-24 MTP into an MLP over level-12 basis values
+(1e5 training environments) and its atom-steps/s and force RMSE on a
+100,000-atom cell against plain level-24 evaluation.  ``roofline`` is the atom kernel's achieved
 algorithmic FP64 FLOP rate (fixed per-atom FLOP formula)
 over its average launch time, measured with HIP events on its own stream.
 ``cpu_baseline`` times the FP64 CPU restatement of the reference's
@@ -48,6 +51,27 @@ def algorithmic_bytes(nbr_counts):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_bytes(workload, kernel):
+    """HBM bytes per atom-kernel launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter's reading for a plain copy of known size), or None."""
+    f = ROOT / "profiles" / f"atom_kernel_{workload}_traffic.txt"
+    if not f.exists():
+        return None
+    kv = dict(line.split("=", 1) for line in f.read_text().splitlines() if "=" in line and not line.startswith("#"))
+    if kv.get("kernel", "").strip() != kernel:
+        return None
+    return float(kv["traffic_bytes"])
+
+
+def head_leg(seed, steps):
+    """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py)."""
+    jm = importlib.import_module("jax-mtp_amd")
+    res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, seed=seed, decay=0.35, log=log)
+    res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
+    return res
 
 
 def cpu_baseline(w, nb, gpu_F, threads):
@@ -101,6 +125,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ncell", type=int, default=20, help="FCC cells per edge (20 -> 32,000 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-head", action="store_true", help="skip the distilled-head leg")
+    ap.add_argument("--head-steps", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -193,12 +219,13 @@ def main():
                    "cutoff": w.cutoff, "max_nbr": K, "mean_neighbors": float(nb["counts"].mean()),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "atom_kernel<double>", "kernel_ms": r["atom_kernel_ms"],
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_bytes(w.name, pot.VARIANTS[best]),
+                     "kernel": f"atom_kernel ({pot.VARIANTS[best]})", "kernel_ms": r["atom_kernel_ms"],
                      "algorithmic_flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(nb["counts"][: w.natoms])},
         "autotune": {"chosen": best, "kernel": pot.VARIANTS[best],
-                     "ms_per_step_by_variant": {pot.VARIANTS[v]: tune_ms[v] for v in range(3) if tune_ms[v] >= 0}},
+                     "ms_per_step_by_variant": {pot.VARIANTS[v]: tune_ms[v] for v in range(len(tune_ms))
+                                                if tune_ms[v] >= 0}},
         "radial_fp32": {"value": w.natoms / (results["radial_fp32"]["ms_per_step"] * 1e-3),
                         "ms_per_step": results["radial_fp32"]["ms_per_step"],
                         "atom_kernel_ms": results["radial_fp32"]["atom_kernel_ms"]},
@@ -215,8 +242,11 @@ def main():
             "radial_fp32": float(np.max(np.abs(f32 - Fref)) / np.max(np.abs(Fref))),
             "norm": "max|F-F_ref| / max|F_ref| over all atoms and components",
         }
-    print(json.dumps(line), flush=True)
     pot.close()
+    if not args.no_head:
+        torch.cuda.empty_cache()
+        line["distilled_head"] = head_leg(args.seed, args.head_steps)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -66,10 +66,11 @@ def traffic_bytes(workload, kernel):
     return float(kv["traffic_bytes"])
 
 
-def head_leg(seed, steps):
+def head_leg(seed, steps, save_dir=None):
     """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py)."""
     jm = importlib.import_module("jax-mtp_amd")
-    res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, seed=seed, decay=0.35, log=log)
+    res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, seed=seed, decay=0.35, log=log,
+                         save_dir=save_dir)
     res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
     return res
 
@@ -127,6 +128,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-head", action="store_true", help="skip the distilled-head leg")
     ap.add_argument("--head-steps", type=int, default=3000)
+    ap.add_argument("--head-save", default=None, help="directory to keep the trained head and its teacher")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -245,7 +247,7 @@ def main():
     pot.close()
     if not args.no_head:
         torch.cuda.empty_cache()
-        line["distilled_head"] = head_leg(args.seed, args.head_steps)
+        line["distilled_head"] = head_leg(args.seed, args.head_steps, args.head_save)
     print(json.dumps(line), flush=True)
 
 

@@ -30,8 +30,8 @@ import time
 import numpy as np
 
 from . import configs, training
-from .head import DistilledHead, HeadTrainer
-from .mtp_io import MTPData, untrained
+from .head import DistilledHead, HeadModel, HeadTrainer
+from .mtp_io import MTPData, read_mtp, write_mtp
 from .potential import MTPPotential
 
 A_FCC = 3.6
@@ -151,7 +151,11 @@ def _time(fn, reps=5, warmup=2):
     return t0.elapsed_time(t1) / reps
 
 
-def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print):
+def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
+        save_dir=None):
+    """Teacher, training set, level-12 fit, head training, then evaluate();
+    with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
+    are written there for infer()."""
     import torch
 
     t_start = time.time()
@@ -176,9 +180,24 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
     desc.close()
     del tr
     torch.cuda.empty_cache()
+    if save_dir is not None:
+        import pathlib
 
-    # inference on the large cell
-    w = inference_cell(infer_ncell, seed=seed + 2)
+        d = pathlib.Path(save_dir)
+        d.mkdir(parents=True, exist_ok=True)
+        model.save(d / "head.npz")
+        write_mtp(d / "teacher.mtp", teacher)
+    res = evaluate(model, teacher, infer_ncell, seed=seed + 2)
+    res.update({"train_environments": batch.natoms, "train_force_rmse_meV_per_A": f_rmse * 1e3,
+                "head_fit_s": t_fit, "total_s": time.time() - t_start})
+    return res
+
+
+def evaluate(model, teacher: MTPData, infer_ncell=(25, 25, 40), seed=2, reps=5):
+    """Head (with exact fallback) vs plain teacher on the inference cell: timing and errors."""
+    import torch
+
+    w = inference_cell(infer_ncell, seed=seed)
     nb = w.neighbors()
     dev = torch.device("cuda")
     it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
@@ -195,10 +214,12 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
     Eh, Fh, _, extra = head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
     El, Fl, _ = lin.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
     rm = lambda a, b: float(torch.sqrt(((a - b) ** 2).mean()))
-    ms_exact = _time(lambda: exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False))
-    ms_head = _time(lambda: head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False))
+    ms_exact = _time(lambda: exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False),
+                     reps=reps)
+    ms_head = _time(lambda: head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False),
+                    reps=reps)
     res = {
-        "workload": w.name, "natoms": A, "train_environments": batch.natoms,
+        "workload": w.name, "natoms": A,
         "atom_steps_per_s_head": A / (ms_head * 1e-3), "atom_steps_per_s_exact": A / (ms_exact * 1e-3),
         "speedup": ms_exact / ms_head, "ms_head": ms_head, "ms_exact": ms_exact,
         "n_fallback": extra["n_fallback"],
@@ -207,18 +228,40 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
         "force_rms_exact_eV_per_A": float(torch.sqrt((Fx ** 2).mean())),
         "energy_err_head_meV_per_atom": abs(float(Eh - Ex)) / A * 1e3,
         "energy_err_level12_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
-        "train_force_rmse_meV_per_A": f_rmse * 1e3, "head_fit_s": t_fit,
         "variant_exact": exact.variant, "variant_desc": head.desc.variant,
-        "total_s": time.time() - t_start,
     }
     head.close()
     exact.close()
     return res
 
 
-if __name__ == "__main__":
-    import json
-    import sys
+def infer(save_dir, infer_ncell=(25, 25, 40), seed=2, reps=20):
+    """evaluate() of a head saved by run(save_dir=...)."""
+    import pathlib
 
-    n = int(sys.argv[1]) if len(sys.argv) > 1 else 100_000
-    print(json.dumps(run(n_env=n)))
+    d = pathlib.Path(save_dir)
+    return evaluate(HeadModel.load(d / "head.npz"), read_mtp(d / "teacher.mtp"), infer_ncell, seed=seed, reps=reps)
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="distil a level-24 MTP into the head and evaluate it")
+    ap.add_argument("--n-env", type=int, default=100_000)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--decay", type=float, default=0.35)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--save", default=None, help="directory for head.npz and teacher.mtp")
+    ap.add_argument("--infer", default=None, help="only evaluate the head saved in this directory")
+    ap.add_argument("--reps", type=int, default=20)
+    a = ap.parse_args(argv)
+    if a.infer:
+        res = infer(a.infer, reps=a.reps)
+    else:
+        res = run(n_env=a.n_env, head_steps=a.steps, decay=a.decay, seed=a.seed, save_dir=a.save)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

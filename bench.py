@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-head", action="store_true", help="skip the distilled-head leg")
     ap.add_argument("--head-steps", type=int, default=3000)
     ap.add_argument("--head-save", default=None, help="directory to keep the trained head and its teacher")
+    ap.add_argument("--head-infer", default=None,
+                    help="only evaluate a head saved with --head-save (prints its JSON; for kernel profiles)")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -139,6 +141,9 @@ def main():
     import torch
 
     jm = importlib.import_module("jax-mtp_amd")
+    if args.head_infer:
+        print(json.dumps(jm.distill.infer(args.head_infer, reps=args.steps)), flush=True)
+        return
     w = jm.configs.fcc32k_level24(seed=args.seed, ncell=args.ncell)
     t0 = time.perf_counter()
     nb = w.neighbors()

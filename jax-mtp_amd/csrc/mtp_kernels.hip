@@ -662,30 +662,32 @@ __global__ void rev_count_kernel(const int* js, int nrows, int nslots, int n_f, 
     }
 }
 
-// Exclusive scan of cnt[0..n) into off[0..n] by one workgroup.
+// Exclusive scan of cnt[0..n) into off[0..n] by one workgroup: each thread
+// owns a contiguous segment (serial sum, block scan of the segment sums,
+// serial write-back), so the workgroup makes one pass over the array.
 __global__ void __launch_bounds__(1024) scan_kernel(const int* cnt, int n, int* off)
 {
     __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    const int t = threadIdx.x;
+    const int C = (n + 1023) / 1024;
+    const int b0 = t * C < n ? t * C : n;
+    const int b1 = b0 + C < n ? b0 + C : n;
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += cnt[i];
+    buf[t] = s;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = i < n ? cnt[i] : 0;
-        buf[threadIdx.x] = v;
+    for (int w = 1; w < 1024; w <<= 1) {
+        const int add = t >= w ? buf[t - w] : 0;
         __syncthreads();
-        for (int w = 1; w < 1024; w <<= 1) {
-            const int add = threadIdx.x >= w ? buf[threadIdx.x - w] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < n) off[i] = carry + buf[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
+        buf[t] += add;
         __syncthreads();
     }
-    if (threadIdx.x == 0) off[n] = carry;
+    int run = buf[t] - s;
+    for (int i = b0; i < b1; ++i) {
+        off[i] = run;
+        run += cnt[i];
+    }
+    if (t == 1023) off[n] = buf[1023];
 }
 
 __global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, const int* off, int* cur, int* rev)
@@ -702,28 +704,62 @@ __global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, c
     }
 }
 
-__global__ void force_kernel(const double* g, int nrows, int nslots, int n_f, const int* off, int* rev, double* F)
+// One wavefront per atom: its own slots strided over the lanes, its incoming
+// slots sorted by slot index (64-lane bitonic sort; a selection loop for
+// longer lists) so that the summation order does not depend on the order in
+// which rev_fill's atomics placed them.  Bitwise reproducible.
+__global__ void __launch_bounds__(256) force_kernel(const double* g, int nrows, int nslots, int n_f, const int* off,
+                                                    const int* rev, double* F)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int a = (int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave);
     if (a >= nrows) return;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (a < n_f) {
         const double* ga = g + (size_t)a * nslots * 3;
-        for (int k = 0; k < nslots; ++k) { fx += ga[3 * k]; fy += ga[3 * k + 1]; fz += ga[3 * k + 2]; }
-        const int s0 = off[a], s1 = off[a + 1];
-        // insertion sort of the (short) incoming segment -> fixed summation order
-        for (int i = s0 + 1; i < s1; ++i) {
-            const int v = rev[i];
-            int j = i - 1;
-            while (j >= s0 && rev[j] > v) { rev[j + 1] = rev[j]; --j; }
-            rev[j + 1] = v;
+        for (int k = lane; k < nslots; k += kWave) { fx += ga[3 * k]; fy += ga[3 * k + 1]; fz += ga[3 * k + 2]; }
+        const int s0 = off[a], cnt = off[a + 1] - s0;
+        if (cnt <= kWave) {
+            int key = lane < cnt ? rev[s0 + lane] : 0x7fffffff;
+#pragma unroll
+            for (int size = 2; size <= kWave; size <<= 1) {
+#pragma unroll
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    const int other = __shfl_xor(key, stride);
+                    const bool up = (lane & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    key = (lower == up) ? min(key, other) : max(key, other);
+                }
+            }
+            if (lane < cnt) {
+                const double* gs = g + (size_t)key * 3;
+                fx -= gs[0]; fy -= gs[1]; fz -= gs[2];
+            }
+        } else {  // long incoming list: ascending selection, accumulated by lane 0
+            int last = -1;
+            for (int r = 0; r < cnt; ++r) {
+                int m = 0x7fffffff;
+                for (int q = lane; q < cnt; q += kWave) {
+                    const int v = rev[s0 + q];
+                    if (v > last && v < m) m = v;
+                }
+#pragma unroll
+                for (int o = kWave / 2; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
+                last = m;
+                if (lane == 0) {
+                    const double* gs = g + (size_t)m * 3;
+                    fx -= gs[0]; fy -= gs[1]; fz -= gs[2];
+                }
+            }
         }
-        for (int i = s0; i < s1; ++i) {
-            const double* gs = g + (size_t)rev[i] * 3;
-            fx -= gs[0]; fy -= gs[1]; fz -= gs[2];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            fx += __shfl_xor(fx, o);
+            fy += __shfl_xor(fy, o);
+            fz += __shfl_xor(fz, o);
         }
     }
-    F[3 * (size_t)a] = fx; F[3 * (size_t)a + 1] = fy; F[3 * (size_t)a + 2] = fz;
+    if (lane == 0) { F[3 * (size_t)a] = fx; F[3 * (size_t)a + 1] = fy; F[3 * (size_t)a + 2] = fz; }
 }
 
 // out[0] = E, out[1..6] = stress (reference Voigt xx yy zz yz xz xy, /V),
@@ -732,25 +768,32 @@ __global__ void force_kernel(const double* g, int nrows, int nslots, int n_f, co
 __global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const double* Wi, int nrows, int n_e,
                                                       int cell_rank, double volume, double* out)
 {
-    __shared__ double buf[1024];
+    __shared__ double part[1024 / kWave][10];
     __shared__ double acc[10];
-    const int tid = threadIdx.x;
-    for (int q = 0; q < 10; ++q) {
-        double s = 0.0;
-        if (q == 0) {
-            for (int i = tid; i < n_e && i < nrows; i += 1024) s += Ei[i];
-        } else {
-            for (int i = tid; i < nrows; i += 1024) s += Wi[(size_t)i * 9 + (q - 1)];
-        }
-        buf[tid] = s;
-        __syncthreads();
-        for (int w = 512; w > 0; w >>= 1) {
-            if (tid < w) buf[tid] += buf[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) acc[q] = buf[0];
-        __syncthreads();
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    double s[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) s[q] = 0.0;
+    for (int i = tid; i < nrows; i += 1024) {  // one pass: E_i and the 9 virial components of each row
+        if (i < n_e) s[0] += Ei[i];
+        const double* w = Wi + (size_t)i * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) s[1 + q] += w[q];
     }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        double v = s[q];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) part[wave][q] = v;
+    }
+    __syncthreads();
+    if (tid < 10) {
+        double v = 0.0;
+        for (int w = 0; w < 1024 / kWave; ++w) v += part[w][tid];
+        acc[tid] = v;
+    }
+    __syncthreads();
     if (tid == 0) {
         const double* W = acc + 1;
         out[0] = acc[0];
@@ -879,7 +922,7 @@ hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, F.cnt, n, F.off);
     hipLaunchKernelGGL(rev_fill_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.off,
                        F.cnt + n, F.rev);
-    hipLaunchKernelGGL(force_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
+    hipLaunchKernelGGL(force_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
                        F.off, F.rev, F.F);
     return hipGetLastError();
 }

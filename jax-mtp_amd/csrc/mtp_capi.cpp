@@ -66,6 +66,7 @@ struct mtp_ctx {
     // per-call buffers
     double *d_rij = nullptr, *d_Ei = nullptr, *d_g = nullptr, *d_Wi = nullptr, *d_F = nullptr, *d_out = nullptr;
     int *d_it = nullptr, *d_js = nullptr, *d_jt = nullptr, *d_cnt = nullptr, *d_off = nullptr, *d_rev = nullptr;
+    double* d_tot = nullptr;
     // pinned staging
     double* h_rij = nullptr;
     int *h_it = nullptr, *h_js = nullptr, *h_jt = nullptr;
@@ -209,7 +210,8 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->d_Wi = c->alloc<double>(A * 9, e);
     c->d_F = c->alloc<double>(A * 3, e);
     c->d_out = c->alloc<double>(16, e);
-    c->d_cnt = c->alloc<int>(2 * A, e);
+    c->d_cnt = c->alloc<int>(2 * A + 2 * ((A + 4095) / 4096) + 16, e);
+    c->d_tot = c->alloc<double>(kTotalsBlocks * 10, e);
     c->d_off = c->alloc<int>(A + 1, e);
     c->d_rev = c->alloc<int>(A * K, e);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rij, A * K * 3 * sizeof(double), hipHostMallocDefault);
@@ -446,7 +448,7 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
     A.g = ctx->d_g; A.Wi = ctx->d_Wi;
     ForceArgs F{};
     F.n_e = natoms_energy; F.n_f = natoms_force; F.cell_rank = cell_rank; F.volume = volume;
-    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.out = d_out;
+    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.out = d_out; F.tot = ctx->d_tot;
     hipEvent_t eb = nullptr, ee = nullptr;
     if (ctx->profiling) {
         if (ctx->ev_used == ctx->ev_pool.size()) {
@@ -497,7 +499,7 @@ int mtp_autotune(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const
     A.Ei = ctx->d_Ei; A.g = ctx->d_g; A.Wi = ctx->d_Wi;
     ForceArgs F{};
     F.n_e = nrows; F.n_f = nrows; F.cell_rank = 3; F.volume = 1.0;
-    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = ctx->d_F; F.out = ctx->d_out;
+    F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = ctx->d_F; F.out = ctx->d_out; F.tot = ctx->d_tot;
     int chosen = ctx->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumVariants; ++v) {
@@ -914,7 +916,7 @@ int mtp_gather_forces_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_j
     AtomArgs A{};
     A.nrows = nrows; A.nslots = nslots; A.js = d_js; A.g = const_cast<double*>(d_g);
     ForceArgs F{};
-    F.n_f = n_f; F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces;
+    F.n_f = n_f; F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.tot = ctx->d_tot;
     hipError_t e = launch_gather(A, F, stream ? (hipStream_t)stream : ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "force gather");
     return MTP_OK;
@@ -1056,7 +1058,7 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     A.js = d_js; A.Ei = E; A.g = D->d_g; A.Wi = D->d_Wi;
     ForceArgs F{};
     F.n_e = n_e; F.n_f = n_f; F.cell_rank = cell_rank; F.volume = volume;
-    F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out;
+    F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out; F.tot = D->d_tot;
     if (nrows > 0) {
         // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
         rc = mtp_descriptor_device(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_basis, s);

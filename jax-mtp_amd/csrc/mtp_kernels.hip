@@ -19,7 +19,8 @@
 //                     F  g_k = dE_i/dr_k from dm of the basic moments
 //                     G  per-atom virial W_i = sum_k r_k (x) g_k
 //   zero/count/scan/fill/force kernels: deterministic Newton-3 force gather
-//   totals_kernel   E = sum_{i<n_e} E_i, W = sum_i W_i -> stress, virial
+//   totals_partial_kernel + totals_kernel: E = sum_{i<n_e} E_i, W = sum_i W_i
+//                   -> stress, virial (two-stage, deterministic)
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -662,32 +663,84 @@ __global__ void rev_count_kernel(const int* js, int nrows, int nslots, int n_f, 
     }
 }
 
-// Exclusive scan of cnt[0..n) into off[0..n] by one workgroup: each thread
-// owns a contiguous segment (serial sum, block scan of the segment sums,
-// serial write-back), so the workgroup makes one pass over the array.
-__global__ void __launch_bounds__(1024) scan_kernel(const int* cnt, int n, int* off)
+// Exclusive scan of cnt[0..n) into off[0..n] in three passes over tiles of
+// kScanTile counts: tile sums, a scan of the tile sums, then each tile's own
+// scan from its offset.
+constexpr int kScanTile = 4096;  // 256 threads x 16
+
+__device__ __forceinline__ int block_excl_scan_256(int v, int* sh, int& total)
 {
-    __shared__ int buf[1024];
     const int t = threadIdx.x;
-    const int C = (n + 1023) / 1024;
-    const int b0 = t * C < n ? t * C : n;
-    const int b1 = b0 + C < n ? b0 + C : n;
-    int s = 0;
-    for (int i = b0; i < b1; ++i) s += cnt[i];
-    buf[t] = s;
+    sh[t] = v;
     __syncthreads();
-    for (int w = 1; w < 1024; w <<= 1) {
-        const int add = t >= w ? buf[t - w] : 0;
+    for (int w = 1; w < 256; w <<= 1) {
+        const int add = t >= w ? sh[t - w] : 0;
         __syncthreads();
-        buf[t] += add;
+        sh[t] += add;
         __syncthreads();
     }
-    int run = buf[t] - s;
-    for (int i = b0; i < b1; ++i) {
-        off[i] = run;
-        run += cnt[i];
+    const int incl = sh[t];
+    total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(256) scan_tiles_kernel(const int* cnt, int n, int* tile_sum)
+{
+    __shared__ int sh[256];
+    const int base = blockIdx.x * kScanTile;
+    int s = 0;
+    for (int q = 0; q < kScanTile / 256; ++q) {
+        const int i = base + q * 256 + threadIdx.x;
+        if (i < n) s += cnt[i];
     }
-    if (t == 1023) off[n] = buf[1023];
+    int total;
+    block_excl_scan_256(s, sh, total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) scan_top_kernel(const int* tile_sum, int ntiles, int n, int* tile_off, int* off)
+{
+    __shared__ int sh[256];
+    int carry = 0;
+    for (int b0 = 0; b0 < ntiles; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < ntiles ? tile_sum[b] : 0;
+        int total;
+        const int ex = block_excl_scan_256(v, sh, total);
+        if (b < ntiles) tile_off[b] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) off[n] = carry;
+}
+
+__global__ void __launch_bounds__(256) scan_apply_kernel(const int* cnt, int n, const int* tile_off, int* off)
+{
+    __shared__ int tile[kScanTile];
+    __shared__ int sh[256];
+    const int base = blockIdx.x * kScanTile;
+    for (int q = 0; q < kScanTile / 256; ++q) {  // coalesced load of the tile
+        const int i = base + q * 256 + threadIdx.x;
+        tile[q * 256 + threadIdx.x] = i < n ? cnt[i] : 0;
+    }
+    __syncthreads();
+    constexpr int C = kScanTile / 256;  // each thread scans 16 consecutive counts
+    int v[C];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < C; ++q) { v[q] = tile[threadIdx.x * C + q]; s += v[q]; }
+    int total;
+    int run = tile_off[blockIdx.x] + block_excl_scan_256(s, sh, total);
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+        tile[threadIdx.x * C + q] = run;
+        run += v[q];
+    }
+    __syncthreads();
+    for (int q = 0; q < kScanTile / 256; ++q) {
+        const int i = base + q * 256 + threadIdx.x;
+        if (i < n) off[i] = tile[q * 256 + threadIdx.x];
+    }
 }
 
 __global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, const int* off, int* cur, int* rev)
@@ -765,16 +818,18 @@ __global__ void __launch_bounds__(256) force_kernel(const double* g, int nrows, 
 // out[0] = E, out[1..6] = stress (reference Voigt xx yy zz yz xz xy, /V),
 // out[7..12] = LAMMPS virial (xx yy zz xy xz yz, energy units), out[13] =
 // status (0 ok, 1 non-finite energy or virial).
-__global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const double* Wi, int nrows, int n_e,
-                                                      int cell_rank, double volume, double* out)
+// Stage 1: block b sums rows b, b + G, ... (E_i for i < n_e, and the 9
+// virial components) into tot[b][10]; stage 2 adds the kTotalsBlocks partials
+// in block order.  Deterministic for a given row count.
+__global__ void __launch_bounds__(256) totals_partial_kernel(const double* Ei, const double* Wi, int nrows, int n_e,
+                                                             double* tot)
 {
-    __shared__ double part[1024 / kWave][10];
-    __shared__ double acc[10];
+    __shared__ double part[256 / kWave][10];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     double s[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) s[q] = 0.0;
-    for (int i = tid; i < nrows; i += 1024) {  // one pass: E_i and the 9 virial components of each row
+    for (int i = blockIdx.x * 256 + tid; i < nrows; i += kTotalsBlocks * 256) {
         if (i < n_e) s[0] += Ei[i];
         const double* w = Wi + (size_t)i * 9;
 #pragma unroll
@@ -790,7 +845,18 @@ __global__ void __launch_bounds__(1024) totals_kernel(const double* Ei, const do
     __syncthreads();
     if (tid < 10) {
         double v = 0.0;
-        for (int w = 0; w < 1024 / kWave; ++w) v += part[w][tid];
+        for (int w = 0; w < 256 / kWave; ++w) v += part[w][tid];
+        tot[blockIdx.x * 10 + tid] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64) totals_kernel(const double* tot, int cell_rank, double volume, double* out)
+{
+    __shared__ double acc[10];
+    const int tid = threadIdx.x;
+    if (tid < 10) {
+        double v = 0.0;
+        for (int b = 0; b < kTotalsBlocks; ++b) v += tot[b * 10 + tid];
         acc[tid] = v;
     }
     __syncthreads();
@@ -919,7 +985,14 @@ hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
     if (gs > 65536) gs = 65536;
     hipLaunchKernelGGL(zero_ints_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, F.cnt, 2 * n);
     hipLaunchKernelGGL(rev_count_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.cnt);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, F.cnt, n, F.off);
+    {
+        const int ntiles = (n + kScanTile - 1) / kScanTile;
+        int* tile_sum = F.cnt + 2 * n;
+        int* tile_off = tile_sum + ntiles;
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_sum);
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, stream, tile_sum, ntiles, n, tile_off, F.off);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_off, F.off);
+    }
     hipLaunchKernelGGL(rev_fill_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.off,
                        F.cnt + n, F.rev);
     hipLaunchKernelGGL(force_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
@@ -929,8 +1002,9 @@ hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
 
 hipError_t launch_totals(const AtomArgs& A, const ForceArgs& F, hipStream_t stream)
 {
-    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(1024), 0, stream, A.Ei, A.Wi, A.nrows > 0 ? A.nrows : 0,
-                       A.nrows > 0 ? F.n_e : 0, F.cell_rank, F.volume, F.out);
+    hipLaunchKernelGGL(totals_partial_kernel, dim3(kTotalsBlocks), dim3(256), 0, stream, A.Ei, A.Wi,
+                       A.nrows > 0 ? A.nrows : 0, A.nrows > 0 ? F.n_e : 0, F.tot);
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, stream, F.tot, F.cell_rank, F.volume, F.out);
     return hipGetLastError();
 }
 

@@ -30,12 +30,14 @@ struct AtomArgs {
 struct ForceArgs {
     int n_e, n_f, cell_rank;
     double volume;
-    int* cnt;              // [2A] scratch (counts, cursors)
+    int* cnt;              // [2A + 2 ceil(A / 4096) + 16] scratch (counts, cursors, scan tiles)
     int* off;              // [A+1]
     int* rev;              // [A*K]
     double* F;             // [A][3]
     double* out;           // [14]: E, stress6, virial6, status
+    double* tot;           // [kTotalsBlocks * 10] scratch of the totals reduction
 };
+constexpr int kTotalsBlocks = 128;
 
 struct AtomLds {
     int m, dm, f, fp, vx, vy, vz, r, red, ints, total_doubles;

@@ -6,7 +6,9 @@ A step is one full force evaluation of the padded neighbor arrays already
 resident in HBM: radial basis -> moments -> contraction -> energy ->
 adjoint -> per-slot gradients -> Newton-3 force gather -> virial.
 
-Prints ONE JSON line.  ``distilled_head`` adds the head configuration:
+Prints ONE JSON line.  ``training_step`` adds the packed training
+configuration (512 uneven configurations, level 16, eager vs HIP-graph
+evaluation).  ``distilled_head`` adds the head configuration:
 distillation of the level-24 MTP into an MLP over level-12 basis values
 (1e5 training environments) and its atom-steps/s and force RMSE on a
 100,000-atom cell against plain level-24 evaluation.  ``roofline`` is the atom kernel's achieved
@@ -75,6 +77,52 @@ def head_leg(seed, steps, save_dir=None):
     return res
 
 
+def training_leg(seed, lbfgs_steps=5, reps=10):
+    """Config 4: one training evaluation (forward + loss gradients + hand-written parameter backward) over
+    512 packed configurations (8-256 atoms, FCC/BCC/random-dense, 2 species), level-16 student, labels from
+    a seeded level-20 teacher.  Eager (autograd Function) vs replayed HIP graph."""
+    import torch
+
+    jm = importlib.import_module("jax-mtp_amd")
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    cells = jm.training.random_cells(512, rng, natoms_range=(8, 256), species=2)
+    b = jm.training.pack_configs(cells, cutoff=5.0, max_nbr=64)
+    t_pack = time.perf_counter() - t0
+    teacher = jm.configs.random_params(20, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    tp = jm.MTPPotential(teacher, max_atoms=b.natoms, max_nbr=b.nslots)
+    E_ref, F_ref, S_ref = [t.clone() for t in jm.training.batch_forward(tp, b.to_device())]
+    tp.close()
+    student = jm.configs.random_params(16, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    pot = jm.MTPPotential(student, max_atoms=b.natoms, max_nbr=b.nslots)
+    tr = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=True, graph=True)
+    th = tr.theta.detach().clone()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t) / reps
+
+    tr_e = jm.training.Trainer.__new__(jm.training.Trainer)
+    tr_e.__dict__.update(tr.__dict__)
+    tr_e.graphed = None
+    ms_eager = timed(lambda: tr_e.value_and_grad(th))
+    ms_graph = timed(lambda: tr.value_and_grad(th))
+    db = tr.dbatch
+    ms_fwd = timed(lambda: jm.training.batch_forward(pot, db))
+    l0 = tr.value_and_grad(th)[0]
+    losses = [tr.step() for _ in range(lbfgs_steps)]
+    pot.close()
+    return {"configs": b.nconf, "atoms": b.natoms, "max_nbr": b.nslots, "level": 16, "teacher_level": 20,
+            "ms_eval_eager": ms_eager, "ms_eval_graph": ms_graph, "ms_forward": ms_fwd,
+            "atom_evals_per_s_graph": b.natoms / (ms_graph * 1e-3), "loss_after_lls": l0,
+            "loss_after_lbfgs": losses[-1], "lbfgs_steps": lbfgs_steps, "host_batch_build_s": t_pack}
+
+
 def cpu_baseline(w, nb, gpu_F, threads):
     """Reference-algorithm (full-tensor tree) CPU timing on a bounded sample,
     plus the compact CPU oracle on the whole workload (and the GPU force
@@ -127,6 +175,7 @@ def main():
     ap.add_argument("--ncell", type=int, default=20, help="FCC cells per edge (20 -> 32,000 atoms)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-head", action="store_true", help="skip the distilled-head leg")
+    ap.add_argument("--no-train", action="store_true", help="skip the packed training-step leg")
     ap.add_argument("--head-steps", type=int, default=3000)
     ap.add_argument("--head-save", default=None, help="directory to keep the trained head and its teacher")
     ap.add_argument("--head-infer", default=None,
@@ -250,6 +299,10 @@ def main():
             "norm": "max|F-F_ref| / max|F_ref| over all atoms and components",
         }
     pot.close()
+    if not args.no_train:
+        torch.cuda.empty_cache()
+        line["training_step"] = training_leg(args.seed)
+        log(f"[bench] training: {line['training_step']}")
     if not args.no_head:
         torch.cuda.empty_cache()
         line["distilled_head"] = head_leg(args.seed, args.head_steps, args.head_save)

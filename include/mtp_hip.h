@@ -64,6 +64,14 @@ int mtp_info(const mtp_ctx *ctx, int *level, int *species_count, int *radial_fun
 int mtp_set_params(mtp_ctx *ctx, const double *moment_coeffs, const double *species_coeffs,
                    const double *radial_coeffs, double scaling);
 
+/* Same from device arrays, enqueued on stream (no host synchronisation,
+ * so it can be captured in a graph with the evaluation that follows).  The
+ * host copy of the parameters (mtp_set_params, mtp_info) is not updated.
+ * No reference counterpart: the reference re-traced its jitted loss with new
+ * parameters (motep_jax_train_function.py:154-191). */
+int mtp_set_params_device(mtp_ctx *ctx, const double *d_moment_coeffs, const double *d_species_coeffs,
+                          const double *d_radial_coeffs, void *stream);
+
 /* Host-array evaluation with the data contract of
  * ZeroOverheadManager::execute_potential (zero_overhead.hpp:58-64), as the
  * pair style calls it (pair_jax_mtp.cpp:183-189):

@@ -125,7 +125,8 @@ hipError_t upload(T* d, const std::vector<T>& h, hipStream_t s)
 int upload_params(mtp_ctx* c)
 {
     const MtpFile& f = c->plan.file;
-    hipError_t e = hipSuccess;
+    // kernels on any stream may still read the old parameters
+    hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = upload(c->d_radial, f.radial_coeffs, c->stream);
     if (e == hipSuccess) e = upload(c->d_species, f.species_coeffs, c->stream);
     if (e == hipSuccess) e = upload(c->d_moment, f.moment_coeffs, c->stream);
@@ -424,6 +425,24 @@ int mtp_set_params(mtp_ctx* ctx, const double* moment_coeffs, const double* spec
     if (std::isfinite(scaling) && scaling > 0) f.scaling = scaling;
     refresh_plan_params(ctx->plan);
     return upload_params(ctx);
+}
+
+int mtp_set_params_device(mtp_ctx* ctx, const double* d_moment_coeffs, const double* d_species_coeffs,
+                          const double* d_radial_coeffs, void* stream)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    const MtpFile& f = ctx->plan.file;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipError_t e = hipSuccess;
+    auto cp = [&](double* dst, const double* src, size_t n) {
+        if (e == hipSuccess && src && n) e = hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    };
+    cp(ctx->d_moment, d_moment_coeffs, f.moment_coeffs.size());
+    cp(ctx->d_species, d_species_coeffs, f.species_coeffs.size());
+    cp(ctx->d_radial, d_radial_coeffs, f.radial_coeffs.size());
+    if (e == hipSuccess && d_moment_coeffs) e = launch_xi_of(ctx->dev, ctx->d_xi, s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "device parameter update");
+    return MTP_OK;
 }
 
 int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const int* d_js,

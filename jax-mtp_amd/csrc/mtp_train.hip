@@ -395,6 +395,27 @@ __global__ void __launch_bounds__(256) seg_totals_kernel(const double* Ei, const
     }
 }
 
+// xi_of[x] = sum of the moment coefficients mapped to moment x, summed in
+// increasing scalar index (the host's order), for the mapped moments only
+// (the others stay zero).
+__global__ void xi_of_kernel(const double* moment, int n_mapped, const int* map_x, const int* map_beg,
+                             const int* map_s, double* xi)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_mapped) return;
+    double v = 0.0;
+    for (int t = map_beg[q]; t < map_beg[q + 1]; ++t) v += moment[map_s[t]];
+    xi[map_x[q]] = v;
+}
+
+hipError_t launch_xi_of(const DevPlan& P, double* xi, hipStream_t stream)
+{
+    if (P.n_mapped <= 0) return hipSuccess;
+    hipLaunchKernelGGL(xi_of_kernel, dim3((P.n_mapped + 255) / 256), dim3(256), 0, stream, P.moment_coeffs,
+                       P.n_mapped, P.map_x, P.map_beg, P.map_s, xi);
+    return hipGetLastError();
+}
+
 // Raises (never lowers) the kernel's dynamic-LDS limit, shared by every open
 // potential.
 hipError_t set_train_kernel_lds(size_t bytes)

@@ -29,6 +29,8 @@ size_t train_lds_bytes(int amc, int K, int R, int N, int nu_max, int n_parts);
 hipError_t set_train_kernel_lds(size_t bytes);
 hipError_t launch_train_grad(const DevPlan& P, const TrainArgs& A, size_t lds_bytes, double* scratch_partial,
                              double* xi_sum, double* typed_sum, double* grad, hipStream_t stream);
+// Recompute P's xi_of (device) from its moment coefficients (device).
+hipError_t launch_xi_of(const DevPlan& P, double* xi, hipStream_t stream);
 hipError_t launch_seg_totals(const double* Ei, const double* Wi, const int* conf_off, const double* volumes, int nconf,
                              double* E, double* S6, hipStream_t stream);
 

@@ -123,6 +123,23 @@ class MTPPotential:
         if scaling:
             self.params.scaling = float(scaling)
 
+    def set_params_device(self, theta, stream=None):
+        """Parameters from a device tensor theta = [moment, species, radial] (the
+        training order), stream-ordered (capturable in a graph).  self.params is
+        not updated."""
+        import torch
+
+        t = theta.detach()
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("theta must be a contiguous float64 device tensor")
+        S = len(self.params.moment_coeffs)
+        sp = self.params.species_count
+        if t.numel() != S + sp + int(np.prod(self.params.radial_coeffs.shape)):
+            raise ValueError("theta has the wrong length")
+        base = t.data_ptr()
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        capi.check(capi.lib().mtp_set_params_device(self.ctx, base, base + 8 * S, base + 8 * (S + sp), st), self.ctx)
+
     # ------------------------------------------------------------------
     def calc_energy_forces_stress_padded(self, itypes, all_js, all_rijs, all_jtypes, cell_rank, volume,
                                          natoms_energy, natoms_force, return_all: bool = False, sync: bool = True):

@@ -256,6 +256,45 @@ def lls_init(pot, dbatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
     return sol
 
 
+class GraphedLossGrad:
+    """One training evaluation theta -> (loss, gradient) replayed as a HIP graph.
+
+    Captured once: device parameter update, batch forward, the loss's
+    upstream gradients (elementwise), the hand-written parameter backward.
+    The loss value itself (three reductions) runs eagerly after the replay:
+    multi-block reductions of the framework are kept out of the graph.
+    Nothing in the captured region allocates, synchronises or memsets.
+    """
+
+    def __init__(self, pot, dbatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS):
+        import torch
+
+        self.pot, self.dbatch, self.weights = pot, dbatch, weights
+        self.E_ref, self.F_ref, self.S_ref = E_ref, F_ref, S_ref
+        self.theta = torch.tensor(theta_of(pot.params), dtype=torch.float64, device="cuda")
+        self._body()  # warm-up: allocates the library's training buffers outside the capture
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = self._body()
+
+    def _body(self):
+        wE, wF, wS = self.weights
+        self.pot.set_params_device(self.theta)
+        E, F, S = batch_forward(self.pot, self.dbatch)
+        gE = (2.0 * wE) * (E - self.E_ref)
+        gF = (2.0 * wF) * (F - self.F_ref)
+        gS = (2.0 * wS) * (S - self.S_ref)
+        grad = batch_param_grad(self.pot, self.dbatch, gE, gF, gS)
+        return E, F, S, grad
+
+    def __call__(self, theta):
+        self.theta.copy_(theta)
+        self.graph.replay()
+        E, F, S, grad = self.out
+        return loss_fn(E, F, S, self.E_ref, self.F_ref, self.S_ref, self.weights), grad
+
+
 class LBFGS:
     """Limited-memory BFGS with Armijo backtracking on a flat parameter tensor.
 
@@ -322,7 +361,7 @@ class Trainer:
     """
 
     def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0,
-                 lls: bool = True):
+                 lls: bool = True, graph: bool = False):
         import torch
 
         self.pot = pot
@@ -335,6 +374,7 @@ class Trainer:
         if lls:  # motep_jax_train_function.py:160-172: LLS for the moment coefficients first
             lls_init(pot, self.dbatch, self.E_ref, self.F_ref, self.S_ref, weights)
         self.theta = torch.tensor(theta_of(pot.params), dtype=torch.float64, device="cuda", requires_grad=True)
+        self.graphed = GraphedLossGrad(pot, self.dbatch, self.E_ref, self.F_ref, self.S_ref, weights) if graph else None
         self.opt = LBFGS(self.value_and_grad)
 
     def loss(self, theta=None):
@@ -345,6 +385,9 @@ class Trainer:
     def value_and_grad(self, theta):
         import torch
 
+        if self.graphed is not None:
+            L, g = self.graphed(theta.detach())
+            return float(L), g.clone()
         th = theta.detach().clone().requires_grad_(True)
         L = self.loss(th)
         (g,) = torch.autograd.grad(L, th)

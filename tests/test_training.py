@@ -138,3 +138,86 @@ def test_gpu_training_reduces_loss(jm):
     assert l0 < l_init
     assert l1 < l0
     pot.close()
+
+
+@pytest.mark.gpu
+def test_gpu_device_params_match_host(jm):
+    import torch
+
+    b, p, rng = _batch(jm, nconf=4, seed=11, level=16, species=2)
+    p1 = jm.configs.random_params(16, 2, 8, 1.6, 5.0, rng)
+    db = b.to_device()
+    A = jm.MTPPotential(p, max_atoms=b.natoms, max_nbr=b.nslots)
+    B = jm.MTPPotential(p, max_atoms=b.natoms, max_nbr=b.nslots)
+    A.set_params(p1.moment_coeffs, p1.species_coeffs, p1.radial_coeffs)
+    B.set_params_device(torch.tensor(jm.training.theta_of(p1), dtype=torch.float64, device="cuda"))
+    for x, y in zip(jm.training.batch_forward(A, db), jm.training.batch_forward(B, db)):
+        assert torch.equal(x, y)
+    A.close()
+    B.close()
+
+
+@pytest.mark.gpu
+def test_gpu_graphed_training_evaluation(jm):
+    """Loss and gradient from the captured graph equal the eager autograd path on
+    every replay, with eager GPU work between replays."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    cells = jm.training.random_cells(6, rng, natoms_range=(8, 40), species=2)
+    b = jm.training.pack_configs(cells, cutoff=5.0, max_nbr=96)
+    teacher = jm.configs.random_params(12, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    tp = jm.MTPPotential(teacher, max_atoms=b.natoms, max_nbr=b.nslots)
+    E_ref, F_ref, S_ref = [t.clone() for t in jm.training.batch_forward(tp, b.to_device())]
+    tp.close()
+    student = jm.configs.random_params(10, 2, 8, 1.6, 5.0, rng, sigma=0.05)
+    pot = jm.MTPPotential(student, max_atoms=b.natoms, max_nbr=b.nslots)
+    tr_e = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=False, graph=False)
+    tr_g = jm.training.Trainer(pot, b, E_ref, F_ref, S_ref, lls=False, graph=True)
+    theta0 = tr_e.theta.detach().clone()
+    for rep in range(4):
+        th = theta0 + 0.01 * rep * torch.randn_like(theta0)
+        Lg, gg = tr_g.value_and_grad(th)
+        junk = torch.randn(4096, device="cuda").sum()  # eager work between replays
+        Le, ge = tr_e.value_and_grad(th)
+        print(f"  replay {rep}: loss graph {Lg:.12e} eager {Le:.12e}", flush=True)
+        assert Lg == pytest.approx(Le, rel=1e-12)
+        assert float((gg - ge).abs().max() / ge.abs().max()) <= 1e-12
+        del junk
+    # a short L-BFGS run on the graphed evaluation still lowers the loss
+    l0 = tr_g.value_and_grad(tr_g.theta.detach())[0]
+    for _ in range(3):
+        l1 = tr_g.step()
+    assert l1 < l0
+    pot.close()
+
+
+@pytest.mark.gpu
+def test_gpu_graphed_force_evaluation(jm):
+    """mtp_compute_padded_device captured in a graph: each replay equals the eager result."""
+    import torch
+
+    w = jm.configs.fcc256_level10(0)
+    nb = w.neighbors()
+    pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=w.max_nbr)
+    dev = torch.device("cuda")
+    rij = torch.as_tensor(nb["rij"], dtype=torch.float64, device=dev)
+    it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
+    js = torch.as_tensor(nb["js"], dtype=torch.int32, device=dev)
+    jt = torch.as_tensor(nb["jtypes"], dtype=torch.int32, device=dev)
+    static_r = rij.clone()
+    args = lambda r: (it, js, r, jt, 3, w.volume, w.natoms, w.natoms)
+    pot.calc_energy_forces_stress_padded(*args(static_r), sync=False)  # warm-up
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        E_s, F_s, S_s = pot.calc_energy_forces_stress_padded(*args(static_r), sync=False)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    for rep in range(4):
+        r = rij + 0.01 * rep * torch.randn(rij.shape, dtype=torch.float64, device=dev, generator=gen)
+        static_r.copy_(r)
+        g.replay()
+        E_g, F_g, S_g = E_s.clone(), F_s.clone(), S_s.clone()
+        E_e, F_e, S_e = pot.calc_energy_forces_stress_padded(*args(r))
+        assert torch.equal(E_g, E_e) and torch.equal(F_g, F_e) and torch.equal(S_g, S_e), rep
+    pot.close()

@@ -361,11 +361,14 @@ class Trainer:
     """
 
     def __init__(self, pot, batch: PackedBatch, E_ref, F_ref, S_ref, weights=LOSS_WEIGHTS, lr=1.0,
-                 lls: bool = True, graph: bool = False):
+                 lls: bool = True, graph: bool = False, autotune: bool = True):
         import torch
 
         self.pot = pot
         self.dbatch = batch.to_device()
+        if autotune:  # kernel variant for this potential on this batch
+            d = self.dbatch
+            pot.autotune(d["itypes"], d["js"], d["rij"], d["jtypes"], reps=2)
         self.E_ref = torch.as_tensor(E_ref, dtype=torch.float64, device="cuda")
         self.F_ref = torch.as_tensor(F_ref, dtype=torch.float64, device="cuda")
         self.S_ref = torch.as_tensor(S_ref, dtype=torch.float64, device="cuda")

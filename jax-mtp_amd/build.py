@@ -19,6 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libmtp_hip.so"
+MD = PKG / "md" / "md_nve"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MTP_OFFLOAD_ARCH", "gfx950")
@@ -41,8 +42,9 @@ def _compile(src: str) -> pathlib.Path:
 
 def build(force: bool = True) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
-    deps = [CSRC / s for s in SOURCES + HEADERS] + [ROOT / "include" / "mtp_hip.h"]
-    if not force and LIB.exists() and LIB.stat().st_mtime >= max(d.stat().st_mtime for d in deps):
+    deps = [CSRC / s for s in SOURCES + HEADERS] + [ROOT / "include" / "mtp_hip.h", PKG / "md" / "md_nve.cpp"]
+    if (not force and LIB.exists() and MD.exists()
+            and min(LIB.stat().st_mtime, MD.stat().st_mtime) >= max(d.stat().st_mtime for d in deps)):
         return LIB
     try:
         with cf.ThreadPoolExecutor(max_workers=4) as ex:
@@ -51,6 +53,10 @@ def build(force: bool = True) -> pathlib.Path:
         subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)],
                        check=True, capture_output=True, text=True)
         os.replace(tmp, LIB)
+        # the standalone NVE driver over the pair style's host path (plain C++, links the library)
+        subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{CSRC}",
+                        str(PKG / "md" / "md_nve.cpp"), f"-L{PKG}", "-lmtp_hip", "-Wl,-rpath,$ORIGIN/..",
+                        "-o", str(MD)], check=True, capture_output=True, text=True)
     except subprocess.CalledProcessError as e:
         sys.stderr.write(e.stdout or "")
         sys.stderr.write(e.stderr or "")

@@ -14,7 +14,8 @@ The workloads are the named configurations of the evaluation path:
                        N=10, K=80, scaling=1.0.
 
 All parameters (moment, species, radial coefficients) are drawn N(0, 0.1)
-from the seed.  The level's alpha tables come from the untrained potential
+from the seed.  ``bcc2889_nve_params`` builds the level-16 potential of the
+NVE run on the same cell (physically scaled; see its docstring).  The level's alpha tables come from the untrained potential
 of that level with species_count and radial_basis_size overridden.
 """
 from __future__ import annotations
@@ -140,3 +141,70 @@ def small_cluster(level: int, species: int, natoms: int, seed: int, cell=8.0, cu
     types = rng.integers(0, species, natoms).astype(np.int32)
     params = random_params(level, species, radial_basis_size, 1.5, cutoff, rng)
     return Workload(f"cluster{natoms}_level{level}", params, pos, types, L, cutoff, max_nbr)
+
+
+def _morse_chebyshev(D, alpha, r0, rmin, rmax, N, scaling=1.0):
+    """Chebyshev coefficients c_n with scaling (rmax - r)^2 sum_n c_n T_n(x) ~ shifted-force Morse."""
+    r = np.linspace(rmin, rmax - 1e-3, 400)
+
+    def phi(x):
+        e = np.exp(-alpha * (x - r0))
+        return D * (e * e - 2.0 * e)
+
+    def dphi(x):
+        e = np.exp(-alpha * (x - r0))
+        return D * (-2.0 * alpha * e * e + 2.0 * alpha * e)
+
+    target = phi(r) - phi(rmax) - dphi(rmax) * (r - rmax)
+    g = target / (scaling * (rmax - r) ** 2)
+    x = (2.0 * r - (rmin + rmax)) / (rmax - rmin)
+    V = np.polynomial.chebyshev.chebvander(x, N - 1)
+    return np.linalg.lstsq(V, g, rcond=None)[0]
+
+
+def bcc2889_nve_params(seed: int = 0, many_body: float = 0.1) -> MTPData:
+    """Level-16, 2-species potential for 1 fs NVE dynamics on the 2889-atom BCC cell.
+
+    The N(0, 0.1) parameters of ``bcc2889_level16`` give RMS forces of ~4e3
+    eV/A, which no 1 fs velocity-Verlet run can follow.  Here the mu = 0
+    radial function of every species pair is a least-squares Chebyshev fit of
+    a shifted-force Morse pair potential (D 0.30/0.35/0.40 eV, alpha
+    1.5/A, r0 = 2.77 A, the BCC nearest-neighbour distance), carried by the
+    level-2 basis function with coefficient 1/2 (each pair is seen from both
+    atoms).  Every other basis function gets a random coefficient normalised
+    by its RMS on the starting cell, scaled so that these many-body terms
+    carry ``many_body`` times the RMS pair force.  Needs the GPU (basis values
+    and forces for the normalisation)."""
+    import torch
+
+    from .potential import MTPPotential
+
+    rng = np.random.default_rng(seed)
+    w = bcc2889_level16(seed)
+    p = w.params
+    D = np.array([[0.30, 0.35], [0.35, 0.40]])
+    rc = np.array(p.radial_coeffs)
+    for a in range(2):
+        for b in range(2):
+            rc[a, b, 0] = _morse_chebyshev(D[a, b], 1.5, 2.77, p.min_dist, p.max_dist, p.radial_basis_size,
+                                           p.scaling)
+    p.radial_coeffs = rc
+    p.species_coeffs = np.zeros(2)
+    S = len(p.moment_coeffs)
+    nb = w.neighbors()
+    pot = MTPPotential(p, max_atoms=w.natoms, max_nbr=w.max_nbr)
+    pair = np.zeros(S)
+    pair[0] = 0.5
+    pot.set_params(moment_coeffs=pair)
+    args = (nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3, w.volume, w.natoms, w.natoms)
+    f_pair = float(torch.sqrt((pot.calc_energy_forces_stress_padded(*args)[1] ** 2).mean()))
+    b = pot.descriptor(nb["itypes"], nb["rij"], nb["jtypes"])
+    rms = torch.sqrt((b ** 2).mean(0)).cpu().numpy()
+    mb = np.where(rms > 0, rng.normal(0.0, 1.0, S) / np.maximum(rms, 1e-300), 0.0)
+    mb[0] = 0.0
+    pot.set_params(moment_coeffs=mb)
+    f_mb = float(torch.sqrt((pot.calc_energy_forces_stress_padded(*args)[1] ** 2).mean()))
+    pot.close()
+    p.moment_coeffs = pair + mb * (many_body * f_pair / max(f_mb, 1e-300))
+    p.potential_name = "bcc2889_level16_nve"
+    return p

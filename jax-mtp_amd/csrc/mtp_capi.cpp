@@ -243,6 +243,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     DevPlan& d = c->dev;
     d.Sp = f.species_count; d.R = f.radial_funcs_count; d.N = f.radial_basis_size;
     d.B = f.B(); d.T = f.T(); d.amc = f.alpha_moments_count; d.S = f.S(); d.nu_max = p.nu_max;
+    d.n_live = p.n_live;
     d.scaling = f.scaling; d.min_dist = f.min_dist; d.max_dist = f.max_dist;
     d.radial_coeffs = c->d_radial; d.species_coeffs = c->d_species; d.moment_coeffs = c->d_moment;
     d.mapping = mapping; d.xi_of = c->d_xi;
@@ -322,13 +323,13 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                  &dv.rev_items, &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo, &dv.rev_xa);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host tables die with this iteration
             dv.max_partials = ip.max_partials;
-            const size_t lds = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
-                                               dv.n_parts, nt, ip.max_partials);
+            const size_t lds = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                                               p.nu_max, dv.n_parts, nt, ip.max_partials);
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
                 c->vdev[v] = dv;
                 c->vlds[v] = lds;
-                const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max,
-                                                   dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
+                const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                                                   p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
                 if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
                     c->vlds_xi[v] = lxi;
             }

@@ -270,34 +270,47 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // threads of a wave read consecutive 8-byte entries (coalesced), and each
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int K, int R, int nu_max, int n_parts, int nt,
-                                                    int max_partials, int xi_len)
+// LDS of the balanced-item kernels (doubles).  Regions are reused across
+// phases:
+//   m   [mlen]  moments [0, n_live) (basic or read as operands); in phase F
+//               the per-part gradients and the recomputed f, f'
+//   dm  [dlen]  A/B: power table, then f; C/D: the values of the
+//               never-operand moments (m[x] for x >= n_live aliases
+//               dm[x - n_live], dead after D) and, at the end, the energy
+//               reduction; E/F: adjoints
+//   part, vx, vy, vz, r, xl (per-row coefficients), ints
+__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
+                                                    int nt, int max_partials, int xi_len)
 {
     AtomLds2 L;
-    int mlen = amc;
-    if (mlen < n_parts * K * 3) mlen = n_parts * K * 3;
+    const int pw_len = K * 3 * (nu_max + 1);
+    int mlen = n_live;
+    if (mlen < n_parts * K * 3 + 2 * R * K) mlen = n_parts * K * 3 + 2 * R * K;
     int dlen = amc;
-    if (dlen < K * 3 * (nu_max + 1)) dlen = K * 3 * (nu_max + 1);
+    if (dlen < pw_len + R * K) dlen = pw_len + R * K;
+    if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
     L.m = 0;
     L.dm = L.m + mlen;
+    L.f = L.dm + pw_len;
+    L.gf = L.m + n_parts * K * 3;
+    L.gfp = L.gf + R * K;
+    L.red = L.dm + dlen - nt;
     L.part = L.dm + dlen;
-    L.f = L.part + (max_partials > 0 ? max_partials : 1);
-    L.fp = L.f + R * K;
-    L.vx = L.fp + R * K;
+    L.vx = L.part + (max_partials > 0 ? max_partials : 1);
     L.vy = L.vx + K;
     L.vz = L.vy + K;
     L.r = L.vz + K;
-    L.red = L.r + K;
-    L.xl = L.red + nt;
+    L.xl = L.r + K;
     L.ints = L.xl + xi_len;
     L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
     return L;
 }
 
-size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials, int xi_len)
+size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,
+                       int xi_len)
 {
     return sizeof(double) *
-           (size_t)atom_lds2_layout(amc, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
+           (size_t)atom_lds2_layout(amc, n_live, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
 }
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
@@ -390,13 +403,12 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int tid = threadIdx.x;
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
-    const AtomLds2 L = atom_lds2_layout(P.amc, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
+    const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
                                         A.xi_atom ? P.amc : 0);
     double* m = lds + L.m;
     double* dm = lds + L.dm;
     double* part = lds + L.part;
     double* f = lds + L.f;
-    double* fp = lds + L.fp;
     double* vx = lds + L.vx;
     double* vy = lds + L.vy;
     double* vz = lds + L.vz;
@@ -437,7 +449,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
                 radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
                 for (int mu = 0; mu < kMaxR; ++mu)
-                    if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
+                    if (mu < P.R) f[mu * K + pos] = fv[mu];
                 vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
                 slot_of[pos] = k;
             }
@@ -466,7 +478,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
             for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
+                if (mu < P.R) f[mu * K + pos] = fv[mu];
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             slot_of[pos] = k;
         }
@@ -478,7 +490,6 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         __syncthreads();
         nv = *nvp;
     }
-    for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;  // moments that are never targets stay 0
 
     // ---- B: basic moments (power table in the dm region)
     const int np1 = P.nu_max + 1;
@@ -517,6 +528,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         for (int mu = 0; mu < kMaxR; ++mu)
             if (mu <= mumax) m[P.blk_off[mu][nu] + alpha] = acc[mu];
     }
+    __syncthreads();
+    // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
+    // region, free once the power table and f are consumed)
+    for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
     __syncthreads();
 
     // ---- C: contraction layers (depth 1 .. n_depth)
@@ -560,8 +575,19 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     }
     for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, m, dm, part, tid, xi);
 
-    // ---- F: per-slot gradient, thread = (slot, part)
+    // ---- F: per-slot gradient, thread = (slot, part).  m is dead: it holds
+    // the per-part gradients and f, f' recomputed for the valid slots.
     double* gpart = m;
+    double* gf = lds + L.gf;
+    double* gfp = lds + L.gfp;
+    for (int c = tid; c < nv; c += NT) {
+        double fv[kMaxR], dfv[kMaxR];
+        radial_slot<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], fv, dfv);
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu)
+            if (mu < P.R) { gf[mu * K + c] = fv[mu]; gfp[mu * K + c] = dfv[mu]; }
+    }
+    __syncthreads();
     {
         const int kk = tid & (kWave - 1), prt = tid / kWave;
         for (int k = kk; k < nv && prt < P.n_parts; k += kWave) {
@@ -577,7 +603,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
                 const I2 bl = P.part_blk[bi];
                 const int mu = bl.x, nu = bl.y;
                 const double* d = dm + P.blk_off[mu][nu];
-                const double fm = f[mu * K + k], fpm = fp[mu * K + k];
+                const double fm = gf[mu * K + k], fpm = gfp[mu * K + k];
                 if (nu == 0) {
                     sdf += fpm * d[0];
                 } else {
@@ -619,9 +645,11 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     }
 }
 
-// One workgroup per row.
+// One workgroup per row.  512-thread workgroups are compiled for two per CU
+// (<= 128 VGPRs; the LDS layout keeps level 24 under half of the CU's LDS).
 template <typename RadialReal, int NT>
-__global__ void __launch_bounds__(NT) atom_kernel_v2(const DevPlan P, AtomArgs A)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 1)))
+atom_kernel_v2(const DevPlan P, AtomArgs A)
 {
     extern __shared__ double lds[];
     atom_row_v2<RadialReal, NT>(P, A, blockIdx.x, lds);

@@ -44,7 +44,7 @@ struct AtomLds {
 };
 
 struct AtomLds2 {
-    int m, dm, part, f, fp, vx, vy, vz, r, red, xl, ints, total_doubles;
+    int m, dm, part, f, gf, gfp, vx, vy, vz, r, red, xl, ints, total_doubles;
 };
 
 // Atom-kernel variants (chosen per potential by the launch autotuner):
@@ -61,7 +61,8 @@ constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : (v == 
 constexpr int variant_parts(int v) { return v == kVariantV2x512 ? 8 : (v == kVariantV2x64 ? 1 : 4); }
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
-size_t atom_lds2_bytes(int amc, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials, int xi_len = 0);
+size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,
+                       int xi_len = 0);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.

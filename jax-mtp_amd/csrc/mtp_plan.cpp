@@ -14,11 +14,45 @@ void refresh_plan_params(HostPlan& p)
     for (int s = 0; s < f.S(); ++s) p.xi_of[f.alpha_moment_mapping[s]] += f.moment_coeffs[s];
 }
 
-bool build_plan(const MtpFile& f, int n_parts, HostPlan& p, std::string& err)
+// Renumber the non-basic moments so that those never read as an operand
+// come last.  Their values are needed only for the energy and the basis
+// values (phase D), so the balanced-item kernels keep them in LDS that the
+// adjoint pass reuses later.  Entry order and the dependency order are
+// unchanged; basic moments keep their indices.  Returns the number of
+// moments that are basic or read as an operand.
+static int renumber_never_operands(MtpFile& f)
 {
-    if (!validate_mtp(f, err)) return false;
+    const int amc = f.alpha_moments_count, B = f.B(), T = f.T();
+    std::vector<char> operand(amc, 0);
+    for (int e = 0; e < T; ++e) {
+        operand[f.alpha_index_times[4 * e]] = 1;
+        operand[f.alpha_index_times[4 * e + 1]] = 1;
+    }
+    std::vector<int> perm(amc);
+    int next = 0;
+    for (int x = 0; x < B; ++x) perm[x] = next++;
+    for (int x = B; x < amc; ++x)
+        if (operand[x]) perm[x] = next++;
+    const int n_live = next;
+    for (int x = B; x < amc; ++x)
+        if (!operand[x]) perm[x] = next++;
+    for (int e = 0; e < T; ++e) {
+        int* t = &f.alpha_index_times[4 * e];
+        t[0] = perm[t[0]];
+        t[1] = perm[t[1]];
+        t[3] = perm[t[3]];
+    }
+    for (int& s : f.alpha_moment_mapping) s = perm[s];
+    return n_live;
+}
+
+bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
+{
+    if (!validate_mtp(f_in, err)) return false;
     p = HostPlan();
-    p.file = f;
+    p.file = f_in;
+    p.n_live = renumber_never_operands(p.file);
+    const MtpFile& f = p.file;
     p.level = level_from_moments_count(f.alpha_moments_count);
     const int amc = f.alpha_moments_count, B = f.B(), T = f.T(), R = f.radial_funcs_count;
     if (R > kMaxR) { err = "radial_funcs_count above " + std::to_string(kMaxR) + " is not supported"; return false; }

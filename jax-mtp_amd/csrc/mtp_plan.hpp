@@ -38,6 +38,7 @@ constexpr int kMaxDepth = 8;  // contraction layers supported
 // Device-side view of the plan (all pointers are device pointers).
 struct DevPlan {
     int Sp, R, N, B, T, amc, S, nu_max;
+    int n_live;                      // moments [0, n_live) are basic or read as operands
     double scaling, min_dist, max_dist;
     const double* radial_coeffs;   // [Sp][Sp][R][N]
     const double* species_coeffs;  // [Sp]
@@ -106,8 +107,9 @@ struct ItemPlan {
 };
 
 struct HostPlan {
-    MtpFile file;
+    MtpFile file;           // moments renumbered: never-operand moments last (see build_plan)
     int level = -1;
+    int n_live = 0;         // basic moments + moments read as an operand
     int nu_max = 0, n_depth = 0, n_parts = 4;
     std::vector<int> depth;  // [amc]
     std::vector<I4> mono;

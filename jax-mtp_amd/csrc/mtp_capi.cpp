@@ -183,6 +183,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);  // default priority
     // plan buffers
     auto* mono = c->alloc<I4>(p.mono.size(), e);
+    auto* mono_dst = c->alloc<I4>(p.mono_dst.size(), e);
     auto* fwd_tgt = c->alloc<int>(p.fwd_tgt.size(), e);
     auto* fwd_beg = c->alloc<int>(p.fwd_beg.size(), e);
     auto* fwd_pair = c->alloc<uint32_t>(p.fwd_pair.size(), e);
@@ -222,6 +223,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_F, A * 3 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_out, 16 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = upload(mono, p.mono, c->stream);
+    if (e == hipSuccess) e = upload(mono_dst, p.mono_dst, c->stream);
     if (e == hipSuccess) e = upload(fwd_tgt, p.fwd_tgt, c->stream);
     if (e == hipSuccess) e = upload(fwd_beg, p.fwd_beg, c->stream);
     if (e == hipSuccess) e = upload(fwd_pair, p.fwd_pair, c->stream);
@@ -249,7 +251,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.mapping = mapping; d.xi_of = c->d_xi;
     std::memcpy(d.blk_off, p.blk_off, sizeof(d.blk_off));
     std::memcpy(d.mu_max_of_nu, p.mu_max_of_nu, sizeof(d.mu_max_of_nu));
-    d.n_mono = (int)p.mono.size(); d.mono = mono;
+    d.n_mono = (int)p.mono.size(); d.mono = mono; d.mono_dst = mono_dst;
     d.n_depth = p.n_depth;
     for (int q = 0; q < kMaxDepth + 2; ++q) { d.fwd_layer[q] = p.fwd_layer[q]; d.rev_layer[q] = p.rev_layer[q]; }
     d.fwd_tgt = fwd_tgt; d.fwd_beg = fwd_beg; d.fwd_pair = fwd_pair; d.fwd_mult = fwd_mult;
@@ -257,6 +259,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.n_parts = p.n_parts;
     for (int q = 0; q < 9; ++q) d.part_beg[q] = p.part_beg[q];
     d.part_blk = part_blk;
+    d.k_layout = max_nbr;
     d.n_mapped = (int)p.map_x.size(); d.map_x = map_x; d.map_beg = map_beg; d.map_s = map_s;
 
     // --- kernel variants ------------------------------------------------
@@ -279,50 +282,43 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                 for (int q = 0; q < 9; ++q) dv.part_beg[q] = pp.part_beg[q];
                 dv.part_blk = pb;
             }
+            // LDS offsets do not depend on the partial-sum count; keep the
+            // partials of a layer within what leaves room for two workgroups
+            // per CU when the rest of the layout allows it
+            const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                                                 p.nu_max, dv.n_parts, nt, 0, 0);
+            const int half_cu = (int)(lds_max / 2 / sizeof(double));
+            const int part_budget = std::max(64, half_cu - L0.total_doubles);
             ItemPlan ip;
-            build_items(*pv, nt, ip);
-            auto pack = [&](const std::vector<LayerItems>& layers, const std::vector<U2>& table, int* it_beg,
-                            int* cb_beg, int* tbl_base, int* bo_beg, int* xa_beg, const I4** d_items,
-                            const I4** d_comb, const U2** d_tbl, const int** d_bo, const int** d_xa) {
-                std::vector<I4> items, comb;
-                std::vector<int> bo, xa;
+            std::string ierr;
+            if (!build_items(*pv, nt, L0.dm, L0.part, part_budget, ip, ierr)) continue;  // variant unavailable
+            auto pack = [&](const StreamPlan& sp, int* cb_beg, int* bo_beg, int* xa_beg, const I4** d_wave,
+                            const U4** d_tbl, const I4** d_comb, const int** d_bo, const int** d_xa) {
                 for (int q = 0; q < kMaxDepth + 2; ++q) {
-                    it_beg[q] = 0; cb_beg[q] = 0;
-                    if (bo_beg) bo_beg[q] = 0;
-                    if (xa_beg) xa_beg[q] = 0;
+                    cb_beg[q] = sp.cb_beg[q];
+                    if (bo_beg) bo_beg[q] = sp.bo_beg[q];
+                    if (xa_beg) xa_beg[q] = sp.xa_beg[q];
                 }
-                for (size_t L = 0; L < kMaxDepth + 2; ++L) {
-                    it_beg[L] = (int)items.size();
-                    cb_beg[L] = (int)comb.size();
-                    if (bo_beg) bo_beg[L] = (int)bo.size();
-                    if (xa_beg) xa_beg[L] = (int)xa.size();
-                    if (L >= layers.size()) continue;
-                    tbl_base[L] = layers[L].table_base;
-                    items.insert(items.end(), layers[L].items.begin(), layers[L].items.end());
-                    comb.insert(comb.end(), layers[L].comb.begin(), layers[L].comb.end());
-                    bo.insert(bo.end(), layers[L].base_only.begin(), layers[L].base_only.end());
-                    xa.insert(xa.end(), layers[L].xi_add.begin(), layers[L].xi_add.end());
-                }
-                I4* di = c->alloc<I4>(std::max<size_t>(1, items.size()), e);
-                I4* dc = c->alloc<I4>(std::max<size_t>(1, comb.size()), e);
-                U2* dt = c->alloc<U2>(std::max<size_t>(1, table.size()), e);
-                int* db = c->alloc<int>(std::max<size_t>(1, bo.size()), e);
-                int* dx = c->alloc<int>(std::max<size_t>(1, xa.size()), e);
-                if (e == hipSuccess) e = upload(di, items, c->stream);
-                if (e == hipSuccess) e = upload(dc, comb, c->stream);
-                if (e == hipSuccess) e = upload(dt, table, c->stream);
-                if (e == hipSuccess) e = upload(db, bo, c->stream);
-                if (e == hipSuccess) e = upload(dx, xa, c->stream);
-                *d_items = di; *d_comb = dc; *d_tbl = dt;
+                I4* dw = c->alloc<I4>(std::max<size_t>(1, sp.wave.size()), e);
+                U4* dt = c->alloc<U4>(std::max<size_t>(1, sp.tbl.size()), e);
+                I4* dc = c->alloc<I4>(std::max<size_t>(1, sp.comb.size()), e);
+                int* db = c->alloc<int>(std::max<size_t>(1, sp.bo.size()), e);
+                int* dx = c->alloc<int>(std::max<size_t>(1, sp.xa.size()), e);
+                if (e == hipSuccess) e = upload(dw, sp.wave, c->stream);
+                if (e == hipSuccess) e = upload(dt, sp.tbl, c->stream);
+                if (e == hipSuccess) e = upload(dc, sp.comb, c->stream);
+                if (e == hipSuccess) e = upload(db, sp.bo, c->stream);
+                if (e == hipSuccess) e = upload(dx, sp.xa, c->stream);
+                *d_wave = dw; *d_tbl = dt; *d_comb = dc;
                 if (d_bo) *d_bo = db;
                 if (d_xa) *d_xa = dx;
             };
-            pack(ip.fwd, ip.fwd_table, dv.fwd_it_beg, dv.fwd_cb_beg, dv.fwd_tbl_base, nullptr, nullptr,
-                 &dv.fwd_items, &dv.fwd_comb, &dv.fwd_tbl, nullptr, nullptr);
-            pack(ip.rev, ip.rev_table, dv.rev_it_beg, dv.rev_cb_beg, dv.rev_tbl_base, dv.rev_bo_beg, dv.rev_xa_beg,
-                 &dv.rev_items, &dv.rev_comb, &dv.rev_tbl, &dv.rev_bo, &dv.rev_xa);
+            pack(ip.fwd, dv.fwd_cb_beg, nullptr, nullptr, &dv.fwd_wave, &dv.fwd_tbl, &dv.fwd_comb, nullptr, nullptr);
+            pack(ip.rev, dv.rev_cb_beg, dv.rev_bo_beg, dv.rev_xa_beg, &dv.rev_wave, &dv.rev_tbl, &dv.rev_comb,
+                 &dv.rev_bo, &dv.rev_xa);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host tables die with this iteration
             dv.max_partials = ip.max_partials;
+            dv.k_layout = max_nbr;
             const size_t lds = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
                                                p.nu_max, dv.n_parts, nt, ip.max_partials);
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {

@@ -6,7 +6,14 @@
 
 namespace mtp {
 
-constexpr int kWave = 64;
+// Sum over the 64 lanes of a wavefront (butterfly; every lane gets the same
+// value, in a fixed order).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
 
 __device__ __forceinline__ int wrap_species(int t, int Sp)
 {
@@ -123,29 +130,32 @@ __device__ __forceinline__ void grad_block(const double* __restrict__ d, const d
 // products only depend on the slot, so the compiler would hoist all of them
 // (every order 1..12) out of that loop and spill; the empty asm marks the
 // power tables as rewritten each time, which keeps the products per block.
+// PN is the length of the power tables (orders nu <= PN are handled).
+template <int PN = kMaxNu + 1>
 __device__ __forceinline__ void opaque_powers(double* px, double* py, double* pz)
 {
 #pragma unroll
-    for (int p = 0; p <= kMaxNu; ++p) asm volatile("" : "+v"(px[p]), "+v"(py[p]), "+v"(pz[p]));
+    for (int p = 0; p < PN; ++p) asm volatile("" : "+v"(px[p]), "+v"(py[p]), "+v"(pz[p]));
 }
 
+template <int PN = kMaxNu + 1>
 __device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, double* px, double* py, double* pz,
                                                     double& Hx, double& Hy, double& Hz)
 {
-    opaque_powers(px, py, pz);
+    opaque_powers<PN>(px, py, pz);
     switch (nu) {
         case 1: grad_block<1>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 2: grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 3: grad_block<3>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 4: grad_block<4>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 5: grad_block<5>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 6: grad_block<6>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 7: grad_block<7>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 8: grad_block<8>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 9: grad_block<9>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 10: grad_block<10>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 11: grad_block<11>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 12: grad_block<12>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 2: if constexpr (PN >= 2) grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 3: if constexpr (PN >= 3) grad_block<3>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 4: if constexpr (PN >= 4) grad_block<4>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 5: if constexpr (PN >= 5) grad_block<5>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 6: if constexpr (PN >= 6) grad_block<6>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 7: if constexpr (PN >= 7) grad_block<7>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 8: if constexpr (PN >= 8) grad_block<8>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 9: if constexpr (PN >= 9) grad_block<9>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 10: if constexpr (PN >= 10) grad_block<10>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 11: if constexpr (PN >= 11) grad_block<11>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 12: if constexpr (PN >= 12) grad_block<12>(d, px, py, pz, Hx, Hy, Hz); break;
         default: break;
     }
 }

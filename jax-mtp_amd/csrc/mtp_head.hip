@@ -33,13 +33,6 @@ __device__ __forceinline__ double bcast(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 // silu(a) = a * sigmoid(a) and its first two derivatives
 struct Silu {
     double h, s, q;

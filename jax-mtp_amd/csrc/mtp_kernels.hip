@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
     __syncthreads();
     for (int t = tid; t < P.n_mono; t += kBlock) {
         const I4 mo = P.mono[t];
-        const int nu = mo.w & 0xff, alpha = mo.w >> 8;
+        const int nu = mo.w & 0xff, alpha = (mo.w >> 8) & 0xffff;
         const int mumax = P.mu_max_of_nu[nu];
         double acc[kMaxR];
 #pragma unroll
@@ -216,8 +216,8 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
             const int b1 = P.part_beg[part + 1];
             for (int bi = P.part_beg[part]; bi < b1; ++bi) {
                 const I2 bl = P.part_blk[bi];
-                const int mu = bl.x, nu = bl.y;
-                const double* d = dm + P.blk_off[mu][nu];
+                const int mu = bl.x & 0xff, nu = bl.y;
+                const double* d = dm + (bl.x >> 8);
                 const double fm = f[mu * K + k], fpm = fp[mu * K + k];
                 if (nu == 0) {
                     sdf += fpm * d[0];
@@ -270,42 +270,6 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // threads of a wave read consecutive 8-byte entries (coalesced), and each
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
-// LDS of the balanced-item kernels (doubles).  Regions are reused across
-// phases:
-//   m   [mlen]  moments [0, n_live) (basic or read as operands); in phase F
-//               the per-part gradients and the recomputed f, f'
-//   dm  [dlen]  A/B: power table, then f; C/D: the values of the
-//               never-operand moments (m[x] for x >= n_live aliases
-//               dm[x - n_live], dead after D) and, at the end, the energy
-//               reduction; E/F: adjoints
-//   part, vx, vy, vz, r, xl (per-row coefficients), ints
-__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
-                                                    int nt, int max_partials, int xi_len)
-{
-    AtomLds2 L;
-    const int pw_len = K * 3 * (nu_max + 1);
-    int mlen = n_live;
-    if (mlen < n_parts * K * 3 + 2 * R * K) mlen = n_parts * K * 3 + 2 * R * K;
-    int dlen = amc;
-    if (dlen < pw_len + R * K) dlen = pw_len + R * K;
-    if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
-    L.m = 0;
-    L.dm = L.m + mlen;
-    L.f = L.dm + pw_len;
-    L.gf = L.m + n_parts * K * 3;
-    L.gfp = L.gf + R * K;
-    L.red = L.dm + dlen - nt;
-    L.part = L.dm + dlen;
-    L.vx = L.part + (max_partials > 0 ? max_partials : 1);
-    L.vy = L.vx + K;
-    L.vz = L.vy + K;
-    L.r = L.vz + K;
-    L.xl = L.r + K;
-    L.ints = L.xl + xi_len;
-    L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
-    return L;
-}
-
 size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,
                        int xi_len)
 {
@@ -315,78 +279,89 @@ size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
 
-// One contraction layer over its work items.  Each thread walks its item's
-// entries in batches of kUB, with the table loads of the next batch in
-// flight while the current batch is processed (the table is L2-resident and
-// would otherwise cost one full memory latency per entry).  Loads of a batch
-// are issued before its stores: a layer's stores never alias its operands
-// (targets have a higher depth than operands in the forward pass, a lower
-// depth than the targets they read in the reverse pass).
-constexpr int kUB = 8;
+// One contraction layer.  Each wavefront walks its own entry stream (two
+// entries per lane and step, see DevPlan), with the table loads of the next
+// kStreamUnroll steps in flight while the current ones are summed (two
+// register sets used in turn).  A group ends on the same step on all lanes
+// (flagged in the step's fourth word), so the loop has no per-lane control
+// flow.  Operand offsets are byte offsets: a from the LDS base, b from it
+// (forward) or from the adjoints (reverse).  A layer's stores never alias
+// its operands (targets have a higher depth than their operands in the
+// forward pass, a lower depth than the targets they read in the reverse
+// pass).
+template <bool REV>
+__device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, const char* bb, char* lb, double& acc)
+{
+    constexpr int U = kStreamUnroll;
+    double a0[U], b0[U], a1[U], b1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        a0[u] = *reinterpret_cast<const double*>(ab + (q[u].x & 0xffffu));
+        b0[u] = *reinterpret_cast<const double*>(bb + (q[u].x >> 16));
+        a1[u] = *reinterpret_cast<const double*>(ab + (q[u].z & 0xffffu));
+        b1[u] = *reinterpret_cast<const double*>(bb + (q[u].z >> 16));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        acc = fma((double)__uint_as_float(q[u].y) * a0[u], b0[u], acc);
+        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
+            acc = fma((double)__uint_as_float(q[u].w) * a1[u], b1[u], acc);
+        } else {  // group end (wave-uniform): q.z is this lane's destination
+            *reinterpret_cast<double*>(lb + q[u].z) = acc;
+            acc = 0.0;
+        }
+    }
+}
 
 template <int NT, bool REV>
-__device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* m, double* dm, double* part,
+__device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* lds, double* dm, double* part,
                                                   int tid, const double* xi)
 {
-    const int i0 = REV ? P.rev_it_beg[L] : P.fwd_it_beg[L];
-    const int ni = (REV ? P.rev_it_beg[L + 1] : P.fwd_it_beg[L + 1]) - i0;
-    const I4* items = (REV ? P.rev_items : P.fwd_items) + i0;
-    const U2* tbl = (REV ? P.rev_tbl : P.fwd_tbl) + (REV ? P.rev_tbl_base[L] : P.fwd_tbl_base[L]);
-    double* dst = REV ? dm : m;
-    const double* second = REV ? dm : m;
-    for (int it = tid; it < ni; it += NT) {
-        const I4 w = items[it];
-        const int len = w.x;
-        double* pdst = w.z >= 0 ? part + w.z : dst - 1;  // dest + 1 is encoded in the entries
-        const bool chunk = w.z >= 0;
-        const U2* col = tbl + it;
-        U2 nxt[kUB];
-#pragma unroll
-        for (int u = 0; u < kUB; ++u) nxt[u] = u < len ? col[(size_t)u * ni] : U2{0u, 0u};
+    constexpr int NW = NT / kWave;
+    constexpr int U = kStreamUnroll;
+    const int lane = tid & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
+    const int nsteps = wv.y;
+    if (nsteps > 0) {
+        const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
+        char* lb = reinterpret_cast<char*>(lds);
+        const char* bb = REV ? reinterpret_cast<const char*>(dm) : lb;
         double acc = 0.0;
-        for (int j0 = 0; j0 < len; j0 += kUB) {
-            U2 q[kUB];
+        uint4 qa[U], qb[U];
 #pragma unroll
-            for (int u = 0; u < kUB; ++u) q[u] = nxt[u];
+        for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)u * kWave];
+        for (int s0 = 0;;) {
+            if (s0 + U < nsteps) {
 #pragma unroll
-            for (int u = 0; u < kUB; ++u) {
-                const int j = j0 + kUB + u;
-                nxt[u] = j < len ? col[(size_t)j * ni] : U2{0u, 0u};
+                for (int u = 0; u < U; ++u) qb[u] = tb[(size_t)(s0 + U + u) * kWave];
             }
-            double a[kUB], b[kUB];
+            stream_steps<REV>(qa, lb, bb, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
+            if (s0 + U < nsteps) {
 #pragma unroll
-            for (int u = 0; u < kUB; ++u) {
-                a[u] = m[q[u].x & 0xffffu];
-                b[u] = second[q[u].x >> 16];
+                for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)(s0 + U + u) * kWave];
             }
-#pragma unroll
-            for (int u = 0; u < kUB; ++u) {
-                if (j0 + u < len) {
-                    acc += (double)(q[u].y & 0xffffu) * a[u] * b[u];
-                    const uint32_t d = q[u].y >> 16;
-                    if (d) {
-                        pdst[chunk ? 0 : d] = acc;
-                        acc = 0.0;
-                    }
-                }
-            }
+            stream_steps<REV>(qb, lb, bb, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
         }
     }
     __syncthreads();
     const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
     const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
-    const int* mom = REV ? P.rev_x : P.fwd_tgt;
     const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    double* dst = REV ? dm : lds + kLdsM;
     for (int c = c0 + tid; c < c1; c += NT) {
         const I4 cb = comb[c];
-        const int x = mom[cb.x];
-        double acc = REV ? xi[x] : 0.0;
+        double acc = REV ? xi[cb.x] : 0.0;
         for (int p = cb.y; p < cb.z; ++p) acc += part[p];
-        dst[x] = acc;
+        dst[cb.x] = acc;
     }
     if (REV) {
         for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
-            const int x = mom[P.rev_bo[t]];
+            const int x = P.rev_bo[t];
             dm[x] = xi[x];
         }
         for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
@@ -397,14 +372,78 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
     if (c1 > c0 || REV) __syncthreads();
 }
 
+// Gradient of one slot k over the (mu, nu) blocks of part prt: g_k from the
+// adjoints dm of the basic moments.  H = sum_alpha dm_alpha grad_u u^alpha
+// over the monomials of degree nu - 1 (power tables of PN orders, enough
+// for nu <= PN), u . H = nu sum_alpha dm_alpha u^alpha (Euler).
+template <int PN>
+__device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, int K, const double* dm,
+                                              const double* gf, const double* gfp, const double* vx,
+                                              const double* vy, const double* vz, const double* rr, double* gpart)
+{
+    const double ir = 1.0 / rr[k];
+    const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
+    double px[PN], py[PN], pz[PN];
+    px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+    for (int p = 1; p < PN; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
+    double sdf = 0.0, sfn = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
+    const int b1 = P.part_beg[prt + 1];
+    for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
+        const I2 bl = P.part_blk[bi];
+        const int mu = bl.x & 0xff, nu = bl.y;
+        const double* d = dm + (bl.x >> 8);
+        const double fm = gf[mu * K + k], fpm = gfp[mu * K + k];
+        if (nu == 0) {
+            sdf += fpm * d[0];
+        } else {
+            double Hx = 0.0, Hy = 0.0, Hz = 0.0;
+            grad_block_dispatch<PN>(nu, d, px, py, pz, Hx, Hy, Hz);
+            const double uH = ux * Hx + uy * Hy + uz * Hz;
+            sdf += fpm * uH * (1.0 / (double)nu);
+            sfn += fm * uH;
+            Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
+        }
+    }
+    double* gp = gpart + ((size_t)prt * K + k) * 3;
+    gp[0] = sdf * ux + (Gx - sfn * ux) * ir;
+    gp[1] = sdf * uy + (Gy - sfn * uy) * ir;
+    gp[2] = sdf * uz + (Gz - sfn * uz) * ir;
+}
+
+// sum_k f_mu(k) * u_x^a u_y^b u_z^c over the valid slots for mu < NMU
+// (powers px[k], py[k], pz[k]; f[k * kMaxR + mu]), four slots per step.
+template <int NMU>
+__device__ __forceinline__ void basic_moment_sum(const double* px, const double* py, const double* pz,
+                                                 const double* f, int nv, double* acc)
+{
+    int k = 0;
+    for (; k + 4 <= nv; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double Pm = px[k + u] * py[k + u] * pz[k + u];
+            const double* fr = f + (k + u) * kMaxR;
+#pragma unroll
+            for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
+        }
+    }
+    for (; k < nv; ++k) {
+        const double Pm = px[k] * py[k] * pz[k];
+        const double* fr = f + k * kMaxR;
+#pragma unroll
+        for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
+    }
+}
+
 template <typename RadialReal, int NT>
 __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A, const int row, double* lds)
 {
     const int tid = threadIdx.x;
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
-    const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, K, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
+    const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, P.k_layout, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
                                         A.xi_atom ? P.amc : 0);
+    if (tid < kLdsM) lds[tid] = 0.0;  // operand of the padding entries, sink of idle lanes
     double* m = lds + L.m;
     double* dm = lds + L.dm;
     double* part = lds + L.part;
@@ -449,11 +488,12 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
                 radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
                 for (int mu = 0; mu < kMaxR; ++mu)
-                    if (mu < P.R) f[mu * K + pos] = fv[mu];
+                    if (mu < P.R) f[pos * kMaxR + mu] = fv[mu];
                 vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
                 slot_of[pos] = k;
             }
             for (int w = 0; w < NW; ++w) nv += wave_cnt[w];
+            nv = __builtin_amdgcn_readfirstlane(nv);
             __syncthreads();
         }
     } else {  // K <= NT: one slot per thread
@@ -478,7 +518,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
             for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) f[mu * K + pos] = fv[mu];
+                if (mu < P.R) f[pos * kMaxR + mu] = fv[mu];
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             slot_of[pos] = k;
         }
@@ -488,45 +528,50 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             *nvp = tot;
         }
         __syncthreads();
-        nv = *nvp;
+        nv = __builtin_amdgcn_readfirstlane(*nvp);
     }
 
-    // ---- B: basic moments (power table in the dm region)
+    // ---- B: basic moments m[mu, alpha] = sum_k f_mu(k) u_k^alpha.  The
+    // power table pw[(q * np1 + p) * K + k] = u_{k,q}^p (slots fastest) and
+    // f[k * kMaxR + mu] sit in the dm region.  One thread per monomial; the mu
+    // range is the largest of the monomial's wavefront, so the slot loop is a
+    // wave-uniform unrolled instance (lanes of lower degree carry unused mu).
     const int np1 = P.nu_max + 1;
     double* pw = dm;
     for (int t = tid; t < nv * 3; t += NT) {
         const int k = t / 3, q = t - 3 * k;
         const double inv = 1.0 / rr[k];
         const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
-        double* row_pw = pw + (size_t)t * np1;
+        double* col = pw + (size_t)q * np1 * K + k;
         double v = 1.0;
-        row_pw[0] = 1.0;
-        for (int p = 1; p < np1; ++p) { v *= u; row_pw[p] = v; }
+        col[0] = 1.0;
+        for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * K] = v; }
     }
     __syncthreads();
     for (int t = tid; t < P.n_mono; t += NT) {
         const I4 mo = P.mono[t];
-        const int nu = mo.w & 0xff, alpha = mo.w >> 8;
-        const int mumax = P.mu_max_of_nu[nu];
+        const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
+        const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
+        const double* px = pw + (size_t)mo.x * K;
+        const double* py = pw + (size_t)(np1 + mo.y) * K;
+        const double* pz = pw + (size_t)(2 * np1 + mo.z) * K;
         double acc[kMaxR];
 #pragma unroll
         for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
-        const double* px = pw + mo.x;
-        const double* py = pw + np1 + mo.y;
-        const double* pz = pw + 2 * np1 + mo.z;
-        for (int k = 0; k < nv; ++k) {
-            const int o = k * 3 * np1;
-            const double Pm = px[o] * py[o] * pz[o];
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) {  // uniform
-                    const double v = f[mu * K + k] * Pm;
-                    acc[mu] += mu <= mumax ? v : 0.0;  // per-thread mask as a select
-                }
+        switch (gmax) {
+            case 0: basic_moment_sum<1>(px, py, pz, f, nv, acc); break;
+            case 1: basic_moment_sum<2>(px, py, pz, f, nv, acc); break;
+            case 2: basic_moment_sum<3>(px, py, pz, f, nv, acc); break;
+            case 3: basic_moment_sum<4>(px, py, pz, f, nv, acc); break;
+            case 4: basic_moment_sum<5>(px, py, pz, f, nv, acc); break;
+            case 5: basic_moment_sum<6>(px, py, pz, f, nv, acc); break;
+            case 6: basic_moment_sum<7>(px, py, pz, f, nv, acc); break;
+            default: basic_moment_sum<8>(px, py, pz, f, nv, acc); break;
         }
+        const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
 #pragma unroll
         for (int mu = 0; mu < kMaxR; ++mu)
-            if (mu <= mumax) m[P.blk_off[mu][nu] + alpha] = acc[mu];
+            if (dst[mu] >= 0) m[dst[mu]] = acc[mu];
     }
     __syncthreads();
     // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
@@ -535,7 +580,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     __syncthreads();
 
     // ---- C: contraction layers (depth 1 .. n_depth)
-    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, m, dm, part, tid, P.xi_of);
+    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, lds, dm, part, tid, P.xi_of);
 
     // ---- D: site energy (and the basis values m[mapping] when asked for)
     const double* xrow = A.xi_atom ? A.xi_atom + (size_t)row * A.xi_stride : P.moment_coeffs;
@@ -573,7 +618,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         __syncthreads();
         xi = xl;
     }
-    for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, m, dm, part, tid, xi);
+    for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, lds, dm, part, tid, xi);
 
     // ---- F: per-slot gradient, thread = (slot, part).  m is dead: it holds
     // the per-part gradients and f, f' recomputed for the valid slots.
@@ -590,35 +635,13 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     __syncthreads();
     {
         const int kk = tid & (kWave - 1), prt = tid / kWave;
+        const int nu_max = P.nu_max;
         for (int k = kk; k < nv && prt < P.n_parts; k += kWave) {
-            const double ir = 1.0 / rr[k];
-            const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
-            double px[kMaxNu + 1], py[kMaxNu + 1], pz[kMaxNu + 1];
-            px[0] = py[0] = pz[0] = 1.0;
-#pragma unroll
-            for (int p = 1; p <= kMaxNu; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
-            double sdf = 0.0, sfn = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
-            const int b1 = P.part_beg[prt + 1];
-            for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
-                const I2 bl = P.part_blk[bi];
-                const int mu = bl.x, nu = bl.y;
-                const double* d = dm + P.blk_off[mu][nu];
-                const double fm = gf[mu * K + k], fpm = gfp[mu * K + k];
-                if (nu == 0) {
-                    sdf += fpm * d[0];
-                } else {
-                    double Hx = 0.0, Hy = 0.0, Hz = 0.0;
-                    grad_block_dispatch(nu, d, px, py, pz, Hx, Hy, Hz);
-                    const double uH = ux * Hx + uy * Hy + uz * Hz;
-                    sdf += fpm * uH * (1.0 / (double)nu);
-                    sfn += fm * uH;
-                    Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
-                }
-            }
-            double* gp = gpart + ((size_t)prt * K + k) * 3;
-            gp[0] = sdf * ux + (Gx - sfn * ux) * ir;
-            gp[1] = sdf * uy + (Gy - sfn * uy) * ir;
-            gp[2] = sdf * uz + (Gz - sfn * uz) * ir;
+            if (nu_max <= 4) slot_gradient<4>(P, prt, k, K, dm, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 6) slot_gradient<6>(P, prt, k, K, dm, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 8) slot_gradient<8>(P, prt, k, K, dm, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 10) slot_gradient<10>(P, prt, k, K, dm, gf, gfp, vx, vy, vz, rr, gpart);
+            else slot_gradient<kMaxNu>(P, prt, k, K, dm, gf, gfp, vx, vy, vz, rr, gpart);
         }
     }
     __syncthreads();
@@ -635,13 +658,17 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         const int c = cidx[k];
         g_row[t] = c >= 0 ? gk[3 * c + q] : 0.0;
     }
-    // ---- G: per-atom virial
-    if (tid < 9 && A.Wi) {
-        const int a = tid / 3, b = tid - 3 * (tid / 3);
-        const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
-        double s = 0.0;
-        for (int k = 0; k < nv; ++k) s += va[k] * gk[3 * k + b];
-        A.Wi[(size_t)row * 9 + tid] = s;
+    // ---- G: per-atom virial, one component per wavefront (lanes over slots)
+    if (A.Wi) {
+        const int lane = tid & (kWave - 1);
+        for (int c = tid / kWave; c < 9; c += NW) {
+            const int a = c / 3, b = c - 3 * (c / 3);
+            const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
+            double s = 0.0;
+            for (int k = lane; k < nv; k += kWave) s += va[k] * gk[3 * k + b];
+            s = wave_sum(s);
+            if (lane == 0) A.Wi[(size_t)row * 9 + c] = s;
+        }
     }
 }
 

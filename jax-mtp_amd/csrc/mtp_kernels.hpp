@@ -47,6 +47,44 @@ struct AtomLds2 {
     int m, dm, part, f, gf, gfp, vx, vy, vz, r, red, xl, ints, total_doubles;
 };
 
+// LDS of the balanced-item kernels (doubles).  Regions are reused across
+// phases:
+//   zero [kLdsM] lds[0] = 0.0: operand of the padding entries of the
+//               contraction streams and the sink of lanes without an item
+//   m   [mlen]  moments [0, n_live) (basic or read as operands); in phase F
+//               the per-part gradients and the recomputed f, f'
+//   dm  [dlen]  A/B: power table, then f; C/D: the values of the
+//               never-operand moments (m[x] for x >= n_live aliases
+//               dm[x - n_live], dead after D) and, at the end, the energy
+//               reduction; E/F: adjoints
+//   part, vx, vy, vz, r, xl (per-row coefficients), ints
+__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
+                                                    int nt, int max_partials, int xi_len)
+{
+    AtomLds2 L;
+    const int pw_len = K * 3 * (nu_max + 1);
+    int mlen = n_live;
+    if (mlen < n_parts * K * 3 + 2 * R * K) mlen = n_parts * K * 3 + 2 * R * K;
+    int dlen = amc;
+    if (dlen < pw_len + kMaxR * K) dlen = pw_len + kMaxR * K;
+    if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
+    L.m = kLdsM;
+    L.dm = L.m + mlen;
+    L.f = L.dm + pw_len;
+    L.gf = L.m + n_parts * K * 3;
+    L.gfp = L.gf + R * K;
+    L.red = L.dm + dlen - nt;
+    L.part = L.dm + dlen;
+    L.vx = L.part + (max_partials > 0 ? max_partials : 1);
+    L.vy = L.vx + K;
+    L.vz = L.vy + K;
+    L.r = L.vz + K;
+    L.xl = L.r + K;
+    L.ints = L.xl + xi_len;
+    L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
+    return L;
+}
+
 // Atom-kernel variants (chosen per potential by the launch autotuner):
 // v1 = 256 threads, one thread per target moment; v2 = balanced work items
 // with 256, 512 or 64 threads per atom (one wavefront: for low levels, where

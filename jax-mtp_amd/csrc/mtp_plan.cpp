@@ -109,6 +109,22 @@ bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
         for (int a = nu; a >= 0; --a)
             for (int bb = nu - a; bb >= 0; --bb, ++q) p.mono.push_back({a, bb, nu - a - bb, nu | (q << 8)});
     }
+    // top byte: the largest mu of the 64 monomials this one shares a
+    // wavefront with (monomial t runs on lane t % 64 of some wave for every
+    // workgroup size), so the kernels loop over a wave-uniform mu range
+    for (size_t g = 0; g < p.mono.size(); g += 64) {
+        int gmax = 0;
+        for (size_t t = g; t < std::min(p.mono.size(), g + 64); ++t)
+            gmax = std::max(gmax, p.mu_max_of_nu[p.mono[t].w & 0xff]);
+        for (size_t t = g; t < std::min(p.mono.size(), g + 64); ++t) p.mono[t].w |= gmax << 24;
+    }
+    static_assert(kMaxR == 8, "mono_dst holds eight mu per monomial");
+    p.mono_dst.assign(p.mono.size() * 2, I4{-1, -1, -1, -1});
+    for (size_t t = 0; t < p.mono.size(); ++t) {
+        const int nu = p.mono[t].w & 0xff, alpha = (p.mono[t].w >> 8) & 0xffff;
+        int* d = &p.mono_dst[2 * t].x;
+        for (int mu = 0; mu <= p.mu_max_of_nu[nu]; ++mu) d[mu] = p.blk_off[mu][nu] + alpha;
+    }
 
     // --- dependency depths ------------------------------------------------
     p.depth.assign(amc, 0);
@@ -184,138 +200,215 @@ bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
     p.part_beg.assign(9, 0);
     for (int k = 0; k < n_parts; ++k) {
         p.part_beg[k] = (int)p.part_blk.size();
-        for (auto& bl : parts[k]) p.part_blk.push_back(bl);
+        for (auto& bl : parts[k]) p.part_blk.push_back(I2{bl.x | (p.blk_off[bl.x][bl.y] << 8), bl.y});
     }
     for (int k = n_parts; k < 9; ++k) p.part_beg[k] = (int)p.part_blk.size();
-
-    // --- packed entries: (i1 | i2 << 16, mult as float bits) ---------------
-    p.fwd_pk.resize(p.fwd_pair.size());
-    for (size_t e = 0; e < p.fwd_pair.size(); ++e) {
-        float mf = (float)p.fwd_mult[e];
-        uint32_t bits;
-        std::memcpy(&bits, &mf, 4);
-        p.fwd_pk[e] = U2{p.fwd_pair[e], bits};
-    }
-    p.rev_pk.resize(p.rev_pair.size());
-    for (size_t e = 0; e < p.rev_pair.size(); ++e) {
-        float mf = (float)p.rev_mult[e];
-        uint32_t bits;
-        std::memcpy(&bits, &mf, 4);
-        p.rev_pk[e] = U2{p.rev_pair[e], bits};
-    }
 
     refresh_plan_params(p);
     return true;
 }
 
-// Split one layer (moment positions [l0, l1) with entry ranges beg[] into
-// the packed entry array pk, moment ids mom[]) into work items of about
-// `chunk` entries: runs of whole moments, or one chunk of a long moment that
-// writes a partial sum (partials are combined afterwards in a fixed order).
-// Moments without entries go to `base_only`; moments of run items that carry
-// a coefficient (`mapped`) go to `xi_add`.  The item tables are transposed,
-// entry j of item i at [j * n_items + i], so that the threads of a wave (one
-// item each) read consecutive addresses.  Entry word y holds the multiplier
-// in its low 16 bits; the last entry of a moment (or chunk) also holds the
-// destination moment + 1 (1 for a chunk) in its high 16 bits.
-static void split_layer(const std::vector<int>& beg, const std::vector<U2>& pk, const std::vector<int>& mom,
-                        const std::vector<char>& mapped, int l0, int l1, int chunk, LayerItems& L,
-                        std::vector<U2>& table, int& npart)
+// ---------------------------------------------------------------------------
+// Contraction streams.
+//
+// A layer's rows (a forward target with its entries, or a reverse moment
+// with the entries that read it) are cut into items: a row of at most C
+// entries is one item that stores its sum in the row's own moment; a longer
+// row becomes k = ceil(n / C) near-equal chunks that store partial sums,
+// added afterwards in chunk order.  Items are sorted by length (longest
+// first, ties in row order) and taken 64 at a time as groups: one item per
+// lane, every lane padded with zero entries to the group's longest item, so
+// a group ends at the same step on all lanes of a wavefront and the kernel's
+// loop has no per-lane control flow.  The group's last step carries each
+// lane's destination in place of a second entry.  Groups go to the least-loaded wave
+// (longest group first).  C is chosen per layer from a small set by the
+// modelled time of the slowest wave plus the partial-sum combination.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Item { int len, e0, row, dest; };
+
+constexpr double kFlushSteps = 1.5;    // modelled cost of a group end, in steps
+constexpr double kCombSteps = 0.5;     // per partial sum added in the combination
+
+struct LayerChoice {
+    std::vector<Item> items;           // sorted
+    std::vector<std::vector<int>> wave_groups;
+    std::vector<int> gsteps;
+    int npart = 0;
+    double cost = 0;
+};
+
+void choose_layer(const std::vector<int>& beg, int l0, int l1, int C, int nw, int nt, LayerChoice& lc,
+                  const std::vector<int>& dest_of_row, int part_off)
 {
-    npart = 0;
-    std::vector<std::vector<U2>> lists;
-    auto entry = [&](int e, int dest_plus1) {
-        U2 q = pk[e];
-        float mf;
-        std::memcpy(&mf, &q.y, 4);
-        q.y = (uint32_t)mf | ((uint32_t)dest_plus1 << 16);
-        return q;
-    };
-    int pos = l0;
-    while (pos < l1) {
+    lc = LayerChoice();
+    std::vector<int> comb_k;
+    for (int pos = l0; pos < l1; ++pos) {
         const int n = beg[pos + 1] - beg[pos];
-        if (n == 0) { L.base_only.push_back(pos); ++pos; continue; }
-        if (n > chunk) {
-            const int k = (n + chunk - 1) / chunk;
-            const int c = (n + k - 1) / k;
-            const int p0 = npart;
-            for (int q = 0; q < k; ++q) {
-                const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
-                std::vector<U2> lst;
-                for (int e = e0; e < e1; ++e) lst.push_back(entry(e, e == e1 - 1 ? 1 : 0));
-                L.items.push_back(I4{(int)lst.size(), pos, npart++, 0});
-                lists.push_back(std::move(lst));
-            }
-            L.comb.push_back(I4{pos, p0, npart, 0});
-            ++pos;
+        if (n == 0) continue;
+        if (n <= C) {
+            lc.items.push_back(Item{n, beg[pos], pos, dest_of_row[pos - l0]});
             continue;
         }
-        const int start = pos;
-        int cost = 0;
-        std::vector<U2> lst;
-        while (pos < l1) {
-            const int m = beg[pos + 1] - beg[pos];
-            if (m == 0 || m > chunk || (pos > start && cost + m > chunk)) break;
-            for (int e = beg[pos]; e < beg[pos + 1]; ++e)
-                lst.push_back(entry(e, e == beg[pos + 1] - 1 ? mom[pos] + 1 : 0));
-            if (mapped[mom[pos]]) L.xi_add.push_back(mom[pos]);
-            cost += m;
-            ++pos;
+        const int k = (n + C - 1) / C, c = (n + k - 1) / k;
+        for (int q = 0; q < k; ++q) {
+            const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
+            lc.items.push_back(Item{e1 - e0, e0, pos, (part_off + lc.npart) * 8});
+            ++lc.npart;
         }
-        L.items.push_back(I4{(int)lst.size(), start, -1, 0});
-        lists.push_back(std::move(lst));
+        comb_k.push_back(k);
     }
-    size_t maxlen = 0;
-    for (auto& l : lists) maxlen = std::max(maxlen, l.size());
-    L.table_base = (int)table.size();
-    const size_t ni = lists.size();
-    table.resize(table.size() + maxlen * ni, U2{0, 0});
-    for (size_t i = 0; i < ni; ++i)
-        for (size_t j = 0; j < lists[i].size(); ++j) table[L.table_base + j * ni + i] = lists[i][j];
+    std::stable_sort(lc.items.begin(), lc.items.end(), [](const Item& a, const Item& b) { return a.len > b.len; });
+    const int ng = ((int)lc.items.size() + 63) / 64;
+    lc.gsteps.resize(ng);
+    for (int g = 0; g < ng; ++g) lc.gsteps[g] = lc.items[(size_t)g * 64].len / 2 + 1;  // longest first, + dest
+    std::vector<int> order(ng);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lc.gsteps[a] > lc.gsteps[b]; });
+    std::vector<double> load(nw, 0.0);
+    lc.wave_groups.assign(nw, {});
+    for (int g : order) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += lc.gsteps[g] + kFlushSteps;
+        lc.wave_groups[w].push_back(g);
+    }
+    // combination: rows round-robin over the threads
+    std::sort(comb_k.begin(), comb_k.end(), std::greater<int>());
+    std::vector<int> tl(nt, 0);
+    for (size_t i = 0; i < comb_k.size(); ++i) tl[i % nt] += comb_k[i];
+    const int comb_max = comb_k.empty() ? 0 : *std::max_element(tl.begin(), tl.end());
+    lc.cost = *std::max_element(load.begin(), load.end()) + kCombSteps * comb_max + (comb_k.empty() ? 0.0 : 2.0);
 }
 
-// Smallest chunk (>= ceil(entries / nt)) whose split yields at most nt items,
-// so that every thread of the workgroup gets one item per layer.
-static void split_layer_fit(const std::vector<int>& beg, const std::vector<U2>& pk, const std::vector<int>& mom,
-                            const std::vector<char>& mapped, int l0, int l1, int nt, LayerItems& L,
-                            std::vector<U2>& table, int& npart)
+bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair, const std::vector<double>& mult,
+                  const std::vector<int>& mom, const std::vector<char>& mapped, int l0, int l1, bool rev, int nt,
+                  int dm_off, int part_off, int part_budget, StreamPlan& sp, int L, std::string& err)
 {
-    const int ent = beg[l1] - beg[l0];
-    int chunk = std::max(4, (ent + nt - 1) / nt);
-    const size_t base = table.size();
-    for (;;) {
-        L = LayerItems();
-        table.resize(base);
-        split_layer(beg, pk, mom, mapped, l0, l1, chunk, L, table, npart);
-        if ((int)L.items.size() <= nt || chunk >= ent) return;
-        chunk += std::max(1, chunk / 32);
+    const int nw = nt / 64;
+    // entry words (operand byte offsets, FP32 multiplier)
+    auto entry = [&](int e, uint32_t& x, uint32_t& y) -> bool {
+        const uint32_t i1 = pair[e] & 0xffffu, i2 = pair[e] >> 16;
+        const uint32_t a = (uint32_t)(kLdsM + i1) * 8u;
+        const uint32_t b = rev ? i2 * 8u : (uint32_t)(kLdsM + i2) * 8u;
+        if (a > 0xffffu || b > 0xffffu) return false;
+        const float mf = (float)mult[e];
+        if ((double)mf != mult[e]) return false;
+        x = a | (b << 16);
+        std::memcpy(&y, &mf, 4);
+        return true;
+    };
+    std::vector<int> dest_of_row(l1 - l0);
+    for (int pos = l0; pos < l1; ++pos)
+        dest_of_row[pos - l0] = (rev ? dm_off + mom[pos] : kLdsM + mom[pos]) * 8;
+
+    static const int kC[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 30};
+    LayerChoice best, lc;
+    bool have = false;
+    for (int C : kC) {
+        choose_layer(beg, l0, l1, C, nw, nt, lc, dest_of_row, part_off);
+        if (lc.npart > part_budget && C < (1 << 30)) continue;
+        if (!have || lc.cost < best.cost) { best = std::move(lc); have = true; }
     }
+    sp.max_partials = std::max(sp.max_partials, best.npart);
+    sp.cost += best.cost;
+
+    // comb / base-only / xi-add lists of this layer (row order)
+    sp.cb_beg[L] = (int)sp.comb.size();
+    sp.bo_beg[L] = (int)sp.bo.size();
+    sp.xa_beg[L] = (int)sp.xa.size();
+    // partial slots are numbered in row order (choose_layer); a row's
+    // range is read back from its items' destinations
+    {
+        std::vector<int> first(l1 - l0, -1), cnt(l1 - l0, 0);
+        for (const Item& it : best.items) {
+            if (it.dest < part_off * 8) continue;
+            const int pslot = it.dest / 8 - part_off;
+            int& f0 = first[it.row - l0];
+            f0 = f0 < 0 ? pslot : std::min(f0, pslot);
+            ++cnt[it.row - l0];
+        }
+        for (int pos = l0; pos < l1; ++pos) {
+            const int n = beg[pos + 1] - beg[pos];
+            if (n == 0) { sp.bo.push_back(mom[pos]); continue; }
+            if (cnt[pos - l0] > 0) {
+                sp.comb.push_back(I4{mom[pos], first[pos - l0], first[pos - l0] + cnt[pos - l0], 0});
+            } else if (rev && mapped[mom[pos]]) {
+                sp.xa.push_back(mom[pos]);
+            }
+        }
+    }
+
+    // streams
+    for (int w = 0; w < nw; ++w) {
+        const int first_step = (int)(sp.tbl.size() / 64);
+        int step = 0;
+        for (int g : best.wave_groups[w]) {
+            const int steps = best.gsteps[g];
+            const size_t t0 = sp.tbl.size();
+            sp.tbl.resize(t0 + (size_t)steps * 64, U4{0u, 0u, 0u, 0u});
+            for (int l = 0; l < 64; ++l) {
+                const size_t ii = (size_t)g * 64 + l;
+                U4& last = sp.tbl[t0 + (size_t)(steps - 1) * 64 + l];
+                last.z = 0u;  // lane without an item: zero entries, sum to lds[0]
+                last.w = kStreamFlush;
+                if (ii >= best.items.size()) continue;
+                const Item& it = best.items[ii];
+                last.z = (uint32_t)it.dest;
+                for (int j = 0; j < it.len; ++j) {
+                    uint32_t x, y;
+                    if (!entry(it.e0 + j, x, y)) {
+                        err = "an operand offset or multiplier does not fit the packed contraction entries";
+                        return false;
+                    }
+                    U4& q = sp.tbl[t0 + (size_t)(j / 2) * 64 + l];
+                    if (j % 2 == 0) { q.x = x; q.y = y; } else { q.z = x; q.w = y; }
+                }
+            }
+            step += steps;
+        }
+        const int padded = (step + kStreamUnroll - 1) / kStreamUnroll * kStreamUnroll;
+        sp.tbl.resize(sp.tbl.size() + (size_t)(padded - step) * 64, U4{0u, 0u, 0u, 0u});
+        sp.wave.push_back(I4{first_step, padded, 0, 0});
+    }
+    return true;
 }
 
-void build_items(const HostPlan& p, int nt, ItemPlan& ip)
+}  // namespace
+
+bool build_items(const HostPlan& p, int nt, int dm_off, int part_off, int part_budget, ItemPlan& ip,
+                 std::string& err)
 {
     ip = ItemPlan();
     ip.nt = nt;
-    ip.max_partials = 0;
     const int amc = p.file.alpha_moments_count;
     std::vector<char> mapped(amc, 0), no_map(amc, 0);
     for (int x : p.file.alpha_moment_mapping) mapped[x] = 1;
-    for (int d = 1; d <= p.n_depth; ++d) {
-        LayerItems L;
-        int np = 0;
-        split_layer_fit(p.fwd_beg, p.fwd_pk, p.fwd_tgt, no_map, p.fwd_layer[d], p.fwd_layer[d + 1], nt, L,
-                        ip.fwd_table, np);
-        ip.max_partials = std::max(ip.max_partials, np);
-        ip.fwd.push_back(std::move(L));
+    for (int L = 0; L < kMaxDepth + 2; ++L) {
+        ip.fwd.cb_beg[L] = ip.fwd.bo_beg[L] = ip.fwd.xa_beg[L] = 0;
+        ip.rev.cb_beg[L] = ip.rev.bo_beg[L] = ip.rev.xa_beg[L] = 0;
     }
-    for (int li = 0; li <= p.n_depth; ++li) {
-        LayerItems L;
-        int np = 0;
-        split_layer_fit(p.rev_beg, p.rev_pk, p.rev_x, mapped, p.rev_layer[li], p.rev_layer[li + 1], nt, L,
-                        ip.rev_table, np);
-        ip.max_partials = std::max(ip.max_partials, np);
-        ip.rev.push_back(std::move(L));
+    for (int d = 1; d <= p.n_depth; ++d)
+        if (!stream_layer(p.fwd_beg, p.fwd_pair, p.fwd_mult, p.fwd_tgt, no_map, p.fwd_layer[d], p.fwd_layer[d + 1],
+                          false, nt, dm_off, part_off, part_budget, ip.fwd, d - 1, err))
+            return false;
+    for (int li = 0; li <= p.n_depth; ++li)
+        if (!stream_layer(p.rev_beg, p.rev_pair, p.rev_mult, p.rev_x, mapped, p.rev_layer[li], p.rev_layer[li + 1],
+                          true, nt, dm_off, part_off, part_budget, ip.rev, li, err))
+            return false;
+    // end markers of the per-layer list ranges
+    for (int L = p.n_depth; L < kMaxDepth + 2; ++L) {
+        ip.fwd.cb_beg[L] = (int)ip.fwd.comb.size();
+        ip.fwd.bo_beg[L] = (int)ip.fwd.bo.size();
+        ip.fwd.xa_beg[L] = (int)ip.fwd.xa.size();
     }
+    for (int L = p.n_depth + 1; L < kMaxDepth + 2; ++L) {
+        ip.rev.cb_beg[L] = (int)ip.rev.comb.size();
+        ip.rev.bo_beg[L] = (int)ip.rev.bo.size();
+        ip.rev.xa_beg[L] = (int)ip.rev.xa.size();
+    }
+    ip.max_partials = std::max(ip.fwd.max_partials, ip.rev.max_partials);
+    return true;
 }
 
 }  // namespace mtp

@@ -29,11 +29,16 @@ namespace mtp {
 struct I4 { int x, y, z, w; };
 struct I2 { int x, y; };
 struct U2 { uint32_t x, y; };
+struct U4 { uint32_t x, y, z, w; };
 
+constexpr int kWave = 64;     // wavefront width of gfx950
 constexpr int kMaxR = 8;      // radial functions (mu) supported
 constexpr int kMaxNu = 12;    // tensor order supported
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
+constexpr int kStreamUnroll = 4;  // steps per prefetch batch of a contraction stream
+constexpr uint32_t kStreamFlush = 0xffffffffu;  // 4th word of a group's last step (never an FP32 multiplier)
+constexpr int kLdsM = 2;      // m[] starts at this double of the atom kernels' LDS (lds[0] is a zero)
 
 // Device-side view of the plan (all pointers are device pointers).
 struct DevPlan {
@@ -49,7 +54,8 @@ struct DevPlan {
     int blk_off[kMaxR][kMaxNu + 1];  // offset of block (mu, nu) in m[], -1 if absent
     int mu_max_of_nu[kMaxNu + 1];    // largest mu with a (mu, nu) block, -1 if none
     int n_mono;                      // monomials of degree 0..nu_max
-    const I4* mono;                // [n_mono] (a, b, c, nu | alpha_in_block << 8)
+    const I4* mono;                // [n_mono] (a, b, c, nu | alpha_in_block << 8 | wave's max mu << 24)
+    const I4* mono_dst;            // [n_mono][2]: basic moment index of (mu, monomial), mu = 0..7, or -1
     // forward layers
     int n_depth;
     int fwd_layer[kMaxDepth + 2];    // target index ranges per depth 1..n_depth (prefix, into fwd_tgt)
@@ -66,22 +72,30 @@ struct DevPlan {
     // per-neighbor gradient: (mu, nu) blocks split into parts
     int n_parts;
     int part_beg[9];                 // prefix into part_blk
-    const I2* part_blk;            // (mu, nu)
-    // balanced work items for the contraction layers (see build_items).
-    // Layer L (forward: depth L+1; reverse: rev_layer index L) owns items
-    // [it_beg[L], it_beg[L+1]), combines [cb_beg[L], cb_beg[L+1]), base-only
-    // moments [bo_beg[L], bo_beg[L+1]) and a transposed table at tbl_base[L].
-    int fwd_it_beg[kMaxDepth + 2], fwd_cb_beg[kMaxDepth + 2], fwd_tbl_base[kMaxDepth + 1];
-    int rev_it_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2], rev_tbl_base[kMaxDepth + 1];
+    const I2* part_blk;            // (mu | offset of the (mu, nu) block << 8, nu)
+    // contraction layers as per-wavefront entry streams (see build_items).
+    // Layer L (forward: depth L+1; reverse: rev_layer index L) gives wave w
+    // of the workgroup the steps [wave[L * NW + w].x, + .y) of tbl (.y a
+    // multiple of kStreamUnroll).  Step s holds two entries per lane:
+    // tbl[s * 64 + lane] = (a | b << 16, mult, a | b << 16, mult), with a, b
+    // LDS byte offsets of the operands (a from the workgroup's LDS base; b
+    // from it in the forward pass, from the adjoint region in the reverse
+    // pass) and mult an FP32 value.  The last step of a group of 64 items
+    // holds one entry and (dest, kStreamFlush): each lane stores its sum at
+    // the LDS byte offset dest and restarts from zero.  Rows split into
+    // chunks leave partial sums, combined per layer by comb[cb_beg[L] ..
+    // cb_beg[L+1]] = (moment, first partial, end partial, -) in a fixed order.
+    int fwd_cb_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2];
     int rev_bo_beg[kMaxDepth + 2], rev_xa_beg[kMaxDepth + 2];
-    const I4* fwd_items;             // (n entries, first moment position, partial slot or -1, -)
-    const I4* fwd_comb;              // (moment position, p0, p1, -)
-    const U2* fwd_tbl;               // (i1 | i2 << 16, mult | (dest + 1) << 16 on the last entry of a moment)
-    const I4* rev_items;
+    const I4* fwd_wave;
+    const U4* fwd_tbl;
+    const I4* fwd_comb;
+    const I4* rev_wave;
+    const U4* rev_tbl;
     const I4* rev_comb;
-    const U2* rev_tbl;               // (other | target << 16, mult | (dest + 1) << 16)
-    const int* rev_bo;               // moment positions without entries (dm = xi)
-    const int* rev_xa;               // moments of run items whose dm gets + xi after the sums
+    const int* rev_bo;               // moments without entries (dm = xi)
+    const int* rev_xa;               // moments of unsplit rows whose dm gets + xi after the sums
+    int k_layout;                    // slots the LDS layout is sized for (>= the slots of any call)
     int max_partials;
     // inverse of alpha_moment_mapping: distinct mapped moments (increasing)
     // and, per moment, the scalar indices s mapped to it (increasing)
@@ -91,19 +105,21 @@ struct DevPlan {
     const int* map_s;                // [S]
 };
 
-// Work items of one contraction layer for a workgroup of nt threads (see
-// split_layer in mtp_plan.cpp).
-struct LayerItems {
-    std::vector<I4> items, comb;
-    std::vector<int> base_only;   // positions
-    std::vector<int> xi_add;      // moment ids
-    int table_base = 0;
+// Contraction streams of one direction (all layers) for a workgroup of nt
+// threads, in the DevPlan layout above.
+struct StreamPlan {
+    std::vector<I4> wave;          // [n_layers * nt / 64]
+    std::vector<U4> tbl;           // [steps * 64]
+    std::vector<I4> comb;
+    std::vector<int> bo, xa;
+    int cb_beg[kMaxDepth + 2], bo_beg[kMaxDepth + 2], xa_beg[kMaxDepth + 2];
+    int max_partials = 0;
+    double cost = 0;               // modelled steps of the slowest wave, summed over layers
 };
 
 struct ItemPlan {
     int nt = 0, max_partials = 0;
-    std::vector<LayerItems> fwd, rev;   // fwd[d-1] = depth d; rev[li] = rev_layer li
-    std::vector<U2> fwd_table, rev_table;
+    StreamPlan fwd, rev;
 };
 
 struct HostPlan {
@@ -113,6 +129,7 @@ struct HostPlan {
     int nu_max = 0, n_depth = 0, n_parts = 4;
     std::vector<int> depth;  // [amc]
     std::vector<I4> mono;
+    std::vector<I4> mono_dst;   // [n_mono * 2]
     std::vector<int> fwd_tgt, fwd_beg;
     std::vector<uint32_t> fwd_pair;
     std::vector<double> fwd_mult;
@@ -124,13 +141,17 @@ struct HostPlan {
     std::vector<double> xi_of;
     std::vector<I2> part_blk;
     std::vector<int> part_beg;
-    std::vector<U2> fwd_pk, rev_pk;
     std::vector<int> map_x, map_beg, map_s;
     int blk_off[kMaxR][kMaxNu + 1];
     int mu_max_of_nu[kMaxNu + 1];
 };
 
-void build_items(const HostPlan& p, int nt, ItemPlan& ip);
+// Streams for a workgroup of nt threads whose LDS holds the adjoints from
+// double dm_off and the partial sums from double part_off (m at kLdsM).
+// part_budget caps the partial sums of a layer when the chunking allows.
+// Returns false when an operand offset does not fit the 16-bit fields.
+bool build_items(const HostPlan& p, int nt, int dm_off, int part_off, int part_budget, ItemPlan& ip,
+                 std::string& err);
 
 // Build the plan from a validated file.  n_parts splits the per-neighbor
 // gradient work.  Returns false with a message for unsupported layouts.

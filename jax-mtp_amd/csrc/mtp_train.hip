@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A
     __syncthreads();
     for (int t = tid; t < P.n_mono; t += kTB) {
         const I4 mo = P.mono[t];
-        const int nu = mo.w & 0xff, alpha_i = mo.w >> 8;
+        const int nu = mo.w & 0xff, alpha_i = (mo.w >> 8) & 0xffff;
         const int mumax = P.mu_max_of_nu[nu];
         double acc[kMaxR], acct[kMaxR];
 #pragma unroll
@@ -261,9 +261,9 @@ __global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A
             const int b1 = P.part_beg[part + 1];
             for (int bi = P.part_beg[part]; bi < b1; ++bi) {
                 const I2 bl = P.part_blk[bi];
-                const int mu = bl.x, nu = bl.y;
-                const double* d = dm + P.blk_off[mu][nu];
-                const double* dt = dmt + P.blk_off[mu][nu];
+                const int mu = bl.x & 0xff, nu = bl.y;
+                const double* d = dm + (bl.x >> 8);
+                const double* dt = dmt + (bl.x >> 8);
                 double a0, a1, q;
                 if (nu == 0) {
                     a0 = d[0]; a1 = dt[0]; q = 0.0;

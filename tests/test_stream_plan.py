@@ -1,0 +1,39 @@
+"""The contraction streams of the atom kernel, checked on the host.
+
+``stream_plan_check.cpp`` builds the plan (jax-mtp_amd/csrc/mtp_plan.cpp)
+for a workgroup size and executes every layer's entry streams on the CPU
+the way the kernel walks them: lane accumulators, group ends read from
+lane 0 (and required to agree on all lanes), sums stored at the encoded LDS
+offsets, partial sums combined in their fixed order.  The moments and
+their adjoints must equal a direct evaluation of alpha_index_times (the
+reference's conversion.py:95-102 recurrence and its reverse) to 1e-12.
+"""
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+CSRC = ROOT / "jax-mtp_amd" / "csrc"
+POT = ROOT / "jax-mtp_amd" / "potentials"
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ is not available")
+    exe = tmp_path_factory.mktemp("spc") / "stream_plan_check"
+    subprocess.run([gxx, "-O2", "-std=c++17", f"-I{CSRC}", str(ROOT / "tests" / "stream_plan_check.cpp"),
+                    str(CSRC / "mtp_plan.cpp"), str(CSRC / "mtp_format.cpp"), "-o", str(exe)], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("level", [2, 6, 10, 12, 16, 20, 24])
+@pytest.mark.parametrize("threads", [64, 256, 512])
+def test_streams_reproduce_the_contraction(checker, level, threads):
+    for max_nbr in (32, 80):
+        r = subprocess.run([str(checker), str(POT / f"untrained_{level:02d}.mtp"), str(threads), str(max_nbr),
+                            str(level)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout + r.stderr

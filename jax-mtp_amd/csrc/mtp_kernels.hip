@@ -412,7 +412,7 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
 }
 
 // sum_k f_mu(k) * u_x^a u_y^b u_z^c over the valid slots for mu < NMU
-// (powers px[k], py[k], pz[k]; f[k * kMaxR + mu]), four slots per step.
+// (powers px[k], py[k], pz[k]; f[k * kFStride + mu]), four slots per step.
 template <int NMU>
 __device__ __forceinline__ void basic_moment_sum(const double* px, const double* py, const double* pz,
                                                  const double* f, int nv, double* acc)
@@ -422,14 +422,14 @@ __device__ __forceinline__ void basic_moment_sum(const double* px, const double*
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const double Pm = px[k + u] * py[k + u] * pz[k + u];
-            const double* fr = f + (k + u) * kMaxR;
+            const double* fr = f + (k + u) * kFStride;
 #pragma unroll
             for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
         }
     }
     for (; k < nv; ++k) {
         const double Pm = px[k] * py[k] * pz[k];
-        const double* fr = f + k * kMaxR;
+        const double* fr = f + k * kFStride;
 #pragma unroll
         for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
     }
@@ -488,7 +488,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
                 radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
                 for (int mu = 0; mu < kMaxR; ++mu)
-                    if (mu < P.R) f[pos * kMaxR + mu] = fv[mu];
+                    if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
                 vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
                 slot_of[pos] = k;
             }
@@ -518,7 +518,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
 #pragma unroll
             for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) f[pos * kMaxR + mu] = fv[mu];
+                if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             slot_of[pos] = k;
         }
@@ -532,29 +532,31 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     }
 
     // ---- B: basic moments m[mu, alpha] = sum_k f_mu(k) u_k^alpha.  The
-    // power table pw[(q * np1 + p) * K + k] = u_{k,q}^p (slots fastest) and
-    // f[k * kMaxR + mu] sit in the dm region.  One thread per monomial; the mu
-    // range is the largest of the monomial's wavefront, so the slot loop is a
-    // wave-uniform unrolled instance (lanes of lower degree carry unused mu).
+    // power table pw[(q * np1 + p) * Kp + k] = u_{k,q}^p (slots fastest, odd
+    // row stride Kp) and f[k * kFStride + mu] sit in the dm region.  One
+    // thread per monomial; the mu range is the largest of the monomial's
+    // wavefront, so the slot loop is a wave-uniform unrolled instance (lanes
+    // of lower degree carry unused mu).
     const int np1 = P.nu_max + 1;
+    const int Kp = P.k_layout | 1;
     double* pw = dm;
     for (int t = tid; t < nv * 3; t += NT) {
         const int k = t / 3, q = t - 3 * k;
         const double inv = 1.0 / rr[k];
         const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
-        double* col = pw + (size_t)q * np1 * K + k;
+        double* col = pw + (size_t)q * np1 * Kp + k;
         double v = 1.0;
         col[0] = 1.0;
-        for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * K] = v; }
+        for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
     }
     __syncthreads();
     for (int t = tid; t < P.n_mono; t += NT) {
         const I4 mo = P.mono[t];
         const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
         const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
-        const double* px = pw + (size_t)mo.x * K;
-        const double* py = pw + (size_t)(np1 + mo.y) * K;
-        const double* pz = pw + (size_t)(2 * np1 + mo.z) * K;
+        const double* px = pw + (size_t)mo.x * Kp;
+        const double* py = pw + (size_t)(np1 + mo.y) * Kp;
+        const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp;
         double acc[kMaxR];
 #pragma unroll
         for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;

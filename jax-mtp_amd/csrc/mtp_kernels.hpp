@@ -53,20 +53,24 @@ struct AtomLds2 {
 //               contraction streams and the sink of lanes without an item
 //   m   [mlen]  moments [0, n_live) (basic or read as operands); in phase F
 //               the per-part gradients and the recomputed f, f'
-//   dm  [dlen]  A/B: power table, then f; C/D: the values of the
+//   dm  [dlen]  A/B: power table (row stride K | 1), then f (row stride
+//               kFStride; odd strides spread a wave's accesses over the
+//               LDS banks); C/D: the values of the
 //               never-operand moments (m[x] for x >= n_live aliases
 //               dm[x - n_live], dead after D) and, at the end, the energy
 //               reduction; E/F: adjoints
 //   part, vx, vy, vz, r, xl (per-row coefficients), ints
+constexpr int kFStride = kMaxR + 1;  // doubles per slot of the radial values in LDS
+
 __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
                                                     int nt, int max_partials, int xi_len)
 {
     AtomLds2 L;
-    const int pw_len = K * 3 * (nu_max + 1);
+    const int pw_len = (K | 1) * 3 * (nu_max + 1);
     int mlen = n_live;
     if (mlen < n_parts * K * 3 + 2 * R * K) mlen = n_parts * K * 3 + 2 * R * K;
     int dlen = amc;
-    if (dlen < pw_len + kMaxR * K) dlen = pw_len + kMaxR * K;
+    if (dlen < pw_len + kFStride * K) dlen = pw_len + kFStride * K;
     if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
     L.m = kLdsM;
     L.dm = L.m + mlen;

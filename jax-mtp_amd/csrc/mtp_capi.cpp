@@ -266,8 +266,9 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->vdev[kVariantV1] = d;
     c->vlds[kVariantV1] = c->lds_bytes;
     {
-        for (int v : {kVariantV2x256, kVariantV2x512, kVariantV2x64}) {
+        for (int v : {kVariantV2x256, kVariantV2x512, kVariantV2x64, kVariantPair1024, kVariantPair512}) {
             const int nt = variant_threads(v);
+            const bool paired = variant_paired(v);
             HostPlan pp;
             const HostPlan* pv = &p;
             DevPlan dv = d;
@@ -285,13 +286,30 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             // LDS offsets do not depend on the partial-sum count; keep the
             // partials of a layer within what leaves room for two workgroups
             // per CU when the rest of the layout allows it
-            const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
-                                                 p.nu_max, dv.n_parts, nt, 0, 0);
-            const int half_cu = (int)(lds_max / 2 / sizeof(double));
-            const int part_budget = std::max(64, half_cu - L0.total_doubles);
+            // (one workgroup per CU for the paired rows: the partial sums
+            // take what the CU's LDS leaves)
+            StreamLayout lay;
+            int part_budget;
+            if (paired) {
+                const AtomLdsPair L0 = atom_lds_pair_layout(f.alpha_moments_count, p.n_live, max_nbr,
+                                                            f.radial_funcs_count, p.nu_max, dv.n_parts, nt, 0);
+                lay.unit = 16;
+                lay.m_off = kPairM;
+                lay.dm_off = L0.dm;
+                lay.part_off = L0.part;
+                lay.absolute = true;
+                part_budget = std::max(64, ((int)(lds_max / sizeof(double)) - L0.total_doubles) / 2);
+            } else {
+                const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                                                     p.nu_max, dv.n_parts, nt, 0, 0);
+                const int half_cu = (int)(lds_max / 2 / sizeof(double));
+                part_budget = std::max(64, half_cu - L0.total_doubles);
+                lay.dm_off = L0.dm;
+                lay.part_off = L0.part;
+            }
             ItemPlan ip;
             std::string ierr;
-            if (!build_items(*pv, nt, L0.dm, L0.part, part_budget, ip, ierr)) continue;  // variant unavailable
+            if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) continue;  // variant unavailable
             auto pack = [&](const StreamPlan& sp, int* cb_beg, int* bo_beg, int* xa_beg, const I4** d_wave,
                             const U4** d_tbl, const I4** d_comb, const int** d_bo, const int** d_xa) {
                 for (int q = 0; q < kMaxDepth + 2; ++q) {
@@ -319,11 +337,16 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host tables die with this iteration
             dv.max_partials = ip.max_partials;
             dv.k_layout = max_nbr;
-            const size_t lds = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
-                                               p.nu_max, dv.n_parts, nt, ip.max_partials);
+            const size_t lds = paired ? sizeof(double) * (size_t)atom_lds_pair_layout(
+                                                f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                                                p.nu_max, dv.n_parts, nt, ip.max_partials).total_doubles
+                                      : atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr,
+                                                        f.radial_funcs_count, p.nu_max, dv.n_parts, nt,
+                                                        ip.max_partials);
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
                 c->vdev[v] = dv;
                 c->vlds[v] = lds;
+                if (paired) continue;  // plain force evaluation only
                 const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
                                                    p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
                 if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
@@ -776,7 +799,7 @@ namespace {
 int mode_variant(const mtp_ctx* c, bool with_xi)
 {
     const size_t* l = with_xi ? c->vlds_xi : c->vlds;
-    if (c->variant != kVariantV1 && l[c->variant]) return c->variant;
+    if (c->variant != kVariantV1 && !variant_paired(c->variant) && l[c->variant]) return c->variant;
     if (l[kVariantV2x256]) return kVariantV2x256;
     if (l[kVariantV2x512]) return kVariantV2x512;
     return -1;

@@ -106,7 +106,7 @@ __host__ __device__ constexpr int mono_idx(int NU, int a, int b) { return (NU - 
 
 // H = sum_alpha d[alpha] grad_u u^alpha over one (mu, NU) block, written as a
 // sum over the degree NU-1 monomials (Euler: u . H = NU * sum d P).
-template <int NU>
+template <int NU, int DS = 1>
 __device__ __forceinline__ void grad_block(const double* __restrict__ d, const double* px, const double* py,
                                            const double* pz, double& Hx, double& Hy, double& Hz)
 {
@@ -116,9 +116,9 @@ __device__ __forceinline__ void grad_block(const double* __restrict__ d, const d
         for (int b = NU - 1 - a; b >= 0; --b) {
             const int c = NU - 1 - a - b;
             const double P = px[a] * py[b] * pz[c];
-            Hx += (double)(a + 1) * d[mono_idx(NU, a + 1, b)] * P;
-            Hy += (double)(b + 1) * d[mono_idx(NU, a, b + 1)] * P;
-            Hz += (double)(c + 1) * d[mono_idx(NU, a, b)] * P;
+            Hx += (double)(a + 1) * d[DS * mono_idx(NU, a + 1, b)] * P;
+            Hy += (double)(b + 1) * d[DS * mono_idx(NU, a, b + 1)] * P;
+            Hz += (double)(c + 1) * d[DS * mono_idx(NU, a, b)] * P;
         }
         // one row of monomials at a time: keeps the LDS loads of the fully
         // unrolled block from being hoisted together (register pressure)
@@ -138,24 +138,25 @@ __device__ __forceinline__ void opaque_powers(double* px, double* py, double* pz
     for (int p = 0; p < PN; ++p) asm volatile("" : "+v"(px[p]), "+v"(py[p]), "+v"(pz[p]));
 }
 
-template <int PN = kMaxNu + 1>
+// DS is the stride of the adjoints in d (2 when two rows are interleaved).
+template <int PN = kMaxNu + 1, int DS = 1>
 __device__ __forceinline__ void grad_block_dispatch(int nu, const double* d, double* px, double* py, double* pz,
                                                     double& Hx, double& Hy, double& Hz)
 {
     opaque_powers<PN>(px, py, pz);
     switch (nu) {
-        case 1: grad_block<1>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 2: if constexpr (PN >= 2) grad_block<2>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 3: if constexpr (PN >= 3) grad_block<3>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 4: if constexpr (PN >= 4) grad_block<4>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 5: if constexpr (PN >= 5) grad_block<5>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 6: if constexpr (PN >= 6) grad_block<6>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 7: if constexpr (PN >= 7) grad_block<7>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 8: if constexpr (PN >= 8) grad_block<8>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 9: if constexpr (PN >= 9) grad_block<9>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 10: if constexpr (PN >= 10) grad_block<10>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 11: if constexpr (PN >= 11) grad_block<11>(d, px, py, pz, Hx, Hy, Hz); break;
-        case 12: if constexpr (PN >= 12) grad_block<12>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 1: grad_block<1, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 2: if constexpr (PN >= 2) grad_block<2, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 3: if constexpr (PN >= 3) grad_block<3, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 4: if constexpr (PN >= 4) grad_block<4, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 5: if constexpr (PN >= 5) grad_block<5, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 6: if constexpr (PN >= 6) grad_block<6, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 7: if constexpr (PN >= 7) grad_block<7, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 8: if constexpr (PN >= 8) grad_block<8, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 9: if constexpr (PN >= 9) grad_block<9, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 10: if constexpr (PN >= 10) grad_block<10, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 11: if constexpr (PN >= 11) grad_block<11, DS>(d, px, py, pz, Hx, Hy, Hz); break;
+        case 12: if constexpr (PN >= 12) grad_block<12, DS>(d, px, py, pz, Hx, Hy, Hz); break;
         default: break;
     }
 }

@@ -375,8 +375,9 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
 // Gradient of one slot k over the (mu, nu) blocks of part prt: g_k from the
 // adjoints dm of the basic moments.  H = sum_alpha dm_alpha grad_u u^alpha
 // over the monomials of degree nu - 1 (power tables of PN orders, enough
-// for nu <= PN), u . H = nu sum_alpha dm_alpha u^alpha (Euler).
-template <int PN>
+// for nu <= PN), u . H = nu sum_alpha dm_alpha u^alpha (Euler).  The
+// adjoints of this row are dm[DS * x].
+template <int PN, int DS = 1>
 __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, int K, const double* dm,
                                               const double* gf, const double* gfp, const double* vx,
                                               const double* vy, const double* vz, const double* rr, double* gpart)
@@ -392,13 +393,13 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
     for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
         const I2 bl = P.part_blk[bi];
         const int mu = bl.x & 0xff, nu = bl.y;
-        const double* d = dm + (bl.x >> 8);
+        const double* d = dm + DS * (bl.x >> 8);
         const double fm = gf[mu * K + k], fpm = gfp[mu * K + k];
         if (nu == 0) {
             sdf += fpm * d[0];
         } else {
             double Hx = 0.0, Hy = 0.0, Hz = 0.0;
-            grad_block_dispatch<PN>(nu, d, px, py, pz, Hx, Hy, Hz);
+            grad_block_dispatch<PN, DS>(nu, d, px, py, pz, Hx, Hy, Hz);
             const double uH = ux * Hx + uy * Hy + uz * Hz;
             sdf += fpm * uH * (1.0 / (double)nu);
             sfn += fm * uH;
@@ -700,6 +701,301 @@ __global__ void __launch_bounds__(NT) atom_kernel_v2_list(const DevPlan P, AtomA
 }
 
 // ---------------------------------------------------------------------------
+// Paired rows: one workgroup evaluates two atoms.  Their moments, adjoints
+// and partial sums are interleaved in 16-byte LDS slots, so each entry of a
+// contraction stream (its table load, operand offsets and FP32 multiplier)
+// serves both rows with one 128-bit LDS read per operand.  The per-atom
+// phases (radial basis, basic moments, energy, slot gradients, virial) run
+// on one half of the workgroup per row, exactly as the single-row kernel.
+// Operand fields are slot indices from the LDS base (StreamLayout absolute).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stream_steps_pair(const uint4* q, char* lb, double2& acc)
+{
+    constexpr int U = kStreamUnroll;
+    double2 a0[U], b0[U], a1[U], b1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        a0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x & 0xffffu) << 4));
+        b0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x >> 16) << 4));
+        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
+        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const double w0 = (double)__uint_as_float(q[u].y);
+        acc.x = fma(w0 * a0[u].x, b0[u].x, acc.x);
+        acc.y = fma(w0 * a0[u].y, b0[u].y, acc.y);
+        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
+            const double w1 = (double)__uint_as_float(q[u].w);
+            acc.x = fma(w1 * a1[u].x, b1[u].x, acc.x);
+            acc.y = fma(w1 * a1[u].y, b1[u].y, acc.y);
+        } else {  // group end (wave-uniform): q.z is this lane's destination
+            *reinterpret_cast<double2*>(lb + q[u].z) = acc;
+            acc = make_double2(0.0, 0.0);
+        }
+    }
+}
+
+template <int NT, bool REV>
+__device__ __forceinline__ void contraction_layer_pair(const DevPlan& P, int L, double* lds, double* m2, double* dm2,
+                                                       double* part2, int tid)
+{
+    constexpr int NW = NT / kWave;
+    constexpr int U = kStreamUnroll;
+    const int lane = tid & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
+    const int nsteps = wv.y;
+    if (nsteps > 0) {
+        const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
+        char* lb = reinterpret_cast<char*>(lds);
+        double2 acc = make_double2(0.0, 0.0);
+        uint4 qa[U], qb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)u * kWave];
+        for (int s0 = 0;;) {
+            if (s0 + U < nsteps) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) qb[u] = tb[(size_t)(s0 + U + u) * kWave];
+            }
+            stream_steps_pair(qa, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
+            if (s0 + U < nsteps) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)(s0 + U + u) * kWave];
+            }
+            stream_steps_pair(qb, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
+        }
+    }
+    __syncthreads();
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    double2* dst = reinterpret_cast<double2*>(REV ? dm2 : m2);
+    const double2* part = reinterpret_cast<const double2*>(part2);
+    for (int c = c0 + tid; c < c1; c += NT) {
+        const I4 cb = comb[c];
+        const double x0 = REV ? P.xi_of[cb.x] : 0.0;
+        double2 acc = make_double2(x0, x0);
+        for (int p = cb.y; p < cb.z; ++p) {
+            const double2 v = part[p];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        dst[cb.x] = acc;
+    }
+    if (REV) {
+        for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
+            const int x = P.rev_bo[t];
+            dst[x] = make_double2(P.xi_of[x], P.xi_of[x]);
+        }
+        for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
+            const int x = P.rev_xa[t];
+            const double v = P.xi_of[x];
+            dst[x].x += v;
+            dst[x].y += v;
+        }
+    }
+    if (c1 > c0 || REV) __syncthreads();
+}
+
+template <typename RadialReal, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 1024 ? 4 : 2)))
+atom_kernel_pair(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    constexpr int NH = NT / 2, NWH = NH / kWave;
+    const int tid = threadIdx.x;
+    const int h = tid / NH, ht = tid - h * NH;  // row of the pair, thread within the row
+    const int lane = tid & (kWave - 1), wr = ht / kWave;
+    const int row = 2 * blockIdx.x + h;
+    const bool active = row < A.nrows;
+    const int K = A.nslots, Kl = P.k_layout;
+    const AtomLdsPair L = atom_lds_pair_layout(P.amc, P.n_live, Kl, P.R, P.nu_max, P.n_parts, NT, P.max_partials);
+    double* m2 = lds + 2 * L.m;          // m of row r of moment x: m2[2 x + r]
+    double* dm2 = lds + 2 * L.dm;
+    double* part2 = lds + 2 * L.part;
+    double* vx = lds + L.vec + h * 4 * Kl;
+    double* vy = vx + Kl;
+    double* vz = vy + Kl;
+    double* rr = vz + Kl;
+    int* ints = reinterpret_cast<int*>(lds + L.ints) + h * L.ints_per_row;
+    int* slot_of = ints;
+    int* cidx = ints + Kl;
+    int* wave_cnt = ints + 2 * Kl;
+    int* nvp = wave_cnt + NWH;
+    double* scr_dm = dm2 + h * L.dlen;   // this row's scratch in the adjoint region (A/B, gk)
+    const int pw_len = (Kl | 1) * 3 * (P.nu_max + 1);
+    double* f = scr_dm + pw_len;
+    if (tid < 2 * kPairM) lds[tid] = 0.0;  // zero pair: operand of padding entries, sink of idle lanes
+
+    const int ti = active ? A.itypes[row] : 0;
+    const double* rij_row = A.rij + (size_t)(active ? row : 0) * K * 3;
+    const int* jt_row = A.jtypes + (size_t)(active ? row : 0) * K;
+
+    // ---- A: radial basis, compaction of valid slots (K <= NH slots per row)
+    int nv;
+    {
+        const int k = ht;
+        double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        bool valid = false;
+        if (active && k < K) {
+            v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+        }
+        const uint64_t mask = __ballot(valid);
+        if (lane == 0) wave_cnt[wr] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wr; ++w) off += wave_cnt[w];
+        const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+        if (k < K) cidx[k] = valid ? pos : -1;
+        if (valid) {
+            double fv[kMaxR], dfv[kMaxR];
+            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
+            vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
+            slot_of[pos] = k;
+        }
+        if (ht == 0) {
+            int tot = 0;
+            for (int w = 0; w < NWH; ++w) tot += wave_cnt[w];
+            *nvp = tot;
+        }
+        __syncthreads();
+        nv = __builtin_amdgcn_readfirstlane(*nvp);
+    }
+
+    // ---- B: basic moments of this row (as in the single-row kernel), stored
+    // interleaved
+    {
+        const int np1 = P.nu_max + 1;
+        const int Kp = Kl | 1;
+        double* pw = scr_dm;
+        for (int t = ht; t < nv * 3; t += NH) {
+            const int k = t / 3, q = t - 3 * k;
+            const double inv = 1.0 / rr[k];
+            const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
+            double* col = pw + (size_t)q * np1 * Kp + k;
+            double v = 1.0;
+            col[0] = 1.0;
+            for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
+        }
+        __syncthreads();
+        for (int t = ht; t < P.n_mono; t += NH) {
+            const I4 mo = P.mono[t];
+            const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
+            const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
+            const double* px = pw + (size_t)mo.x * Kp;
+            const double* py = pw + (size_t)(np1 + mo.y) * Kp;
+            const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp;
+            double acc[kMaxR];
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+            switch (gmax) {
+                case 0: basic_moment_sum<1>(px, py, pz, f, nv, acc); break;
+                case 1: basic_moment_sum<2>(px, py, pz, f, nv, acc); break;
+                case 2: basic_moment_sum<3>(px, py, pz, f, nv, acc); break;
+                case 3: basic_moment_sum<4>(px, py, pz, f, nv, acc); break;
+                case 4: basic_moment_sum<5>(px, py, pz, f, nv, acc); break;
+                case 5: basic_moment_sum<6>(px, py, pz, f, nv, acc); break;
+                case 6: basic_moment_sum<7>(px, py, pz, f, nv, acc); break;
+                default: basic_moment_sum<8>(px, py, pz, f, nv, acc); break;
+            }
+            const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu)
+                if (dst[mu] >= 0) m2[2 * dst[mu] + h] = acc[mu];
+        }
+        __syncthreads();
+        // moments that are never targets stay 0 (both rows)
+        for (int x = P.B + tid; x < P.amc; x += NT) reinterpret_cast<double2*>(m2)[x] = make_double2(0.0, 0.0);
+        __syncthreads();
+    }
+
+    // ---- C: contraction layers, both rows
+    for (int d = 0; d < P.n_depth; ++d) contraction_layer_pair<NT, false>(P, d, lds, m2, dm2, part2, tid);
+
+    // ---- D: site energy of this row
+    {
+        double acc = 0.0;
+        for (int s = ht; s < P.S; s += NH) acc += P.moment_coeffs[s] * m2[2 * P.mapping[s] + h];
+        acc = wave_sum(acc);
+        // after the never-operand moments, which alias the adjoint region
+        double* red = dm2 + 2 * (P.amc - P.n_live) + h * NWH;
+        if (lane == 0) red[wr] = acc;
+        __syncthreads();
+        if (ht == 0 && active) {
+            double e = 0.0;
+            for (int w = 0; w < NWH; ++w) e += red[w];
+            A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + e;
+        }
+        __syncthreads();
+    }
+
+    // ---- E: reverse layers, deepest first, both rows
+    for (int li = 0; li <= P.n_depth; ++li) contraction_layer_pair<NT, true>(P, li, lds, m2, dm2, part2, tid);
+
+    // ---- F: per-slot gradient of this row; its half of the dead moment
+    // region holds the per-part gradients and f, f'
+    double* gpart = m2 + h * L.mlen;
+    double* gf = gpart + P.n_parts * Kl * 3;
+    double* gfp = gf + P.R * Kl;
+    for (int c = ht; c < nv; c += NH) {
+        double fv[kMaxR], dfv[kMaxR];
+        radial_slot<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], fv, dfv);
+#pragma unroll
+        for (int mu = 0; mu < kMaxR; ++mu)
+            if (mu < P.R) { gf[mu * Kl + c] = fv[mu]; gfp[mu * Kl + c] = dfv[mu]; }
+    }
+    __syncthreads();
+    {
+        const int prt = wr;
+        const int nu_max = P.nu_max;
+        const double* dmr = dm2 + h;
+        for (int k = lane; k < nv && prt < P.n_parts; k += kWave) {
+            if (nu_max <= 4) slot_gradient<4, 2>(P, prt, k, Kl, dmr, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 6) slot_gradient<6, 2>(P, prt, k, Kl, dmr, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 8) slot_gradient<8, 2>(P, prt, k, Kl, dmr, gf, gfp, vx, vy, vz, rr, gpart);
+            else if (nu_max <= 10) slot_gradient<10, 2>(P, prt, k, Kl, dmr, gf, gfp, vx, vy, vz, rr, gpart);
+            else slot_gradient<kMaxNu, 2>(P, prt, k, Kl, dmr, gf, gfp, vx, vy, vz, rr, gpart);
+        }
+    }
+    __syncthreads();
+    double* gk = scr_dm;  // the adjoints are dead
+    for (int t = ht; t < nv * 3; t += NH) {
+        double s = 0.0;
+        for (int prt = 0; prt < P.n_parts; ++prt) s += gpart[(size_t)prt * Kl * 3 + t];
+        gk[t] = s;
+    }
+    __syncthreads();
+    if (!active) return;  // no barrier follows
+    double* g_row = A.g + (size_t)row * K * 3;
+    for (int t = ht; t < K * 3; t += NH) {
+        const int k = t / 3, q = t - 3 * k;
+        const int c = cidx[k];
+        g_row[t] = c >= 0 ? gk[3 * c + q] : 0.0;
+    }
+    // ---- G: per-atom virial, one component per wavefront of the row
+    if (A.Wi) {
+        for (int c = wr; c < 9; c += NWH) {
+            const int a = c / 3, b = c - 3 * (c / 3);
+            const double* va = a == 0 ? vx : (a == 1 ? vy : vz);
+            double s = 0.0;
+            for (int k = lane; k < nv; k += kWave) s += va[k] * gk[3 * k + b];
+            s = wave_sum(s);
+            if (lane == 0) A.Wi[(size_t)row * 9 + c] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Newton-3 force gather (jax_backend.py:263-303) without FP64 atomics:
 // a reverse-neighbor CSR is built per call; each atom sums its incoming
 // slots in increasing slot order, so results are bitwise reproducible.
@@ -985,6 +1281,20 @@ hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes)
             return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 64>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
         }
+        case kVariantPair1024: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_pair<double, 1024>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_pair<float, 1024>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
+        case kVariantPair512: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_pair<double, 512>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_pair<float, 512>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
         default: {
             hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
@@ -1009,6 +1319,18 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
             hipLaunchKernelGGL(atom_kernel_v2_list<64>, dim3(grid), dim3(64), lds_bytes, stream, P, A);
         else
             return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    if (variant_paired(variant)) {  // plain force evaluation only
+        if (modes || !A.g) return hipErrorInvalidValue;
+        const int pairs = (grid + 1) / 2;
+        if (variant == kVariantPair1024) {
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_pair<float, 1024>), dim3(pairs), dim3(1024), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_pair<double, 1024>), dim3(pairs), dim3(1024), lds_bytes, stream, P, A);
+        } else {
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_pair<float, 512>), dim3(pairs), dim3(512), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_pair<double, 512>), dim3(pairs), dim3(512), lds_bytes, stream, P, A);
+        }
         return hipGetLastError();
     }
     switch (variant) {

@@ -89,18 +89,66 @@ __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K,
     return L;
 }
 
+// LDS of the paired-row kernels: two rows per workgroup whose moments,
+// adjoints and partial sums are interleaved in 16-byte slots (row r of slot
+// x at double 2x + r), so one contraction entry serves both rows.  Slot 0
+// is a zero pair.  Per-row scratch: in phases A/B and after F, row r uses
+// the doubles [2 dm + r dlen, + dlen) of the adjoint region; in F, the
+// doubles [2 m + r mlen, + mlen) of the dead moment region.
+constexpr int kPairM = 1;  // first slot of m
+
+struct AtomLdsPair {
+    int m, dm, part;       // first slots (16 bytes each)
+    int mlen, dlen;        // slots of the m and dm regions
+    int vec;               // double offset of the per-row vectors (vx, vy, vz, r: 4 K per row)
+    int ints;              // double offset of the per-row ints (ints_per_row each)
+    int ints_per_row;
+    int total_doubles;
+};
+
+__host__ __device__ inline AtomLdsPair atom_lds_pair_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
+                                                         int nt, int max_partials)
+{
+    AtomLdsPair L;
+    L.mlen = n_live;
+    if (L.mlen < n_parts * K * 3 + 2 * R * K) L.mlen = n_parts * K * 3 + 2 * R * K;
+    const int pw_len = (K | 1) * 3 * (nu_max + 1);
+    L.dlen = amc;
+    if (L.dlen < pw_len + kFStride * K) L.dlen = pw_len + kFStride * K;
+    if (L.dlen < (amc - n_live) + nt / 2) L.dlen = (amc - n_live) + nt / 2;
+    L.m = kPairM;
+    L.dm = L.m + L.mlen;
+    L.part = L.dm + L.dlen;
+    L.vec = 2 * (L.part + (max_partials > 0 ? max_partials : 1));
+    L.ints = L.vec + 8 * K;
+    L.ints_per_row = 2 * K + nt / 2 / kWave + 2;
+    L.total_doubles = L.ints + (2 * L.ints_per_row + 1) / 2 + 1;
+    return L;
+}
+
 // Atom-kernel variants (chosen per potential by the launch autotuner):
-// v1 = 256 threads, one thread per target moment; v2 = balanced work items
+// v1 = 256 threads, one thread per target moment; v2 = contraction streams
 // with 256, 512 or 64 threads per atom (one wavefront: for low levels, where
-// a wide workgroup per atom leaves the CU waiting on one atom's barriers).
+// a wide workgroup per atom leaves the CU waiting on one atom's barriers);
+// paired = two atoms per workgroup sharing the contraction streams (plain
+// force evaluation only: no per-row coefficients, basis output or row list).
 constexpr int kVariantV1 = 0;
 constexpr int kVariantV2x256 = 1;
 constexpr int kVariantV2x512 = 2;
 constexpr int kVariantV2x64 = 3;
-constexpr int kNumVariants = 4;
-constexpr int variant_threads(int v) { return v == kVariantV2x512 ? 512 : (v == kVariantV2x64 ? 64 : 256); }
+constexpr int kVariantPair1024 = 4;  // two rows per 1024-thread workgroup (512 threads per row)
+constexpr int kVariantPair512 = 5;   // two rows per 512-thread workgroup
+constexpr int kNumVariants = 6;
+constexpr bool variant_paired(int v) { return v == kVariantPair1024 || v == kVariantPair512; }
+constexpr int variant_threads(int v)
+{
+    return v == kVariantPair1024 ? 1024 : ((v == kVariantV2x512 || v == kVariantPair512) ? 512 : (v == kVariantV2x64 ? 64 : 256));
+}
 // gradient parts (thread groups over the basic-moment blocks) per variant
-constexpr int variant_parts(int v) { return v == kVariantV2x512 ? 8 : (v == kVariantV2x64 ? 1 : 4); }
+constexpr int variant_parts(int v)
+{
+    return (v == kVariantV2x512 || v == kVariantPair1024) ? 8 : (v == kVariantV2x64 ? 1 : 4);
+}
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
 size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,

@@ -240,7 +240,7 @@ struct LayerChoice {
 };
 
 void choose_layer(const std::vector<int>& beg, int l0, int l1, int C, int nw, int nt, LayerChoice& lc,
-                  const std::vector<int>& dest_of_row, int part_off)
+                  const std::vector<int>& dest_of_row, const StreamLayout& lay)
 {
     lc = LayerChoice();
     std::vector<int> comb_k;
@@ -254,7 +254,7 @@ void choose_layer(const std::vector<int>& beg, int l0, int l1, int C, int nw, in
         const int k = (n + C - 1) / C, c = (n + k - 1) / k;
         for (int q = 0; q < k; ++q) {
             const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
-            lc.items.push_back(Item{e1 - e0, e0, pos, (part_off + lc.npart) * 8});
+            lc.items.push_back(Item{e1 - e0, e0, pos, (lay.part_off + lc.npart) * lay.unit});
             ++lc.npart;
         }
         comb_k.push_back(k);
@@ -281,16 +281,36 @@ void choose_layer(const std::vector<int>& beg, int l0, int l1, int C, int nw, in
     lc.cost = *std::max_element(load.begin(), load.end()) + kCombSteps * comb_max + (comb_k.empty() ? 0.0 : 2.0);
 }
 
+// LDS byte addresses (from the LDS base) of the two operands of an entry word.
+void stream_operand_bytes(uint32_t x, const StreamLayout& lay, bool rev, uint32_t& a, uint32_t& b)
+{
+    const uint32_t fa = x & 0xffffu, fb = x >> 16;
+    if (lay.absolute) {
+        a = fa * (uint32_t)lay.unit;
+        b = fb * (uint32_t)lay.unit;
+    } else {
+        a = fa;
+        b = fb + (rev ? (uint32_t)(lay.dm_off * lay.unit) : 0u);
+    }
+}
+
 bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair, const std::vector<double>& mult,
                   const std::vector<int>& mom, const std::vector<char>& mapped, int l0, int l1, bool rev, int nt,
-                  int dm_off, int part_off, int part_budget, StreamPlan& sp, int L, std::string& err)
+                  const StreamLayout& lay, int part_budget, StreamPlan& sp, int L, std::string& err)
 {
     const int nw = nt / 64;
-    // entry words (operand byte offsets, FP32 multiplier)
+    // entry words (operand offsets, FP32 multiplier)
+    const uint32_t ms = (uint32_t)lay.m_off;
     auto entry = [&](int e, uint32_t& x, uint32_t& y) -> bool {
         const uint32_t i1 = pair[e] & 0xffffu, i2 = pair[e] >> 16;
-        const uint32_t a = (uint32_t)(kLdsM + i1) * 8u;
-        const uint32_t b = rev ? i2 * 8u : (uint32_t)(kLdsM + i2) * 8u;
+        uint32_t a, b;
+        if (lay.absolute) {
+            a = ms + i1;
+            b = rev ? (uint32_t)lay.dm_off + i2 : ms + i2;
+        } else {
+            a = (ms + i1) * (uint32_t)lay.unit;
+            b = rev ? i2 * (uint32_t)lay.unit : (ms + i2) * (uint32_t)lay.unit;
+        }
         if (a > 0xffffu || b > 0xffffu) return false;
         const float mf = (float)mult[e];
         if ((double)mf != mult[e]) return false;
@@ -300,13 +320,13 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     };
     std::vector<int> dest_of_row(l1 - l0);
     for (int pos = l0; pos < l1; ++pos)
-        dest_of_row[pos - l0] = (rev ? dm_off + mom[pos] : kLdsM + mom[pos]) * 8;
+        dest_of_row[pos - l0] = (rev ? lay.dm_off + mom[pos] : lay.m_off + mom[pos]) * lay.unit;
 
     static const int kC[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 30};
     LayerChoice best, lc;
     bool have = false;
     for (int C : kC) {
-        choose_layer(beg, l0, l1, C, nw, nt, lc, dest_of_row, part_off);
+        choose_layer(beg, l0, l1, C, nw, nt, lc, dest_of_row, lay);
         if (lc.npart > part_budget && C < (1 << 30)) continue;
         if (!have || lc.cost < best.cost) { best = std::move(lc); have = true; }
     }
@@ -322,8 +342,8 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     {
         std::vector<int> first(l1 - l0, -1), cnt(l1 - l0, 0);
         for (const Item& it : best.items) {
-            if (it.dest < part_off * 8) continue;
-            const int pslot = it.dest / 8 - part_off;
+            if (it.dest < lay.part_off * lay.unit) continue;
+            const int pslot = it.dest / lay.unit - lay.part_off;
             int& f0 = first[it.row - l0];
             f0 = f0 < 0 ? pslot : std::min(f0, pslot);
             ++cnt[it.row - l0];
@@ -347,6 +367,8 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
             const int steps = best.gsteps[g];
             const size_t t0 = sp.tbl.size();
             sp.tbl.resize(t0 + (size_t)steps * 64, U4{0u, 0u, 0u, 0u});
+            // entry words of every lane's item
+            std::vector<std::vector<U2>> ent(64);
             for (int l = 0; l < 64; ++l) {
                 const size_t ii = (size_t)g * 64 + l;
                 U4& last = sp.tbl[t0 + (size_t)(steps - 1) * 64 + l];
@@ -361,8 +383,51 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                         err = "an operand offset or multiplier does not fit the packed contraction entries";
                         return false;
                     }
-                    U4& q = sp.tbl[t0 + (size_t)(j / 2) * 64 + l];
-                    if (j % 2 == 0) { q.x = x; q.y = y; } else { q.z = x; q.w = y; }
+                    ent[l].push_back(U2{x, y});
+                }
+            }
+            // Entry slots 0 .. 2 steps - 2 (the last is the destination).
+            // Each slot is read by two LDS instructions (operand a, operand
+            // b) over all lanes; an LDS pass serves `pass` lanes (256 bytes)
+            // and accesses to the same bank at different addresses
+            // serialize.  Slot by slot, each lane takes the remaining entry
+            // of its item whose operands collide least with those already
+            // placed in its pass (ties: item order).  This fixes the order
+            // in which a lane sums its item: deterministic, as every other
+            // order here.
+            const int pass = 256 / lay.unit, nbank = 256 / lay.unit;
+            std::vector<size_t> next(64, 0);
+            std::vector<std::vector<char>> used(64);
+            for (int l = 0; l < 64; ++l) used[l].assign(ent[l].size(), 0);
+            for (int sl = 0; sl < 2 * steps - 1; ++sl) {
+                for (int l0 = 0; l0 < 64; l0 += pass) {
+                    std::vector<std::vector<uint32_t>> ba(nbank), bb(nbank);  // distinct addresses per bank
+                    auto cost = [](const std::vector<uint32_t>& v, uint32_t addr) {
+                        return std::find(v.begin(), v.end(), addr) != v.end() ? 0 : (int)v.size();
+                    };
+                    for (int l = l0; l < l0 + pass && l < 64; ++l) {
+                        int pick = -1, best_c = 0;
+                        uint32_t pa = 0, pb = 0;
+                        for (size_t j = next[l]; j < ent[l].size(); ++j) {
+                            if (used[l][j]) continue;
+                            uint32_t a, b;
+                            stream_operand_bytes(ent[l][j].x, lay, rev, a, b);
+                            const int c = cost(ba[(a / lay.unit) % nbank], a) + cost(bb[(b / lay.unit) % nbank], b);
+                            if (pick < 0 || c < best_c) { pick = (int)j; best_c = c; pa = a; pb = b; }
+                            if (c == 0) break;
+                        }
+                        if (pick < 0) continue;  // item done: zero entries (one broadcast address)
+                        used[l][pick] = 1;
+                        while (next[l] < ent[l].size() && used[l][next[l]]) ++next[l];
+                        auto add = [](std::vector<uint32_t>& v, uint32_t addr) {
+                            if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
+                        };
+                        add(ba[(pa / lay.unit) % nbank], pa);
+                        add(bb[(pb / lay.unit) % nbank], pb);
+                        U4& q = sp.tbl[t0 + (size_t)(sl / 2) * 64 + l];
+                        if (sl % 2 == 0) { q.x = ent[l][pick].x; q.y = ent[l][pick].y; }
+                        else { q.z = ent[l][pick].x; q.w = ent[l][pick].y; }
+                    }
                 }
             }
             step += steps;
@@ -376,7 +441,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
 
 }  // namespace
 
-bool build_items(const HostPlan& p, int nt, int dm_off, int part_off, int part_budget, ItemPlan& ip,
+bool build_items(const HostPlan& p, int nt, const StreamLayout& lay, int part_budget, ItemPlan& ip,
                  std::string& err)
 {
     ip = ItemPlan();
@@ -390,11 +455,11 @@ bool build_items(const HostPlan& p, int nt, int dm_off, int part_off, int part_b
     }
     for (int d = 1; d <= p.n_depth; ++d)
         if (!stream_layer(p.fwd_beg, p.fwd_pair, p.fwd_mult, p.fwd_tgt, no_map, p.fwd_layer[d], p.fwd_layer[d + 1],
-                          false, nt, dm_off, part_off, part_budget, ip.fwd, d - 1, err))
+                          false, nt, lay, part_budget, ip.fwd, d - 1, err))
             return false;
     for (int li = 0; li <= p.n_depth; ++li)
         if (!stream_layer(p.rev_beg, p.rev_pair, p.rev_mult, p.rev_x, mapped, p.rev_layer[li], p.rev_layer[li + 1],
-                          true, nt, dm_off, part_off, part_budget, ip.rev, li, err))
+                          true, nt, lay, part_budget, ip.rev, li, err))
             return false;
     // end markers of the per-layer list ranges
     for (int L = p.n_depth; L < kMaxDepth + 2; ++L) {

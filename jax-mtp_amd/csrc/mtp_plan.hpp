@@ -146,11 +146,25 @@ struct HostPlan {
     int mu_max_of_nu[kMaxNu + 1];
 };
 
-// Streams for a workgroup of nt threads whose LDS holds the adjoints from
-// double dm_off and the partial sums from double part_off (m at kLdsM).
-// part_budget caps the partial sums of a layer when the chunking allows.
-// Returns false when an operand offset does not fit the 16-bit fields.
-bool build_items(const HostPlan& p, int nt, int dm_off, int part_off, int part_budget, ItemPlan& ip,
+// Where the contraction streams find their operands.  Moments, adjoints and
+// partial sums occupy slots of `unit` bytes: 8 for one row per workgroup,
+// 16 for two rows whose values are interleaved (both rows share every
+// entry).  Offsets are in slots from the LDS base.  With `absolute`, the
+// operand fields hold slot indices from the LDS base; otherwise they hold
+// byte offsets (a and the forward b from the LDS base, the reverse b from
+// dm_off).  Group destinations are always byte offsets from the LDS base.
+struct StreamLayout {
+    int unit = 8;
+    int m_off = kLdsM;
+    int dm_off = 0;
+    int part_off = 0;
+    bool absolute = false;
+};
+
+// Streams for a workgroup of nt threads.  part_budget caps the partial sums
+// of a layer when the chunking allows.  Returns false when an operand
+// offset does not fit the 16-bit fields.
+bool build_items(const HostPlan& p, int nt, const StreamLayout& lay, int part_budget, ItemPlan& ip,
                  std::string& err);
 
 // Build the plan from a validated file.  n_parts splits the per-neighbor

@@ -70,8 +70,8 @@ class MTPPotential:
         self.radial_fp32 = bool(radial_fp32)
         capi.check(capi.lib().mtp_set_precision(self.ctx, int(self.radial_fp32)), self.ctx)
 
-    VARIANTS = {0: "per-target, 256 threads", 1: "balanced items, 256 threads", 2: "balanced items, 512 threads",
-                3: "balanced items, 64 threads"}
+    VARIANTS = {0: "per-target, 256 threads", 1: "streams, 256 threads", 2: "streams, 512 threads",
+                3: "streams, 64 threads", 4: "paired rows, 1024 threads", 5: "paired rows, 512 threads"}
 
     @property
     def variant(self) -> int:

@@ -12,7 +12,8 @@
 // 1e-12 and every structural check held.
 //
 // Build: g++ -O2 -std=c++17 -I<csrc> stream_plan_check.cpp mtp_plan.cpp mtp_format.cpp
-// Usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed]
+// Usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair]
+// (pair: two rows interleaved in 16-byte slots, operand fields as slot indices)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -35,9 +36,11 @@ static int fail(const char* msg)
 
 int main(int argc, char** argv)
 {
-    if (argc < 4) return fail("usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed]");
+    if (argc < 4) return fail("usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair]");
     const int nt = std::atoi(argv[2]), K = std::atoi(argv[3]);
     const unsigned seed = argc > 4 ? (unsigned)std::atoi(argv[4]) : 1u;
+    const bool paired = argc > 5 && std::string(argv[5]) == "pair";
+    const int W = paired ? 2 : 1;  // rows sharing the streams (values per LDS slot)
     MtpFile f;
     std::string err;
     if (!read_mtp_file(argv[1], f, err)) return fail(err.c_str());
@@ -53,100 +56,170 @@ int main(int argc, char** argv)
     if (!build_plan(f, 4, p, err)) return fail(err.c_str());
     const MtpFile& g = p.file;  // renumbered
     const int amc = g.alpha_moments_count, B = g.B(), T = g.T(), S = g.S();
-    // a layout of the same shape as the kernel's: zero slot, m, dm, part
+    // a layout of the same shape as the kernel's (in slots): zero, m, dm, part
     const int mlen = std::max(p.n_live, 4 * K * 3 + 2 * g.radial_funcs_count * K);
     const int dlen = std::max({amc, K * 3 * (p.nu_max + 1) + kMaxR * K, amc - p.n_live + nt});
-    const int dm_off = kLdsM + mlen, part_off = dm_off + dlen;
+    StreamLayout lay;
+    if (paired) {
+        lay.unit = 16;
+        lay.m_off = 1;
+        lay.absolute = true;
+    }
+    lay.dm_off = lay.m_off + mlen;
+    lay.part_off = lay.dm_off + dlen;
     ItemPlan ip;
-    if (!build_items(p, nt, dm_off, part_off, 1 << 20, ip, err)) return fail(err.c_str());
+    if (!build_items(p, nt, lay, 1 << 20, ip, err)) return fail(err.c_str());
     const int nw = nt / 64;
-    std::vector<double> lds((size_t)part_off + ip.max_partials + 8, 0.0);
-    auto at = [&](uint32_t byte_off) -> double& {
-        if (byte_off % 8 || byte_off / 8 >= lds.size()) {
-            std::fprintf(stderr, "FAIL: LDS offset %u out of range\n", byte_off);
+    const size_t nslot = (size_t)lay.part_off + ip.max_partials + 8;
+    std::vector<double> lds(nslot * W, 0.0);
+    // slot of an operand field / destination
+    auto slot = [&](uint32_t field, bool rev_b) -> size_t {
+        size_t s;
+        if (lay.absolute) {
+            s = field;
+        } else {
+            if (field % 8) { std::fprintf(stderr, "FAIL: unaligned offset %u\n", field); std::exit(1); }
+            s = field / 8 + (rev_b ? (size_t)lay.dm_off : 0);
+        }
+        if (s >= nslot) { std::fprintf(stderr, "FAIL: LDS slot %zu out of range\n", s); std::exit(1); }
+        return s;
+    };
+    auto dest_slot = [&](uint32_t byte_off) -> size_t {
+        if (byte_off % lay.unit || byte_off / lay.unit >= nslot) {
+            std::fprintf(stderr, "FAIL: destination %u out of range\n", byte_off);
             std::exit(1);
         }
-        return lds[byte_off / 8];
+        return byte_off / lay.unit;
     };
 
     std::mt19937_64 rng(seed);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
-    std::vector<double> m(amc, 0.0), xi(amc, 0.0), coef(S);
-    for (int x = 0; x < B; ++x) m[x] = U(rng);
+    std::vector<double> xi(amc, 0.0), coef(S);
     for (int s = 0; s < S; ++s) coef[s] = U(rng), xi[g.alpha_moment_mapping[s]] += coef[s];
-    // direct forward
-    std::vector<double> md = m;
-    for (int e = 0; e < T; ++e) {
-        const int* t = &g.alpha_index_times[4 * e];
-        md[t[3]] += (double)t[2] * md[t[0]] * md[t[1]];
-    }
-    // direct reverse
-    std::vector<double> dd = xi;
-    for (int e = T - 1; e >= 0; --e) {
-        const int* t = &g.alpha_index_times[4 * e];
-        dd[t[0]] += (double)t[2] * md[t[1]] * dd[t[3]];
-        dd[t[1]] += (double)t[2] * md[t[0]] * dd[t[3]];
+    std::vector<std::vector<double>> md(W, std::vector<double>(amc, 0.0)), dd(W);
+    for (int r = 0; r < W; ++r) {
+        for (int x = 0; x < B; ++x) md[r][x] = U(rng);
+        for (int x = 0; x < B; ++x) lds[(size_t)(lay.m_off + x) * W + r] = md[r][x];
+        // direct forward
+        for (int e = 0; e < T; ++e) {
+            const int* t = &g.alpha_index_times[4 * e];
+            md[r][t[3]] += (double)t[2] * md[r][t[0]] * md[r][t[1]];
+        }
+        // direct reverse
+        dd[r] = xi;
+        for (int e = T - 1; e >= 0; --e) {
+            const int* t = &g.alpha_index_times[4 * e];
+            dd[r][t[0]] += (double)t[2] * md[r][t[1]] * dd[r][t[3]];
+            dd[r][t[1]] += (double)t[2] * md[r][t[0]] * dd[r][t[3]];
+        }
     }
 
     auto run = [&](const StreamPlan& sp, int L, bool rev) -> int {
         for (int w = 0; w < nw; ++w) {
             const I4 wv = sp.wave[(size_t)L * nw + w];
             if (wv.y % kStreamUnroll) return fail("stream length is not a multiple of the unroll");
-            std::vector<double> acc(64, 0.0);
+            std::vector<double> acc(64 * W, 0.0);
             for (int s = wv.x; s < wv.x + wv.y; ++s) {
                 const U4* q = &sp.tbl[(size_t)s * 64];
                 const bool flush = q[0].w == kStreamFlush;
                 for (int l = 0; l < 64; ++l) {
                     if ((q[l].w == kStreamFlush) != flush) return fail("group end differs between lanes");
-                    const uint32_t bb = rev ? (uint32_t)dm_off * 8u : 0u;
                     float mf;
                     std::memcpy(&mf, &q[l].y, 4);
-                    acc[l] = std::fma((double)mf * at(q[l].x & 0xffffu), at(bb + (q[l].x >> 16)), acc[l]);
+                    const size_t a0 = slot(q[l].x & 0xffffu, false), b0 = slot(q[l].x >> 16, rev);
+                    for (int r = 0; r < W; ++r)
+                        acc[l * W + r] = std::fma((double)mf * lds[a0 * W + r], lds[b0 * W + r], acc[l * W + r]);
                     if (!flush) {
                         std::memcpy(&mf, &q[l].w, 4);
-                        acc[l] = std::fma((double)mf * at(q[l].z & 0xffffu), at(bb + (q[l].z >> 16)), acc[l]);
+                        const size_t a1 = slot(q[l].z & 0xffffu, false), b1 = slot(q[l].z >> 16, rev);
+                        for (int r = 0; r < W; ++r)
+                            acc[l * W + r] = std::fma((double)mf * lds[a1 * W + r], lds[b1 * W + r], acc[l * W + r]);
                     }
                 }
                 if (flush)
-                    for (int l = 0; l < 64; ++l) {
-                        at(q[l].z) = acc[l];
-                        acc[l] = 0.0;
-                    }
+                    for (int l = 0; l < 64; ++l)
+                        for (int r = 0; r < W; ++r) {
+                            lds[dest_slot(q[l].z) * W + r] = acc[l * W + r];
+                            acc[l * W + r] = 0.0;
+                        }
             }
         }
-        if (lds[0] != 0.0) return fail("the zero slot was written with a non-zero value");
-        double* dst = rev ? &lds[dm_off] : &lds[kLdsM];
-        for (int c = sp.cb_beg[L]; c < sp.cb_beg[L + 1]; ++c) {
-            const I4 cb = sp.comb[c];
-            double a = rev ? xi[cb.x] : 0.0;
-            for (int q = cb.y; q < cb.z; ++q) a += lds[(size_t)part_off + q];
-            dst[cb.x] = a;
-        }
-        if (rev) {
-            for (int t = sp.bo_beg[L]; t < sp.bo_beg[L + 1]; ++t) lds[(size_t)dm_off + sp.bo[t]] = xi[sp.bo[t]];
-            for (int t = sp.xa_beg[L]; t < sp.xa_beg[L + 1]; ++t) lds[(size_t)dm_off + sp.xa[t]] += xi[sp.xa[t]];
+        for (int r = 0; r < W; ++r)
+            if (lds[r] != 0.0) return fail("the zero slot was written with a non-zero value");
+        const size_t dst = rev ? lay.dm_off : lay.m_off;
+        for (int r = 0; r < W; ++r) {
+            for (int c = sp.cb_beg[L]; c < sp.cb_beg[L + 1]; ++c) {
+                const I4 cb = sp.comb[c];
+                double a = rev ? xi[cb.x] : 0.0;
+                for (int q = cb.y; q < cb.z; ++q) a += lds[((size_t)lay.part_off + q) * W + r];
+                lds[(dst + cb.x) * W + r] = a;
+            }
+            if (rev) {
+                for (int t = sp.bo_beg[L]; t < sp.bo_beg[L + 1]; ++t)
+                    lds[(dst + sp.bo[t]) * W + r] = xi[sp.bo[t]];
+                for (int t = sp.xa_beg[L]; t < sp.xa_beg[L + 1]; ++t)
+                    lds[(dst + sp.xa[t]) * W + r] += xi[sp.xa[t]];
+            }
         }
         return 0;
     };
 
-    for (int x = 0; x < B; ++x) lds[(size_t)kLdsM + x] = m[x];
     for (int d = 0; d < p.n_depth; ++d)
         if (run(ip.fwd, d, false)) return 1;
-    double ef = 0.0, scale_f = 0.0;
-    for (int x = 0; x < amc; ++x) scale_f = std::max(scale_f, std::fabs(md[x]));
-    for (int x = 0; x < amc; ++x) ef = std::max(ef, std::fabs(lds[(size_t)kLdsM + x] - md[x]));
-    ef /= scale_f;
+    double ef = 0.0, er = 0.0;
+    for (int r = 0; r < W; ++r) {
+        double sc = 0.0, e = 0.0;
+        for (int x = 0; x < amc; ++x) sc = std::max(sc, std::fabs(md[r][x]));
+        for (int x = 0; x < amc; ++x)
+            e = std::max(e, std::fabs(lds[(size_t)(lay.m_off + x) * W + r] - md[r][x]));
+        ef = std::max(ef, e / sc);
+    }
     // the reverse pass reads m[0, n_live) only: keep those, as the kernel does
     for (int li = 0; li <= p.n_depth; ++li)
         if (run(ip.rev, li, true)) return 1;
-    double er = 0.0, scale_r = 0.0;
-    for (int x = 0; x < amc; ++x) scale_r = std::max(scale_r, std::fabs(dd[x]));
-    for (int x = 0; x < amc; ++x) er = std::max(er, std::fabs(lds[(size_t)dm_off + x] - dd[x]));
-    er /= scale_r;
+    for (int r = 0; r < W; ++r) {
+        double sc = 0.0, e = 0.0;
+        for (int x = 0; x < amc; ++x) sc = std::max(sc, std::fabs(dd[r][x]));
+        for (int x = 0; x < amc; ++x)
+            e = std::max(e, std::fabs(lds[(size_t)(lay.dm_off + x) * W + r] - dd[r][x]));
+        er = std::max(er, e / sc);
+    }
     int steps = 0;
     for (const I4& w : ip.fwd.wave) steps += w.y;
     for (const I4& w : ip.rev.wave) steps += w.y;
-    std::printf("level %d nt %d: forward %.3e reverse %.3e partials %d steps %d cost %.1f/%.1f\n", p.level, nt, ef, er,
-                ip.max_partials, steps, ip.fwd.cost, ip.rev.cost);
+    // LDS passes per operand read: lanes per pass = 256 / unit bytes, a pass
+    // costs the largest number of distinct addresses in one bank
+    double passes = 0.0, ideal = 0.0;
+    {
+        const int pass = 256 / lay.unit, nbank = 256 / lay.unit;
+        for (int dir = 0; dir < 2; ++dir) {
+            const StreamPlan& sp = dir ? ip.rev : ip.fwd;
+            for (size_t st = 0; st < sp.tbl.size() / 64; ++st) {
+                const U4* q = &sp.tbl[st * 64];
+                for (int half = 0; half < 2; ++half) {
+                    if (half == 1 && q[0].w == kStreamFlush) continue;
+                    for (int op = 0; op < 2; ++op)
+                        for (int l0 = 0; l0 < 64; l0 += pass) {
+                            std::vector<std::vector<uint32_t>> banks(nbank);
+                            for (int l = l0; l < l0 + pass; ++l) {
+                                const uint32_t x = half ? q[l].z : q[l].x;
+                                const uint32_t fld = op ? x >> 16 : x & 0xffffu;
+                                uint32_t addr = lay.absolute ? fld * lay.unit
+                                                             : fld + ((op && dir) ? (uint32_t)lay.dm_off * 8u : 0u);
+                                auto& v = banks[(addr / lay.unit) % nbank];
+                                if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
+                            }
+                            size_t mx = 1;
+                            for (auto& v : banks) mx = std::max(mx, v.size());
+                            passes += (double)mx;
+                            ideal += 1.0;
+                        }
+                }
+            }
+        }
+    }
+    std::printf("LDS passes per operand read: %.2f (conflict-free 1)\n", passes / ideal);
+    std::printf("level %d nt %d rows %d: forward %.3e reverse %.3e partials %d steps %d cost %.1f/%.1f\n", p.level, nt,
+                W, ef, er, ip.max_partials, steps, ip.fwd.cost, ip.rev.cost);
     return (ef < 1e-12 && er < 1e-12) ? 0 : 1;
 }

@@ -56,7 +56,7 @@ def _check(gpu, ref, tol=TOL):
     assert _rel(gpu["s6"], ref["s6"]) <= tol
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("level", [2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24])
 def test_levels_random_cells(jm, O, level, variant):
     w = jm.configs.small_cluster(level, 3, 24, seed=100 + level, cell=8.0, cutoff=5.0, max_nbr=80)
@@ -82,15 +82,18 @@ def test_bcc2889_level16(jm, O):
     _check(_run_gpu(jm, w, nb), _run_oracle(O, w, nb))
 
 
-def test_masks_and_padding_rows(jm, O):
-    """natoms_energy / natoms_force below the row count (jax_backend.py:279-301, 447-452)."""
+@pytest.mark.parametrize("variant", [1, 2, 4, 5])
+@pytest.mark.parametrize("rows", [24, 25])
+def test_masks_and_padding_rows(jm, O, rows, variant):
+    """natoms_energy / natoms_force below the row count (jax_backend.py:279-301, 447-452); an odd row count
+    leaves the paired-row kernels' last workgroup with one row."""
     w = jm.configs.small_cluster(12, 2, 20, seed=9, cell=7.5, cutoff=5.0, max_nbr=64)
-    nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, w.cutoff, w.max_nbr, rows=24)
-    print("[case] masks rows=24 natoms=20 n_e=17 n_f=15", flush=True)
-    _check(_run_gpu(jm, w, nb, n_e=17, n_f=15), _run_oracle(O, w, nb, n_e=17, n_f=15))
+    nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, w.cutoff, w.max_nbr, rows=rows)
+    print(f"[case] masks rows={rows} natoms=20 n_e=17 n_f=15 variant={variant}", flush=True)
+    _check(_run_gpu(jm, w, nb, n_e=17, n_f=15, variant=variant), _run_oracle(O, w, nb, n_e=17, n_f=15))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_fcc32k_level24_full_size(jm, O, variant):
     """The 32,000-atom level-24 workload at full size against the oracle."""
     w = jm.configs.fcc32k_level24(seed=0)

@@ -31,9 +31,10 @@ def checker(tmp_path_factory):
 
 
 @pytest.mark.parametrize("level", [2, 6, 10, 12, 16, 20, 24])
-@pytest.mark.parametrize("threads", [64, 256, 512])
-def test_streams_reproduce_the_contraction(checker, level, threads):
+@pytest.mark.parametrize("threads,layout", [(64, "one"), (256, "one"), (512, "one"), (512, "pair"), (1024, "pair")])
+def test_streams_reproduce_the_contraction(checker, level, threads, layout):
+    """One row per workgroup (8-byte slots, byte offsets) and two interleaved rows (16-byte slots, slot indices)."""
     for max_nbr in (32, 80):
         r = subprocess.run([str(checker), str(POT / f"untrained_{level:02d}.mtp"), str(threads), str(max_nbr),
-                            str(level)], capture_output=True, text=True)
+                            str(level), layout], capture_output=True, text=True)
         assert r.returncode == 0, r.stdout + r.stderr

@@ -66,6 +66,42 @@ __device__ __forceinline__ void radial_slot(const DevPlan& P, double r, int ti, 
     }
 }
 
+// One radial function of one slot: f_mu(r) and d f_mu / d r, with the same
+// operations as radial_slot for that mu (so the results are identical).  The
+// coefficient loads of a block of eight Chebyshev terms are independent and
+// issue together; one thread per (slot, mu) spreads the basis over a
+// workgroup.
+template <typename Real>
+__device__ __forceinline__ void radial_one(const DevPlan& P, double r, int ti, int tk, int mu, double& f, double& df)
+{
+    const Real rmin = (Real)P.min_dist, rmax = (Real)P.max_dist;
+    const Real rr = (Real)r;
+    const Real range_inv = (Real)1 / (rmax - rmin);
+    const Real x = ((Real)2 * rr - (rmin + rmax)) * range_inv;
+    const Real dxdr = (Real)2 * range_inv;
+    const Real s = (rmax - rr) * (rmax - rr);
+    const Real ds = (Real)-2 * (rmax - rr);
+    const double* c =
+        P.radial_coeffs + ((size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R + mu) * P.N;
+    Real af = (Real)c[0], ad = 0;
+    if (P.N > 1) {
+        af += (Real)c[1] * x;
+        ad += (Real)c[1];
+        Real t2 = 1, t1 = x, d2 = 0, d1 = 1;
+#pragma unroll 8
+        for (int n = 2; n < P.N; ++n) {
+            const Real t0 = (Real)2 * x * t1 - t2;
+            const Real d0 = (Real)2 * t1 + (Real)2 * x * d1 - d2;
+            af += (Real)c[n] * t0;
+            ad += (Real)c[n] * d0;
+            t2 = t1; t1 = t0; d2 = d1; d1 = d0;
+        }
+    }
+    const Real sc = (Real)P.scaling;
+    f = (double)(sc * s * af);
+    df = (double)(sc * (ds * af + s * ad * dxdr));
+}
+
 // FP64 radial basis of one slot with the per-coefficient factors used by the
 // parameter gradient: a[n*stride] = d f_mu / d c_n = scaling * s * T_n and
 // b[n*stride] = d f'_mu / d c_n = scaling * (s' T_n + s T_n' dx/dr).

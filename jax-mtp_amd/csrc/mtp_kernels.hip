@@ -464,7 +464,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const double* rij_row = A.rij + (size_t)row * K * 3;
     const int* jt_row = A.jtypes + (size_t)row * K;
 
-    // ---- A: radial basis, compaction of valid slots in slot order
+    // ---- A: compaction of valid slots in slot order (radial basis below)
     int nv = 0;
     if constexpr (NT < kAtomBlock) {  // slots in chunks of NT (K may exceed NT)
         for (int k0 = 0; k0 < K; k0 += NT) {
@@ -485,11 +485,6 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
             if (k < K) cidx[k] = valid ? pos : -1;
             if (valid) {
-                double fv[kMaxR], dfv[kMaxR];
-                radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
-#pragma unroll
-                for (int mu = 0; mu < kMaxR; ++mu)
-                    if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
                 vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
                 slot_of[pos] = k;
             }
@@ -515,11 +510,6 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
         if (k < K) cidx[k] = valid ? pos : -1;
         if (valid) {
-            double fv[kMaxR], dfv[kMaxR];
-            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             slot_of[pos] = k;
         }
@@ -541,6 +531,12 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int np1 = P.nu_max + 1;
     const int Kp = P.k_layout | 1;
     double* pw = dm;
+    for (int i = tid; i < nv * P.R; i += NT) {  // radial values: one thread per (slot, mu)
+        const int c = i / P.R, mu = i - c * P.R;
+        double fv, dfv;
+        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+        f[c * kFStride + mu] = fv;
+    }
     for (int t = tid; t < nv * 3; t += NT) {
         const int k = t / 3, q = t - 3 * k;
         const double inv = 1.0 / rr[k];
@@ -628,12 +624,12 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* gpart = m;
     double* gf = lds + L.gf;
     double* gfp = lds + L.gfp;
-    for (int c = tid; c < nv; c += NT) {
-        double fv[kMaxR], dfv[kMaxR];
-        radial_slot<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], fv, dfv);
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu)
-            if (mu < P.R) { gf[mu * K + c] = fv[mu]; gfp[mu * K + c] = dfv[mu]; }
+    for (int i = tid; i < nv * P.R; i += NT) {
+        const int c = i / P.R, mu = i - c * P.R;
+        double fv, dfv;
+        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+        gf[mu * K + c] = fv;
+        gfp[mu * K + c] = dfv;
     }
     __syncthreads();
     {
@@ -836,7 +832,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
     const double* rij_row = A.rij + (size_t)(active ? row : 0) * K * 3;
     const int* jt_row = A.jtypes + (size_t)(active ? row : 0) * K;
 
-    // ---- A: radial basis, compaction of valid slots (K <= NH slots per row)
+    // ---- A: compaction of valid slots (K <= NH slots per row; radial basis below)
     int nv;
     {
         const int k = ht;
@@ -855,11 +851,6 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
         const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
         if (k < K) cidx[k] = valid ? pos : -1;
         if (valid) {
-            double fv[kMaxR], dfv[kMaxR];
-            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < P.R) f[pos * kFStride + mu] = fv[mu];
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             slot_of[pos] = k;
         }
@@ -878,6 +869,12 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
         const int np1 = P.nu_max + 1;
         const int Kp = Kl | 1;
         double* pw = scr_dm;
+        for (int i = ht; i < nv * P.R; i += NH) {  // radial values: one thread per (slot, mu)
+            const int c = i / P.R, mu = i - c * P.R;
+            double fv, dfv;
+            radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+            f[c * kFStride + mu] = fv;
+        }
         for (int t = ht; t < nv * 3; t += NH) {
             const int k = t / 3, q = t - 3 * k;
             const double inv = 1.0 / rr[k];
@@ -947,12 +944,12 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
     double* gpart = m2 + h * L.mlen;
     double* gf = gpart + P.n_parts * Kl * 3;
     double* gfp = gf + P.R * Kl;
-    for (int c = ht; c < nv; c += NH) {
-        double fv[kMaxR], dfv[kMaxR];
-        radial_slot<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], fv, dfv);
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu)
-            if (mu < P.R) { gf[mu * Kl + c] = fv[mu]; gfp[mu * Kl + c] = dfv[mu]; }
+    for (int i = ht; i < nv * P.R; i += NH) {
+        const int c = i / P.R, mu = i - c * P.R;
+        double fv, dfv;
+        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+        gf[mu * Kl + c] = fv;
+        gfp[mu * Kl + c] = dfv;
     }
     __syncthreads();
     {

@@ -162,6 +162,59 @@ __device__ __forceinline__ void grad_block(const double* __restrict__ d, const d
     }
 }
 
+// grad_block without a z power table: along a row of monomials (fixed a,
+// b decreasing) c increases by one, so u_z^c is carried as a running
+// product, built by the same multiplications as the table entries (the
+// results are identical).  Frees PN registers of doubles.
+template <int NU, int DS = 1>
+__device__ __forceinline__ void grad_block_xy(const double* __restrict__ d, const double* px, const double* py,
+                                              double uz, double& Hx, double& Hy, double& Hz)
+{
+#pragma unroll
+    for (int a = NU - 1; a >= 0; --a) {
+        double pzc = 1.0;
+#pragma unroll
+        for (int b = NU - 1 - a; b >= 0; --b) {
+            const int c = NU - 1 - a - b;
+            const double P = px[a] * py[b] * pzc;
+            Hx += (double)(a + 1) * d[DS * mono_idx(NU, a + 1, b)] * P;
+            Hy += (double)(b + 1) * d[DS * mono_idx(NU, a, b + 1)] * P;
+            Hz += (double)(c + 1) * d[DS * mono_idx(NU, a, b)] * P;
+            pzc *= uz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int PN>
+__device__ __forceinline__ void opaque_powers_xy(double* px, double* py)
+{
+#pragma unroll
+    for (int p = 0; p < PN; ++p) asm volatile("" : "+v"(px[p]), "+v"(py[p]));
+}
+
+template <int PN, int DS = 1>
+__device__ __forceinline__ void grad_block_dispatch_xy(int nu, const double* d, double* px, double* py, double uz,
+                                                       double& Hx, double& Hy, double& Hz)
+{
+    opaque_powers_xy<PN>(px, py);
+    switch (nu) {
+        case 1: grad_block_xy<1, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 2: if constexpr (PN >= 2) grad_block_xy<2, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 3: if constexpr (PN >= 3) grad_block_xy<3, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 4: if constexpr (PN >= 4) grad_block_xy<4, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 5: if constexpr (PN >= 5) grad_block_xy<5, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 6: if constexpr (PN >= 6) grad_block_xy<6, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 7: if constexpr (PN >= 7) grad_block_xy<7, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 8: if constexpr (PN >= 8) grad_block_xy<8, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 9: if constexpr (PN >= 9) grad_block_xy<9, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 10: if constexpr (PN >= 10) grad_block_xy<10, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 11: if constexpr (PN >= 11) grad_block_xy<11, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        case 12: if constexpr (PN >= 12) grad_block_xy<12, DS>(d, px, py, uz, Hx, Hy, Hz); break;
+        default: break;
+    }
+}
+
 // Called once per (mu, nu) block inside a loop over blocks.  The monomial
 // products only depend on the slot, so the compiler would hoist all of them
 // (every order 1..12) out of that loop and spill; the empty asm marks the

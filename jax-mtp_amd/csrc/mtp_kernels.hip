@@ -384,10 +384,10 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
 {
     const double ir = 1.0 / rr[k];
     const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
-    double px[PN], py[PN], pz[PN];
-    px[0] = py[0] = pz[0] = 1.0;
+    double px[PN], py[PN];
+    px[0] = py[0] = 1.0;
 #pragma unroll
-    for (int p = 1; p < PN; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
+    for (int p = 1; p < PN; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; }
     double sdf = 0.0, sfn = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
     const int b1 = P.part_beg[prt + 1];
     for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
@@ -399,7 +399,7 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
             sdf += fpm * d[0];
         } else {
             double Hx = 0.0, Hy = 0.0, Hz = 0.0;
-            grad_block_dispatch<PN, DS>(nu, d, px, py, pz, Hx, Hy, Hz);
+            grad_block_dispatch_xy<PN, DS>(nu, d, px, py, uz, Hx, Hy, Hz);
             const double uH = ux * Hx + uy * Hy + uz * Hz;
             sdf += fpm * uH * (1.0 / (double)nu);
             sfn += fm * uH;
@@ -413,19 +413,30 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
 }
 
 // sum_k f_mu(k) * u_x^a u_y^b u_z^c over the valid slots for mu < NMU
-// (powers px[k], py[k], pz[k]; f[k * kFStride + mu]), four slots per step.
+// (powers px[k], py[k], pz[k]; f[k * kFStride + mu]).  Slots go in batches
+// whose LDS reads are all issued before their arithmetic (the scheduling
+// barrier keeps the compiler from interleaving each read with its FMAs,
+// which would expose one LDS latency per read).
 template <int NMU>
 __device__ __forceinline__ void basic_moment_sum(const double* px, const double* py, const double* pz,
                                                  const double* f, int nv, double* acc)
 {
+    constexpr int UB = NMU <= 2 ? 8 : (NMU <= 4 ? 4 : 2);
     int k = 0;
-    for (; k + 4 <= nv; k += 4) {
+    for (; k + UB <= nv; k += UB) {
+        double a[UB], b[UB], c[UB], fr[UB][NMU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double Pm = px[k + u] * py[k + u] * pz[k + u];
-            const double* fr = f + (k + u) * kFStride;
+        for (int u = 0; u < UB; ++u) {
+            a[u] = px[k + u]; b[u] = py[k + u]; c[u] = pz[k + u];
 #pragma unroll
-            for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
+            for (int mu = 0; mu < NMU; ++mu) fr[u][mu] = f[(k + u) * kFStride + mu];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const double Pm = a[u] * b[u] * c[u];
+#pragma unroll
+            for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[u][mu], Pm, acc[mu]);
         }
     }
     for (; k < nv; ++k) {
@@ -434,6 +445,60 @@ __device__ __forceinline__ void basic_moment_sum(const double* px, const double*
 #pragma unroll
         for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
     }
+}
+
+// Basic moments of one row from its power table pw (row stride Kp, np1
+// orders per component) and radial values f: monomial t on lane t % 64 of
+// wavefront (t / 64) % NW, all its mu; m[ms * x] = value of basic moment x.
+// Called by all NW * 64 threads of the row (wave w, lane); ends with a
+// barrier.
+__device__ __forceinline__ void basic_mono_unit(const DevPlan& P, int t, int k0, int k1, const double* pw, int Kp,
+                                                int np1, const double* f, double* acc)
+{
+    const I4 mo = P.mono[t];
+    const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
+    const double* px = pw + (size_t)mo.x * Kp + k0;
+    const double* py = pw + (size_t)(np1 + mo.y) * Kp + k0;
+    const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp + k0;
+    const double* fk = f + k0 * kFStride;
+    const int n = k1 - k0;
+#pragma unroll
+    for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+    switch (gmax) {
+        case 0: basic_moment_sum<1>(px, py, pz, fk, n, acc); break;
+        case 1: basic_moment_sum<2>(px, py, pz, fk, n, acc); break;
+        case 2: basic_moment_sum<3>(px, py, pz, fk, n, acc); break;
+        case 3: basic_moment_sum<4>(px, py, pz, fk, n, acc); break;
+        case 4: basic_moment_sum<5>(px, py, pz, fk, n, acc); break;
+        case 5: basic_moment_sum<6>(px, py, pz, fk, n, acc); break;
+        case 6: basic_moment_sum<7>(px, py, pz, fk, n, acc); break;
+        default: basic_moment_sum<8>(px, py, pz, fk, n, acc); break;
+    }
+}
+
+__device__ __forceinline__ void store_basic(const DevPlan& P, int t, const double* acc, double* m, int ms)
+{
+    const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
+    const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+    for (int mu = 0; mu < kMaxR; ++mu)
+        if (dst[mu] >= 0) m[ms * dst[mu]] = acc[mu];
+}
+
+template <int NW>
+__device__ __forceinline__ void basic_moments(const DevPlan& P, int w, int lane, int nv, const double* pw, int Kp,
+                                              int np1, const double* f, double* m, int ms)
+{
+    const int ngrp = (P.n_mono + kWave - 1) / kWave;
+    double acc[kMaxR];
+    for (int g = w; g < ngrp; g += NW) {
+        const int t = g * kWave + lane;
+        if (t < P.n_mono) {
+            basic_mono_unit(P, t, 0, nv, pw, Kp, np1, f, acc);
+            store_basic(P, t, acc, m, ms);
+        }
+    }
+    __syncthreads();
 }
 
 template <typename RadialReal, int NT>
@@ -547,32 +612,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
     }
     __syncthreads();
-    for (int t = tid; t < P.n_mono; t += NT) {
-        const I4 mo = P.mono[t];
-        const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
-        const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
-        const double* px = pw + (size_t)mo.x * Kp;
-        const double* py = pw + (size_t)(np1 + mo.y) * Kp;
-        const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp;
-        double acc[kMaxR];
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
-        switch (gmax) {
-            case 0: basic_moment_sum<1>(px, py, pz, f, nv, acc); break;
-            case 1: basic_moment_sum<2>(px, py, pz, f, nv, acc); break;
-            case 2: basic_moment_sum<3>(px, py, pz, f, nv, acc); break;
-            case 3: basic_moment_sum<4>(px, py, pz, f, nv, acc); break;
-            case 4: basic_moment_sum<5>(px, py, pz, f, nv, acc); break;
-            case 5: basic_moment_sum<6>(px, py, pz, f, nv, acc); break;
-            case 6: basic_moment_sum<7>(px, py, pz, f, nv, acc); break;
-            default: basic_moment_sum<8>(px, py, pz, f, nv, acc); break;
-        }
-        const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu)
-            if (dst[mu] >= 0) m[dst[mu]] = acc[mu];
-    }
-    __syncthreads();
+    basic_moments<NW>(P, tid / kWave, tid & (kWave - 1), nv, pw, Kp, np1, f, m, 1);
     // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
     // region, free once the power table and f are consumed)
     for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
@@ -885,32 +925,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
             for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
         }
         __syncthreads();
-        for (int t = ht; t < P.n_mono; t += NH) {
-            const I4 mo = P.mono[t];
-            const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
-            const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
-            const double* px = pw + (size_t)mo.x * Kp;
-            const double* py = pw + (size_t)(np1 + mo.y) * Kp;
-            const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp;
-            double acc[kMaxR];
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
-            switch (gmax) {
-                case 0: basic_moment_sum<1>(px, py, pz, f, nv, acc); break;
-                case 1: basic_moment_sum<2>(px, py, pz, f, nv, acc); break;
-                case 2: basic_moment_sum<3>(px, py, pz, f, nv, acc); break;
-                case 3: basic_moment_sum<4>(px, py, pz, f, nv, acc); break;
-                case 4: basic_moment_sum<5>(px, py, pz, f, nv, acc); break;
-                case 5: basic_moment_sum<6>(px, py, pz, f, nv, acc); break;
-                case 6: basic_moment_sum<7>(px, py, pz, f, nv, acc); break;
-                default: basic_moment_sum<8>(px, py, pz, f, nv, acc); break;
-            }
-            const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (dst[mu] >= 0) m2[2 * dst[mu] + h] = acc[mu];
-        }
-        __syncthreads();
+        basic_moments<NWH>(P, wr, lane, nv, pw, Kp, np1, f, m2 + h, 2);
         // moments that are never targets stay 0 (both rows)
         for (int x = P.B + tid; x < P.amc; x += NT) reinterpret_cast<double2*>(m2)[x] = make_double2(0.0, 0.0);
         __syncthreads();

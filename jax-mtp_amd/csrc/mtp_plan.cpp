@@ -388,14 +388,15 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
             }
             // Entry slots 0 .. 2 steps - 2 (the last is the destination).
             // Each slot is read by two LDS instructions (operand a, operand
-            // b) over all lanes; an LDS pass serves `pass` lanes (256 bytes)
-            // and accesses to the same bank at different addresses
-            // serialize.  Slot by slot, each lane takes the remaining entry
+            // b) over all lanes.  On gfx950 the LDS serves such a read 16
+            // lanes at a time, and two of those lanes whose slots differ
+            // but coincide modulo 16 slots (128 bytes for 8-byte slots, 256
+            // for 16-byte slots) serialize.  Slot by slot, each lane takes the remaining entry
             // of its item whose operands collide least with those already
             // placed in its pass (ties: item order).  This fixes the order
             // in which a lane sums its item: deterministic, as every other
             // order here.
-            const int pass = 256 / lay.unit, nbank = 256 / lay.unit;
+            const int pass = 16, nbank = 16;
             std::vector<size_t> next(64, 0);
             std::vector<std::vector<char>> used(64);
             for (int l = 0; l < 64; ++l) used[l].assign(ent[l].size(), 0);

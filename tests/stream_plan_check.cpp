@@ -187,11 +187,12 @@ int main(int argc, char** argv)
     int steps = 0;
     for (const I4& w : ip.fwd.wave) steps += w.y;
     for (const I4& w : ip.rev.wave) steps += w.y;
-    // LDS passes per operand read: lanes per pass = 256 / unit bytes, a pass
-    // costs the largest number of distinct addresses in one bank
+    // LDS passes per operand read (gfx950: 16 lanes per pass, slots that
+    // coincide modulo 16 collide); a pass costs the largest number of
+    // distinct slots that collide
     double passes = 0.0, ideal = 0.0;
     {
-        const int pass = 256 / lay.unit, nbank = 256 / lay.unit;
+        const int pass = 16, nbank = 16;
         for (int dir = 0; dir < 2; ++dir) {
             const StreamPlan& sp = dir ? ip.rev : ip.fwd;
             for (size_t st = 0; st < sp.tbl.size() / 64; ++st) {
@@ -219,6 +220,17 @@ int main(int argc, char** argv)
         }
     }
     std::printf("LDS passes per operand read: %.2f (conflict-free 1)\n", passes / ideal);
+    if (std::getenv("STREAM_PLAN_VERBOSE")) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const StreamPlan& sp = dir ? ip.rev : ip.fwd;
+            const int nl = (int)sp.wave.size() / nw;
+            for (int L = 0; L < nl; ++L) {
+                std::printf("%s layer %d steps per wave:", dir ? "rev" : "fwd", L);
+                for (int w = 0; w < nw; ++w) std::printf(" %d", sp.wave[(size_t)L * nw + w].y);
+                std::printf(" | comb %d\n", sp.cb_beg[L + 1] - sp.cb_beg[L]);
+            }
+        }
+    }
     std::printf("level %d nt %d rows %d: forward %.3e reverse %.3e partials %d steps %d cost %.1f/%.1f\n", p.level, nt,
                 W, ef, er, ip.max_partials, steps, ip.fwd.cost, ip.rev.cost);
     return (ef < 1e-12 && er < 1e-12) ? 0 : 1;

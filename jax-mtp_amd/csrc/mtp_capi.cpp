@@ -300,7 +300,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                 lay.absolute = true;
                 part_budget = std::max(64, ((int)(lds_max / sizeof(double)) - L0.total_doubles) / 2);
             } else {
-                const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, f.B(), max_nbr, f.radial_funcs_count,
                                                      p.nu_max, dv.n_parts, nt, 0, 0);
                 const int half_cu = (int)(lds_max / 2 / sizeof(double));
                 part_budget = std::max(64, half_cu - L0.total_doubles);
@@ -340,14 +340,14 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             const size_t lds = paired ? sizeof(double) * (size_t)atom_lds_pair_layout(
                                                 f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
                                                 p.nu_max, dv.n_parts, nt, ip.max_partials).total_doubles
-                                      : atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr,
+                                      : atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
                                                         f.radial_funcs_count, p.nu_max, dv.n_parts, nt,
                                                         ip.max_partials);
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
                 c->vdev[v] = dv;
                 c->vlds[v] = lds;
                 if (paired) continue;  // plain force evaluation only
-                const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
+                const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr, f.radial_funcs_count,
                                                    p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
                 if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
                     c->vlds_xi[v] = lxi;

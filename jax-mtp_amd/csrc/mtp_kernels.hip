@@ -270,11 +270,11 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // threads of a wave read consecutive 8-byte entries (coalesced), and each
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
-size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,
-                       int xi_len)
+size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
+                       int max_partials, int xi_len)
 {
     return sizeof(double) *
-           (size_t)atom_lds2_layout(amc, n_live, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
+           (size_t)atom_lds2_layout(amc, n_live, B, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
 }
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
@@ -413,13 +413,13 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
 }
 
 // sum_k f_mu(k) * u_x^a u_y^b u_z^c over the valid slots for mu < NMU
-// (powers px[k], py[k], pz[k]; f[k * kFStride + mu]).  Slots go in batches
+// (powers px[k], py[k], pz[k]; f[k * R + mu]).  Slots go in batches
 // whose LDS reads are all issued before their arithmetic (the scheduling
 // barrier keeps the compiler from interleaving each read with its FMAs,
 // which would expose one LDS latency per read).
 template <int NMU>
 __device__ __forceinline__ void basic_moment_sum(const double* px, const double* py, const double* pz,
-                                                 const double* f, int nv, double* acc)
+                                                 const double* f, int R, int nv, double* acc)
 {
     constexpr int UB = NMU <= 2 ? 8 : (NMU <= 4 ? 4 : 2);
     int k = 0;
@@ -429,7 +429,7 @@ __device__ __forceinline__ void basic_moment_sum(const double* px, const double*
         for (int u = 0; u < UB; ++u) {
             a[u] = px[k + u]; b[u] = py[k + u]; c[u] = pz[k + u];
 #pragma unroll
-            for (int mu = 0; mu < NMU; ++mu) fr[u][mu] = f[(k + u) * kFStride + mu];
+            for (int mu = 0; mu < NMU; ++mu) fr[u][mu] = f[(k + u) * R + mu];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -441,7 +441,7 @@ __device__ __forceinline__ void basic_moment_sum(const double* px, const double*
     }
     for (; k < nv; ++k) {
         const double Pm = px[k] * py[k] * pz[k];
-        const double* fr = f + k * kFStride;
+        const double* fr = f + k * R;
 #pragma unroll
         for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(fr[mu], Pm, acc[mu]);
     }
@@ -460,19 +460,19 @@ __device__ __forceinline__ void basic_mono_unit(const DevPlan& P, int t, int k0,
     const double* px = pw + (size_t)mo.x * Kp + k0;
     const double* py = pw + (size_t)(np1 + mo.y) * Kp + k0;
     const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp + k0;
-    const double* fk = f + k0 * kFStride;
+    const double* fk = f + k0 * P.R;
     const int n = k1 - k0;
 #pragma unroll
     for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
     switch (gmax) {
-        case 0: basic_moment_sum<1>(px, py, pz, fk, n, acc); break;
-        case 1: basic_moment_sum<2>(px, py, pz, fk, n, acc); break;
-        case 2: basic_moment_sum<3>(px, py, pz, fk, n, acc); break;
-        case 3: basic_moment_sum<4>(px, py, pz, fk, n, acc); break;
-        case 4: basic_moment_sum<5>(px, py, pz, fk, n, acc); break;
-        case 5: basic_moment_sum<6>(px, py, pz, fk, n, acc); break;
-        case 6: basic_moment_sum<7>(px, py, pz, fk, n, acc); break;
-        default: basic_moment_sum<8>(px, py, pz, fk, n, acc); break;
+        case 0: basic_moment_sum<1>(px, py, pz, fk, P.R, n, acc); break;
+        case 1: basic_moment_sum<2>(px, py, pz, fk, P.R, n, acc); break;
+        case 2: basic_moment_sum<3>(px, py, pz, fk, P.R, n, acc); break;
+        case 3: basic_moment_sum<4>(px, py, pz, fk, P.R, n, acc); break;
+        case 4: basic_moment_sum<5>(px, py, pz, fk, P.R, n, acc); break;
+        case 5: basic_moment_sum<6>(px, py, pz, fk, P.R, n, acc); break;
+        case 6: basic_moment_sum<7>(px, py, pz, fk, P.R, n, acc); break;
+        default: basic_moment_sum<8>(px, py, pz, fk, P.R, n, acc); break;
     }
 }
 
@@ -507,7 +507,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int tid = threadIdx.x;
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
-    const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, P.k_layout, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
+    const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, P.B, P.k_layout, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
                                         A.xi_atom ? P.amc : 0);
     if (tid < kLdsM) lds[tid] = 0.0;  // operand of the padding entries, sink of idle lanes
     double* m = lds + L.m;
@@ -589,7 +589,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
 
     // ---- B: basic moments m[mu, alpha] = sum_k f_mu(k) u_k^alpha.  The
     // power table pw[(q * np1 + p) * Kp + k] = u_{k,q}^p (slots fastest, odd
-    // row stride Kp) and f[k * kFStride + mu] sit in the dm region.  One
+    // row stride Kp) and f[k * R + mu] sit in the dm region.  One
     // thread per monomial; the mu range is the largest of the monomial's
     // wavefront, so the slot loop is a wave-uniform unrolled instance (lanes
     // of lower degree carry unused mu).
@@ -600,7 +600,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         const int c = i / P.R, mu = i - c * P.R;
         double fv, dfv;
         radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
-        f[c * kFStride + mu] = fv;
+        f[i] = fv;
     }
     for (int t = tid; t < nv * 3; t += NT) {
         const int k = t / 3, q = t - 3 * k;
@@ -659,9 +659,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     }
     for (int li = 0; li <= P.n_depth; ++li) contraction_layer<NT, true>(P, li, lds, dm, part, tid, xi);
 
-    // ---- F: per-slot gradient, thread = (slot, part).  m is dead: it holds
-    // the per-part gradients and f, f' recomputed for the valid slots.
-    double* gpart = m;
+    // ---- F: per-slot gradient, thread = (slot, part).  F reads the
+    // adjoints of the basic moments only; the rest of the adjoint region
+    // holds the per-part gradients and f, f' recomputed for the valid slots.
+    double* gpart = lds + L.gp;
     double* gf = lds + L.gf;
     double* gfp = lds + L.gfp;
     for (int i = tid; i < nv * P.R; i += NT) {
@@ -913,7 +914,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
             const int c = i / P.R, mu = i - c * P.R;
             double fv, dfv;
             radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
-            f[c * kFStride + mu] = fv;
+            f[i] = fv;
         }
         for (int t = ht; t < nv * 3; t += NH) {
             const int k = t / 3, q = t - 3 * k;

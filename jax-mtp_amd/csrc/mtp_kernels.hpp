@@ -44,38 +44,40 @@ struct AtomLds {
 };
 
 struct AtomLds2 {
-    int m, dm, part, f, gf, gfp, vx, vy, vz, r, red, xl, ints, total_doubles;
+    int m, dm, part, f, gp, gf, gfp, vx, vy, vz, r, red, xl, ints, total_doubles;
 };
 
 // LDS of the balanced-item kernels (doubles).  Regions are reused across
 // phases:
 //   zero [kLdsM] lds[0] = 0.0: operand of the padding entries of the
 //               contraction streams and the sink of lanes without an item
-//   m   [mlen]  moments [0, n_live) (basic or read as operands); in phase F
-//               the per-part gradients and the recomputed f, f'
-//   dm  [dlen]  A/B: power table (row stride K | 1), then f (row stride
-//               kFStride; odd strides spread a wave's accesses over the
-//               LDS banks); C/D: the values of the
-//               never-operand moments (m[x] for x >= n_live aliases
-//               dm[x - n_live], dead after D) and, at the end, the energy
-//               reduction; E/F: adjoints
+//   m   [n_live] moments [0, n_live) (basic or read as operands)
+//   dm  [dlen]  A/B: power table (row stride K | 1, odd so that a wave's
+//               reads of different powers fall in different LDS banks),
+//               then f[k * R + mu]; C/D: the values of the never-operand
+//               moments (m[x] for x >= n_live aliases dm[x - n_live], dead
+//               after D) and, at the end, the energy reduction; E/F:
+//               adjoints, of which F reads only the basic ones [0, B): from
+//               max(B, 3K) on, F keeps the per-part gradients and the
+//               recomputed f, f'; after F, dm[0, 3K) holds the slot
+//               gradients
 //   part, vx, vy, vz, r, xl (per-row coefficients), ints
-constexpr int kFStride = kMaxR + 1;  // doubles per slot of the radial values in LDS
-
-__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int K, int R, int nu_max, int n_parts,
-                                                    int nt, int max_partials, int xi_len)
+__host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B, int K, int R, int nu_max,
+                                                    int n_parts, int nt, int max_partials, int xi_len)
 {
     AtomLds2 L;
     const int pw_len = (K | 1) * 3 * (nu_max + 1);
-    int mlen = n_live;
-    if (mlen < n_parts * K * 3 + 2 * R * K) mlen = n_parts * K * 3 + 2 * R * K;
+    const int mlen = n_live > 0 ? n_live : 1;
+    const int g0 = B > 3 * K ? B : 3 * K;
     int dlen = amc;
-    if (dlen < pw_len + kFStride * K) dlen = pw_len + kFStride * K;
+    if (dlen < pw_len + R * K) dlen = pw_len + R * K;
     if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
+    if (dlen < g0 + n_parts * K * 3 + 2 * R * K) dlen = g0 + n_parts * K * 3 + 2 * R * K;
     L.m = kLdsM;
     L.dm = L.m + mlen;
     L.f = L.dm + pw_len;
-    L.gf = L.m + n_parts * K * 3;
+    L.gp = L.dm + g0;
+    L.gf = L.gp + n_parts * K * 3;
     L.gfp = L.gf + R * K;
     L.red = L.dm + dlen - nt;
     L.part = L.dm + dlen;
@@ -114,7 +116,7 @@ __host__ __device__ inline AtomLdsPair atom_lds_pair_layout(int amc, int n_live,
     if (L.mlen < n_parts * K * 3 + 2 * R * K) L.mlen = n_parts * K * 3 + 2 * R * K;
     const int pw_len = (K | 1) * 3 * (nu_max + 1);
     L.dlen = amc;
-    if (L.dlen < pw_len + kFStride * K) L.dlen = pw_len + kFStride * K;
+    if (L.dlen < pw_len + R * K) L.dlen = pw_len + R * K;
     if (L.dlen < (amc - n_live) + nt / 2) L.dlen = (amc - n_live) + nt / 2;
     L.m = kPairM;
     L.dm = L.m + L.mlen;
@@ -151,8 +153,8 @@ constexpr int variant_parts(int v)
 }
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
-size_t atom_lds2_bytes(int amc, int n_live, int nslots, int R, int nu_max, int n_parts, int nt, int max_partials,
-                       int xi_len = 0);
+size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
+                       int max_partials, int xi_len = 0);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.

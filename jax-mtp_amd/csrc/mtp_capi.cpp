@@ -74,6 +74,7 @@ struct mtp_ctx {
     double* h_out = nullptr;
     // training buffers (allocated at the first training call)
     size_t train_lds = 0;
+    DevPlan tstreams{};                   // contraction streams of the training kernel (stream fields only)
     double *d_gxi = nullptr, *d_gc = nullptr, *d_part = nullptr, *d_xisum = nullptr, *d_tsum = nullptr;
     int lds_max = 0;
     // live per-kernel timing (events recorded around each atom-kernel launch)
@@ -120,6 +121,38 @@ hipError_t upload(T* d, const std::vector<T>& h, hipStream_t s)
 {
     if (h.empty()) return hipSuccess;
     return hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+// Uploads the forward and reverse contraction streams of ip into the
+// stream fields of dv (device buffers owned by c).
+void upload_streams(mtp_ctx* c, const ItemPlan& ip, DevPlan& dv, hipError_t& e)
+{
+    auto pack = [&](const StreamPlan& sp, int* cb_beg, int* bo_beg, int* xa_beg, const I4** d_wave,
+                    const U4** d_tbl, const I4** d_comb, const int** d_bo, const int** d_xa) {
+        for (int q = 0; q < kMaxDepth + 2; ++q) {
+            cb_beg[q] = sp.cb_beg[q];
+            if (bo_beg) bo_beg[q] = sp.bo_beg[q];
+            if (xa_beg) xa_beg[q] = sp.xa_beg[q];
+        }
+        I4* dw = c->alloc<I4>(std::max<size_t>(1, sp.wave.size()), e);
+        U4* dt = c->alloc<U4>(std::max<size_t>(1, sp.tbl.size()), e);
+        I4* dc = c->alloc<I4>(std::max<size_t>(1, sp.comb.size()), e);
+        int* db = c->alloc<int>(std::max<size_t>(1, sp.bo.size()), e);
+        int* dx = c->alloc<int>(std::max<size_t>(1, sp.xa.size()), e);
+        if (e == hipSuccess) e = upload(dw, sp.wave, c->stream);
+        if (e == hipSuccess) e = upload(dt, sp.tbl, c->stream);
+        if (e == hipSuccess) e = upload(dc, sp.comb, c->stream);
+        if (e == hipSuccess) e = upload(db, sp.bo, c->stream);
+        if (e == hipSuccess) e = upload(dx, sp.xa, c->stream);
+        *d_wave = dw; *d_tbl = dt; *d_comb = dc;
+        if (d_bo) *d_bo = db;
+        if (d_xa) *d_xa = dx;
+    };
+    pack(ip.fwd, dv.fwd_cb_beg, nullptr, nullptr, &dv.fwd_wave, &dv.fwd_tbl, &dv.fwd_comb, nullptr, nullptr);
+    pack(ip.rev, dv.rev_cb_beg, dv.rev_bo_beg, dv.rev_xa_beg, &dv.rev_wave, &dv.rev_tbl, &dv.rev_comb, &dv.rev_bo,
+         &dv.rev_xa);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the host tables die with ip
+    dv.max_partials = ip.max_partials;
 }
 
 int upload_params(mtp_ctx* c)
@@ -310,32 +343,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             ItemPlan ip;
             std::string ierr;
             if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) continue;  // variant unavailable
-            auto pack = [&](const StreamPlan& sp, int* cb_beg, int* bo_beg, int* xa_beg, const I4** d_wave,
-                            const U4** d_tbl, const I4** d_comb, const int** d_bo, const int** d_xa) {
-                for (int q = 0; q < kMaxDepth + 2; ++q) {
-                    cb_beg[q] = sp.cb_beg[q];
-                    if (bo_beg) bo_beg[q] = sp.bo_beg[q];
-                    if (xa_beg) xa_beg[q] = sp.xa_beg[q];
-                }
-                I4* dw = c->alloc<I4>(std::max<size_t>(1, sp.wave.size()), e);
-                U4* dt = c->alloc<U4>(std::max<size_t>(1, sp.tbl.size()), e);
-                I4* dc = c->alloc<I4>(std::max<size_t>(1, sp.comb.size()), e);
-                int* db = c->alloc<int>(std::max<size_t>(1, sp.bo.size()), e);
-                int* dx = c->alloc<int>(std::max<size_t>(1, sp.xa.size()), e);
-                if (e == hipSuccess) e = upload(dw, sp.wave, c->stream);
-                if (e == hipSuccess) e = upload(dt, sp.tbl, c->stream);
-                if (e == hipSuccess) e = upload(dc, sp.comb, c->stream);
-                if (e == hipSuccess) e = upload(db, sp.bo, c->stream);
-                if (e == hipSuccess) e = upload(dx, sp.xa, c->stream);
-                *d_wave = dw; *d_tbl = dt; *d_comb = dc;
-                if (d_bo) *d_bo = db;
-                if (d_xa) *d_xa = dx;
-            };
-            pack(ip.fwd, dv.fwd_cb_beg, nullptr, nullptr, &dv.fwd_wave, &dv.fwd_tbl, &dv.fwd_comb, nullptr, nullptr);
-            pack(ip.rev, dv.rev_cb_beg, dv.rev_bo_beg, dv.rev_xa_beg, &dv.rev_wave, &dv.rev_tbl, &dv.rev_comb,
-                 &dv.rev_bo, &dv.rev_xa);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host tables die with this iteration
-            dv.max_partials = ip.max_partials;
+            upload_streams(c, ip, dv, e);
             dv.k_layout = max_nbr;
             const size_t lds = paired ? sizeof(double) * (size_t)atom_lds_pair_layout(
                                                 f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
@@ -758,12 +766,38 @@ int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const i
     const HostPlan& p = ctx->plan;
     const MtpFile& f = p.file;
     if (!ctx->d_gxi) {
-        ctx->train_lds = train_lds_bytes(f.alpha_moments_count, ctx->max_nbr, f.radial_funcs_count,
-                                         f.radial_basis_size, p.nu_max, p.n_parts);
+        // contraction streams of the training kernel: 16-byte (value,
+        // tangent) slots; the partial sums get what leaves room for three
+        // workgroups per CU, else two, else one
+        const int nt = kTrainThreads;
+        const TrainLds L0 = train_lds_layout(f.alpha_moments_count, p.n_live, f.B(), ctx->max_nbr,
+                                             f.radial_funcs_count, f.radial_basis_size, p.nu_max, p.n_parts, nt, 0);
+        StreamLayout lay;
+        lay.unit = 16;
+        lay.m_off = L0.m;
+        lay.dm_off = L0.dm;
+        lay.part_off = L0.part;
+        lay.absolute = true;
+        int budget = 64;
+        for (int occ = 3; occ >= 1; --occ) {
+            const int b = ((int)(ctx->lds_max / sizeof(double)) / occ - L0.total) / 2;
+            if (b >= 64) { budget = b; break; }
+        }
+        ItemPlan ip;
+        std::string ierr;
+        if (!build_items(p, nt, lay, budget, ip, ierr))
+            return ctx->fail(MTP_ERR_UNSUPPORTED, "training contraction streams: " + ierr);
+        ctx->train_lds = sizeof(double) * (size_t)train_lds_layout(f.alpha_moments_count, p.n_live, f.B(),
+                                                                   ctx->max_nbr, f.radial_funcs_count,
+                                                                   f.radial_basis_size, p.nu_max, p.n_parts, nt,
+                                                                   ip.max_partials).total;
         if ((long long)ctx->train_lds > (long long)ctx->lds_max)
             return ctx->fail(MTP_ERR_UNSUPPORTED, "the parameter gradient needs " + std::to_string(ctx->train_lds) +
                                                       " bytes of LDS per atom at this level; the device has " +
                                                       std::to_string(ctx->lds_max));
+        hipError_t eu = hipSuccess;
+        upload_streams(ctx, ip, ctx->tstreams, eu);
+        if (eu != hipSuccess) return ctx->hip_fail(eu, "training streams upload");
         hipError_t e = set_train_kernel_lds(ctx->train_lds);
         const size_t A = (size_t)ctx->max_atoms;
         const size_t ncols = 1 + (size_t)f.species_count * f.radial_funcs_count * f.radial_basis_size;
@@ -782,7 +816,20 @@ int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const i
     A.volumes = d_volumes; A.gE = d_gE; A.gF = d_gF; A.gS = d_gS;
     A.gxi_part = ctx->d_gxi; A.gc_part = ctx->d_gc;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    hipError_t e = launch_train_grad(ctx->dev, A, ctx->train_lds, ctx->d_part, ctx->d_xisum, ctx->d_tsum, d_grad, s);
+    DevPlan tp = ctx->dev;  // parameters and tables of the potential, streams of the training kernel
+    const DevPlan& ts = ctx->tstreams;
+    tp.fwd_wave = ts.fwd_wave; tp.fwd_tbl = ts.fwd_tbl; tp.fwd_comb = ts.fwd_comb;
+    tp.rev_wave = ts.rev_wave; tp.rev_tbl = ts.rev_tbl; tp.rev_comb = ts.rev_comb;
+    tp.rev_bo = ts.rev_bo; tp.rev_xa = ts.rev_xa;
+    for (int q = 0; q < kMaxDepth + 2; ++q) {
+        tp.fwd_cb_beg[q] = ts.fwd_cb_beg[q];
+        tp.rev_cb_beg[q] = ts.rev_cb_beg[q];
+        tp.rev_bo_beg[q] = ts.rev_bo_beg[q];
+        tp.rev_xa_beg[q] = ts.rev_xa_beg[q];
+    }
+    tp.max_partials = ts.max_partials;
+    tp.k_layout = ctx->max_nbr;
+    hipError_t e = launch_train_grad(tp, A, ctx->train_lds, ctx->d_part, ctx->d_xisum, ctx->d_tsum, d_grad, s);
     if (e != hipSuccess) return ctx->hip_fail(e, "parameter-gradient launch");
     return MTP_OK;
 }

@@ -27,69 +27,289 @@
 
 namespace mtp {
 
-constexpr int kTB = 256;
-
-__host__ __device__ inline TrainLds train_lds_layout(int amc, int K, int R, int N, int nu_max, int n_parts)
+// ---------------------------------------------------------------------------
+// Dual contraction streams.  Each entry of the atom kernels' streams
+// (operand slots a, b and an FP32 multiplier w, two entries per lane and
+// step, group ends flagged in the step's fourth word) reads two 16-byte
+// pairs.  Forward: a = (m_a, mt_a), b = (m_b, mt_b) and the sum gains
+// w (m_a m_b, mt_a m_b + m_a mt_b).  Reverse: a = (m_o, mt_o) of the operand,
+// b = (dm_t, dmt_t) of the target it feeds, and the sum gains
+// w (m_o dm_t + mt_o dmt_t, m_o dmt_t).
+// ---------------------------------------------------------------------------
+template <bool REV>
+__device__ __forceinline__ void dual_steps(const uint4* q, char* lb, double2& acc)
 {
-    TrainLds L;
-    int mlen = amc;
-    if (mlen < 3 * n_parts * K * R) mlen = 3 * n_parts * K * R;
-    int dlen = amc;
-    if (dlen < K * 3 * (nu_max + 1)) dlen = K * 3 * (nu_max + 1);
-    L.m = 0;
-    L.mt = L.m + mlen;
-    L.dm = L.mt + amc;
-    L.dmt = L.dm + dlen;
-    L.f = L.dmt + amc;
-    L.fp = L.f + R * K;
-    L.vx = L.fp + R * K;
-    L.vy = L.vx + K;
-    L.vz = L.vy + K;
-    L.r = L.vz + K;
-    L.dx = L.r + K;
-    L.dy = L.dx + K;
-    L.dz = L.dy + K;
-    L.ud = L.dz + K;
-    L.a = L.ud + K;
-    L.b = L.a + N * K;
-    L.red = L.b + N * K;
-    L.ints = L.red + kTB;
-    L.total = L.ints + (2 * K + 8) / 2 + 1;
-    return L;
+    constexpr int U = kStreamUnroll;
+    double2 a0[U], b0[U], a1[U], b1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        a0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x & 0xffffu) << 4));
+        b0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x >> 16) << 4));
+        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
+        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        {
+            const double w = (double)__uint_as_float(q[u].y);
+            const double wa = w * a0[u].x;
+            if (REV) {
+                acc.x = fma(wa, b0[u].x, fma(w * a0[u].y, b0[u].y, acc.x));
+                acc.y = fma(wa, b0[u].y, acc.y);
+            } else {
+                acc.x = fma(wa, b0[u].x, acc.x);
+                acc.y = fma(wa, b0[u].y, fma(w * a0[u].y, b0[u].x, acc.y));
+            }
+        }
+        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
+            const double w = (double)__uint_as_float(q[u].w);
+            const double wa = w * a1[u].x;
+            if (REV) {
+                acc.x = fma(wa, b1[u].x, fma(w * a1[u].y, b1[u].y, acc.x));
+                acc.y = fma(wa, b1[u].y, acc.y);
+            } else {
+                acc.x = fma(wa, b1[u].x, acc.x);
+                acc.y = fma(wa, b1[u].y, fma(w * a1[u].y, b1[u].x, acc.y));
+            }
+        } else {  // group end (wave-uniform): q.z is this lane's destination
+            *reinterpret_cast<double2*>(lb + q[u].z) = acc;
+            acc = make_double2(0.0, 0.0);
+        }
+    }
 }
 
-size_t train_lds_bytes(int amc, int K, int R, int N, int nu_max, int n_parts)
+// One dual contraction layer (forward: m2 targets; reverse: dm2 targets,
+// seeded with (alpha xi, xi) for the mapped moments).
+template <int NT, bool REV>
+__device__ __forceinline__ void dual_layer(const DevPlan& P, int L, double* lds, double2* m2, double2* dm2,
+                                           const double2* part, int tid, double alpha)
 {
-    return sizeof(double) * (size_t)train_lds_layout(amc, K, R, N, nu_max, n_parts).total;
+    constexpr int NW = NT / kWave;
+    constexpr int U = kStreamUnroll;
+    const int lane = tid & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
+    const int nsteps = wv.y;
+    if (nsteps > 0) {
+        const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
+        char* lb = reinterpret_cast<char*>(lds);
+        double2 acc = make_double2(0.0, 0.0);
+        uint4 qa[U], qb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)u * kWave];
+        for (int s0 = 0;;) {
+            if (s0 + U < nsteps) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) qb[u] = tb[(size_t)(s0 + U + u) * kWave];
+            }
+            dual_steps<REV>(qa, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
+            if (s0 + U < nsteps) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)(s0 + U + u) * kWave];
+            }
+            dual_steps<REV>(qb, lb, acc);
+            s0 += U;
+            if (s0 >= nsteps) break;
+        }
+    }
+    __syncthreads();
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    double2* dst = REV ? dm2 : m2;
+    for (int c = c0 + tid; c < c1; c += NT) {
+        const I4 cb = comb[c];
+        const double x0 = REV ? P.xi_of[cb.x] : 0.0;
+        double2 acc = make_double2(alpha * x0, x0);
+        for (int p = cb.y; p < cb.z; ++p) {
+            const double2 v = part[p];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        dst[cb.x] = acc;
+    }
+    if (REV) {
+        for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
+            const int x = P.rev_bo[t];
+            const double xi = P.xi_of[x];
+            dm2[x] = make_double2(alpha * xi, xi);
+        }
+        for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
+            const int x = P.rev_xa[t];
+            const double xi = P.xi_of[x];
+            double2 v = dm2[x];
+            v.x += alpha * xi;
+            v.y += xi;
+            dm2[x] = v;
+        }
+    }
+    if (c1 > c0 || REV) __syncthreads();
 }
 
-__global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A)
+// Basic moment of monomial (a, b, c) and its tangent over slots [0, n), for
+// mu < NMU: acc[mu] = sum_k f_mu(k) P_k, acct[mu] = sum_k fu_mu(k) P_k +
+// f_mu(k) G_k with P = u_x^a u_y^b u_z^c and G = grad_u P . e_k (tables
+// px.. of the powers, ex.. of p u^(p-1) e; f and fu at stride R).  Slots go
+// in batches whose LDS reads issue before their arithmetic.
+template <int NMU>
+__device__ __forceinline__ void dual_moment_sum(const double* px, const double* py, const double* pz,
+                                                const double* ex, const double* ey, const double* ez,
+                                                const double* f, const double* fu, int R, int n, double* acc,
+                                                double* acct)
 {
+    constexpr int UB = NMU <= 2 ? 4 : 2;
+    int k = 0;
+    for (; k + UB <= n; k += UB) {
+        double a[UB], b[UB], c[UB], da[UB], db[UB], dc[UB], fr[UB][NMU], fur[UB][NMU];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            a[u] = px[k + u]; b[u] = py[k + u]; c[u] = pz[k + u];
+            da[u] = ex[k + u]; db[u] = ey[k + u]; dc[u] = ez[k + u];
+#pragma unroll
+            for (int mu = 0; mu < NMU; ++mu) { fr[u][mu] = f[(k + u) * R + mu]; fur[u][mu] = fu[(k + u) * R + mu]; }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const double bc = b[u] * c[u];
+            const double Pm = a[u] * bc;
+            const double G = fma(da[u], bc, a[u] * fma(db[u], c[u], b[u] * dc[u]));
+#pragma unroll
+            for (int mu = 0; mu < NMU; ++mu) {
+                acc[mu] = fma(fr[u][mu], Pm, acc[mu]);
+                acct[mu] = fma(fur[u][mu], Pm, fma(fr[u][mu], G, acct[mu]));
+            }
+        }
+    }
+    for (; k < n; ++k) {
+        const double bc = py[k] * pz[k];
+        const double Pm = px[k] * bc;
+        const double G = fma(ex[k], bc, px[k] * fma(ey[k], pz[k], py[k] * ez[k]));
+#pragma unroll
+        for (int mu = 0; mu < NMU; ++mu) {
+            acc[mu] = fma(f[k * R + mu], Pm, acc[mu]);
+            acct[mu] = fma(fu[k * R + mu], Pm, fma(f[k * R + mu], G, acct[mu]));
+        }
+    }
+}
+
+// H = sum_alpha d_alpha grad_u u^alpha and T = the same over dt, for one
+// (mu, NU) block of (dm, dmt) pairs (u_z^c carried as a running product).
+template <int NU>
+__device__ __forceinline__ void dual_grad_block(const double2* __restrict__ d, const double* px, const double* py,
+                                                double uz, double* H, double* T)
+{
+#pragma unroll
+    for (int a = NU - 1; a >= 0; --a) {
+        double pzc = 1.0;
+#pragma unroll
+        for (int b = NU - 1 - a; b >= 0; --b) {
+            const int c = NU - 1 - a - b;
+            const double P = px[a] * py[b] * pzc;
+            const double2 dx = d[mono_idx(NU, a + 1, b)];
+            const double2 dy = d[mono_idx(NU, a, b + 1)];
+            const double2 dz = d[mono_idx(NU, a, b)];
+            const double Pa = (double)(a + 1) * P, Pb = (double)(b + 1) * P, Pc = (double)(c + 1) * P;
+            H[0] = fma(dx.x, Pa, H[0]); T[0] = fma(dx.y, Pa, T[0]);
+            H[1] = fma(dy.x, Pb, H[1]); T[1] = fma(dy.y, Pb, T[1]);
+            H[2] = fma(dz.x, Pc, H[2]); T[2] = fma(dz.y, Pc, T[2]);
+            pzc *= uz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int PN>
+__device__ __forceinline__ void dual_grad_dispatch(int nu, const double2* d, double* px, double* py, double uz,
+                                                   double* H, double* T)
+{
+    opaque_powers_xy<PN>(px, py);
+    switch (nu) {
+        case 1: dual_grad_block<1>(d, px, py, uz, H, T); break;
+        case 2: if constexpr (PN >= 2) dual_grad_block<2>(d, px, py, uz, H, T); break;
+        case 3: if constexpr (PN >= 3) dual_grad_block<3>(d, px, py, uz, H, T); break;
+        case 4: if constexpr (PN >= 4) dual_grad_block<4>(d, px, py, uz, H, T); break;
+        case 5: if constexpr (PN >= 5) dual_grad_block<5>(d, px, py, uz, H, T); break;
+        case 6: if constexpr (PN >= 6) dual_grad_block<6>(d, px, py, uz, H, T); break;
+        case 7: if constexpr (PN >= 7) dual_grad_block<7>(d, px, py, uz, H, T); break;
+        case 8: if constexpr (PN >= 8) dual_grad_block<8>(d, px, py, uz, H, T); break;
+        case 9: if constexpr (PN >= 9) dual_grad_block<9>(d, px, py, uz, H, T); break;
+        case 10: if constexpr (PN >= 10) dual_grad_block<10>(d, px, py, uz, H, T); break;
+        case 11: if constexpr (PN >= 11) dual_grad_block<11>(d, px, py, uz, H, T); break;
+        case 12: if constexpr (PN >= 12) dual_grad_block<12>(d, px, py, uz, H, T); break;
+        default: break;
+    }
+}
+
+// Slot k, part prt: over the part's (mu, nu) blocks, A = sum dm P
+// (d J / d f_mu), Ap = sum dmt P (d J / d f'_mu, times u . Delta) and
+// Q = sum dmt grad_r P . Delta (d J / d f_mu through the tangent), with
+// grad_r P . Delta = T . e for e = (Delta - (u . Delta) u) / r.  Adds
+// A + Q to acc[mu] and Ap to acc[R + mu] (the factor u . Delta is in cb).
+template <int PN>
+__device__ __forceinline__ void dual_slot(const DevPlan& P, int prt, double ux, double uy, double uz, double e0,
+                                          double e1, double e2, const double2* dm2, double* acc)
+{
+    double px[PN], py[PN];
+    px[0] = py[0] = 1.0;
+#pragma unroll
+    for (int p = 1; p < PN; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; }
+    const int R = P.R;
+    const int b1 = P.part_beg[prt + 1];
+    for (int bi = P.part_beg[prt]; bi < b1; ++bi) {
+        const I2 bl = P.part_blk[bi];
+        const int mu = bl.x & 0xff, nu = bl.y;
+        const double2* d = dm2 + (bl.x >> 8);
+        double a0, a1, q;
+        if (nu == 0) {
+            const double2 v = d[0];
+            a0 = v.x; a1 = v.y; q = 0.0;
+        } else {
+            double H[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+            dual_grad_dispatch<PN>(nu, d, px, py, uz, H, T);
+            const double inu = 1.0 / (double)nu;
+            a0 = (ux * H[0] + uy * H[1] + uz * H[2]) * inu;
+            a1 = (ux * T[0] + uy * T[1] + uz * T[2]) * inu;
+            q = T[0] * e0 + T[1] * e1 + T[2] * e2;
+        }
+        acc[mu] += a0 + q;
+        acc[R + mu] += a1;
+    }
+}
+
+// One row (atom) per workgroup.
+__global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3)))
+train_kernel(const DevPlan P, TrainArgs A)
+{
+    constexpr int NT = kTrainThreads;
+    constexpr int NW = NT / kWave;
     extern __shared__ double lds[];
     const int row = blockIdx.x;
     const int tid = threadIdx.x;
-    const int K = A.nslots, R = P.R, N = P.N;
-    const TrainLds L = train_lds_layout(P.amc, K, R, N, P.nu_max, P.n_parts);
-    double* m = lds + L.m;
-    double* mt = lds + L.mt;
-    double* dm = lds + L.dm;
-    double* dmt = lds + L.dmt;
-    double* f = lds + L.f;
-    double* fp = lds + L.fp;
+    const int lane = tid & (kWave - 1);
+    const int K = A.nslots, KL = P.k_layout, R = P.R, N = P.N;
+    const TrainLds L = train_lds_layout(P.amc, P.n_live, P.B, KL, R, N, P.nu_max, P.n_parts, NT, P.max_partials);
+    double2* l2 = reinterpret_cast<double2*>(lds);
+    double2* m2 = l2 + L.m;
+    double2* dm2 = l2 + L.dm;
+    const double2* part2 = l2 + L.part;
     double* vx = lds + L.vx;
     double* vy = lds + L.vy;
     double* vz = lds + L.vz;
     double* rr = lds + L.r;
-    double* Dx = lds + L.dx;
-    double* Dy = lds + L.dy;
-    double* Dz = lds + L.dz;
+    double* ex = lds + L.ex;
+    double* ey = lds + L.ey;
+    double* ez = lds + L.ez;
     double* ud = lds + L.ud;
-    double* ca = lds + L.a;
-    double* cb = lds + L.b;
+    double* ca = lds + L.ca;
+    double* cb = lds + L.cb;
     int* ints = reinterpret_cast<int*>(lds + L.ints);
-    int* jtc = ints;              // [K] neighbor type of compact slot
-    int* wave_cnt = ints + K;     // [4]
-    int* nvp = wave_cnt + 4;
+    int* jtc = ints;              // [KL] neighbor type of compact slot
+    int* wave_cnt = ints + KL;    // [NW]
+    int* nvp = wave_cnt + NW;
+    if (tid == 0) l2[0] = make_double2(0.0, 0.0);
 
     const int ti = A.itypes[row];
     const int conf = A.conf_of[row];
@@ -104,7 +324,9 @@ __global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A
     const double Lyz = gs[3] * iv, Lxz = gs[4] * iv, Lxy = gs[5] * iv;
     const double gFx = A.gF[3 * (size_t)row], gFy = A.gF[3 * (size_t)row + 1], gFz = A.gF[3 * (size_t)row + 2];
 
-    // ---- A: radial basis, coefficient factors, perturbation Delta per slot
+    // ---- A: compaction of the valid slots; per slot the perturbation
+    // Delta = gF_i - gF_j + Lambda r, u . Delta and e = (Delta - (u . Delta) u) / r
+    int nv;
     {
         const int k = tid;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
@@ -115,19 +337,13 @@ __global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A
             valid = (jt_row[k] >= 0) && (r < P.max_dist);
         }
         const uint64_t mask = __ballot(valid);
-        const int lane = tid & (kWave - 1), wave = tid / kWave;
+        const int wave = tid / kWave;
         if (lane == 0) wave_cnt[wave] = __popcll(mask);
         __syncthreads();
         int off = 0;
         for (int w = 0; w < wave; ++w) off += wave_cnt[w];
         const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
         if (valid) {
-            double fv[kMaxR], dfv[kMaxR];
-            radial_slot_coef(P, r, ti, jt_row[k], fv, dfv, ca + pos, cb + pos, K);
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
-            vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
             jtc[pos] = jt_row[k];
             const int j = js_row[k];
             double dx = gFx, dy = gFy, dz = gFz;
@@ -137,176 +353,197 @@ __global__ void __launch_bounds__(kTB) train_kernel(const DevPlan P, TrainArgs A
             dx += Lxx * v0 + Lxy * v1 + Lxz * v2;
             dy += Lxy * v0 + Lyy * v1 + Lyz * v2;
             dz += Lxz * v0 + Lyz * v1 + Lzz * v2;
-            Dx[pos] = dx; Dy[pos] = dy; Dz[pos] = dz;
-            ud[pos] = (v0 * dx + v1 * dy + v2 * dz) / r;
+            const double ir = 1.0 / r;
+            const double ux = v0 * ir, uy = v1 * ir, uz = v2 * ir;
+            const double udv = ux * dx + uy * dy + uz * dz;
+            vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
+            ud[pos] = udv;
+            ex[pos] = (dx - udv * ux) * ir;
+            ey[pos] = (dy - udv * uy) * ir;
+            ez[pos] = (dz - udv * uz) * ir;
         }
         if (tid == 0) {
             int tot = 0;
-            for (int w = 0; w < kTB / kWave; ++w) tot += wave_cnt[w];
+            for (int w = 0; w < NW; ++w) tot += wave_cnt[w];
             *nvp = tot;
         }
         __syncthreads();
+        nv = __builtin_amdgcn_readfirstlane(*nvp);
     }
-    const int nv = *nvp;
-    for (int x = P.B + tid; x < P.amc; x += kTB) { m[x] = 0.0; mt[x] = 0.0; }
 
-    // ---- B: basic moments and their tangents along Delta
+    // radial values f and fu = f' (u . Delta) per (slot, mu); coefficient
+    // factors ca[n][k] = d f / d c_n, cb[n][k] = (d f' / d c_n) (u . Delta)
+    // per (slot, n); power tables pw[(q np1 + p) Kp + k] = u_q^p and
+    // pe[..] = p u_q^(p-1) e_q per (slot, component)
     const int np1 = P.nu_max + 1;
-    double* pw = dm;
-    for (int t = tid; t < nv * 3; t += kTB) {
+    const int Kp = KL | 1;
+    double* pw = lds + L.pw;
+    double* pe = lds + L.pe;
+    double* f = lds + L.f;
+    double* fu = lds + L.fu;
+    for (int i = tid; i < nv * R; i += NT) {
+        const int c = i / R, mu = i - c * R;
+        double fv, dfv;
+        radial_one<double>(P, rr[c], ti, jtc[c], mu, fv, dfv);
+        f[i] = fv;
+        fu[i] = dfv * ud[c];
+    }
+    {
+        const double rmin = P.min_dist, rmax = P.max_dist;
+        const double range_inv = 1.0 / (rmax - rmin);
+        const double dxdr = 2.0 * range_inv;
+        const double sc = P.scaling;
+        for (int i = tid; i < nv * N; i += NT) {
+            const int c = i / N, n = i - c * N;
+            const double r = rr[c];
+            const double x = (2.0 * r - (rmin + rmax)) * range_inv;
+            const double s = (rmax - r) * (rmax - r);
+            const double ds = -2.0 * (rmax - r);
+            double t0 = 1.0, d0 = 0.0;
+            if (n >= 1) {
+                double t2 = 1.0, t1 = x, d2 = 0.0, d1 = 1.0;
+                t0 = x; d0 = 1.0;
+                for (int q = 2; q <= n; ++q) {
+                    t0 = 2.0 * x * t1 - t2;
+                    d0 = 2.0 * t1 + 2.0 * x * d1 - d2;
+                    t2 = t1; t1 = t0; d2 = d1; d1 = d0;
+                }
+            }
+            ca[n * KL + c] = sc * s * t0;
+            cb[n * KL + c] = sc * (ds * t0 + s * d0 * dxdr) * ud[c];
+        }
+    }
+    for (int t = tid; t < nv * 3; t += NT) {
         const int k = t / 3, q = t - 3 * k;
         const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) / rr[k];
-        double* row_pw = pw + (size_t)t * np1;
+        const double e = q == 0 ? ex[k] : (q == 1 ? ey[k] : ez[k]);
+        double* col = pw + (size_t)q * np1 * Kp + k;
+        double* dcol = pe + (size_t)q * np1 * Kp + k;
         double v = 1.0;
-        row_pw[0] = 1.0;
-        for (int p = 1; p < np1; ++p) { v *= u; row_pw[p] = v; }
-    }
-    __syncthreads();
-    for (int t = tid; t < P.n_mono; t += kTB) {
-        const I4 mo = P.mono[t];
-        const int nu = mo.w & 0xff, alpha_i = (mo.w >> 8) & 0xffff;
-        const int mumax = P.mu_max_of_nu[nu];
-        double acc[kMaxR], acct[kMaxR];
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu) { acc[mu] = 0.0; acct[mu] = 0.0; }
-        for (int k = 0; k < nv; ++k) {
-            const double* pk = pw + (size_t)k * 3 * np1;
-            const double px = pk[mo.x], py = pk[np1 + mo.y], pz = pk[2 * np1 + mo.z];
-            const double Pm = px * py * pz;
-            // grad_u P . Delta
-            double gd = 0.0;
-            if (mo.x > 0) gd += mo.x * pk[mo.x - 1] * py * pz * Dx[k];
-            if (mo.y > 0) gd += mo.y * px * pk[np1 + mo.y - 1] * pz * Dy[k];
-            if (mo.z > 0) gd += mo.z * px * py * pk[2 * np1 + mo.z - 1] * Dz[k];
-            const double ir = 1.0 / rr[k];
-            const double tP = ud[k] * Pm;                      // (u . Delta) P
-            const double tG = (gd - (double)nu * tP) * ir;     // grad_r P . Delta
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu <= mumax) {
-                    acc[mu] += f[mu * K + k] * Pm;
-                    acct[mu] += fp[mu * K + k] * tP + f[mu * K + k] * tG;
-                }
+        col[0] = 1.0;
+        dcol[0] = 0.0;
+        for (int p = 1; p < np1; ++p) {
+            dcol[(size_t)p * Kp] = (double)p * v * e;
+            v *= u;
+            col[(size_t)p * Kp] = v;
         }
-#pragma unroll
-        for (int mu = 0; mu < kMaxR; ++mu)
-            if (mu <= mumax) {
-                m[P.blk_off[mu][nu] + alpha_i] = acc[mu];
-                mt[P.blk_off[mu][nu] + alpha_i] = acct[mu];
-            }
     }
     __syncthreads();
 
-    // ---- C: contraction layers for m and mt
-    for (int d = 1; d <= P.n_depth; ++d) {
-        const int t0 = P.fwd_layer[d], t1 = P.fwd_layer[d + 1];
-        for (int t = t0 + tid; t < t1; t += kTB) {
-            double s = 0.0, st = 0.0;
-            const int e1 = P.fwd_beg[t + 1];
-            for (int e = P.fwd_beg[t]; e < e1; ++e) {
-                const uint32_t pr = P.fwd_pair[e];
-                const int i1 = pr & 0xffffu, i2 = pr >> 16;
-                const double w = P.fwd_mult[e];
-                s += w * m[i1] * m[i2];
-                st += w * (mt[i1] * m[i2] + m[i1] * mt[i2]);
-            }
-            m[P.fwd_tgt[t]] = s;
-            mt[P.fwd_tgt[t]] = st;
-        }
-        __syncthreads();
-    }
-
-    // ---- D: dJ/dxi_s partial for this atom
-    for (int s = tid; s < P.S; s += kTB) {
-        const int x = P.mapping[s];
-        A.gxi_part[(size_t)row * P.S + s] = alpha * m[x] + mt[x];
-    }
-
-    // ---- E: reverse over (m, mt): dmt = dJ/dmt, dm = dJ/dm
-    for (int li = 0; li <= P.n_depth; ++li) {
-        const int t0 = P.rev_layer[li], t1 = P.rev_layer[li + 1];
-        for (int t = t0 + tid; t < t1; t += kTB) {
-            const int x = P.rev_x[t];
-            const double xi = P.xi_of[x];
-            double s = alpha * xi, st = xi;
-            const int e1 = P.rev_beg[t + 1];
-            for (int e = P.rev_beg[t]; e < e1; ++e) {
-                const uint32_t pr = P.rev_pair[e];
-                const int o = pr & 0xffffu, tg = pr >> 16;
-                const double w = P.rev_mult[e];
-                s += w * (m[o] * dm[tg] + mt[o] * dmt[tg]);
-                st += w * m[o] * dmt[tg];
-            }
-            dm[x] = s;
-            dmt[x] = st;
-        }
-        __syncthreads();
-    }
-
-    // ---- F: per slot, per mu: A = sum dm P, Ap = sum dmt P, Q = sum dmt grad_r P . Delta
-    double* accs = m;  // [part][slot][3][R]
+    // ---- B: basic moments and their tangents, one thread per monomial
+    // (all its mu; the mu range is the largest of the monomial's wavefront)
     {
-        const int kk = tid & (kWave - 1), part = tid / kWave;
-        for (int k = kk; k < nv && part < P.n_parts; k += kWave) {
+        const int ngrp = (P.n_mono + kWave - 1) / kWave;
+        for (int g = tid / kWave; g < ngrp; g += NW) {
+            const int t = g * kWave + lane;
+            if (t < P.n_mono) {
+                const I4 mo = P.mono[t];
+                const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
+                const double* px = pw + (size_t)mo.x * Kp;
+                const double* py = pw + (size_t)(np1 + mo.y) * Kp;
+                const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp;
+                const double* qx = pe + (size_t)mo.x * Kp;
+                const double* qy = pe + (size_t)(np1 + mo.y) * Kp;
+                const double* qz = pe + (size_t)(2 * np1 + mo.z) * Kp;
+                double acc[kMaxR], acct[kMaxR];
+#pragma unroll
+                for (int mu = 0; mu < kMaxR; ++mu) { acc[mu] = 0.0; acct[mu] = 0.0; }
+                switch (gmax) {
+                    case 0: dual_moment_sum<1>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 1: dual_moment_sum<2>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 2: dual_moment_sum<3>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 3: dual_moment_sum<4>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 4: dual_moment_sum<5>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 5: dual_moment_sum<6>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    case 6: dual_moment_sum<7>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                    default: dual_moment_sum<8>(px, py, pz, qx, qy, qz, f, fu, R, nv, acc, acct); break;
+                }
+                const I4 d0 = P.mono_dst[2 * t], d1 = P.mono_dst[2 * t + 1];
+                const int dst[kMaxR] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+                for (int mu = 0; mu < kMaxR; ++mu)
+                    if (dst[mu] >= 0) m2[dst[mu]] = make_double2(acc[mu], acct[mu]);
+            }
+        }
+    }
+    __syncthreads();
+    // moments that are never targets stay 0 (m2[x >= n_live] lies in the
+    // adjoint region, free once the tables are consumed)
+    for (int x = P.B + tid; x < P.amc; x += NT) m2[x] = make_double2(0.0, 0.0);
+    __syncthreads();
+
+    // ---- C: contraction layers for (m, mt)
+    for (int d = 0; d < P.n_depth; ++d) dual_layer<NT, false>(P, d, lds, m2, dm2, part2, tid, alpha);
+
+    // ---- D: dJ/dxi_s of this atom
+    for (int s = tid; s < P.S; s += NT) {
+        const double2 v = m2[P.mapping[s]];
+        A.gxi_part[(size_t)row * P.S + s] = alpha * v.x + v.y;
+    }
+    __syncthreads();  // never-operand moments alias the adjoints written next
+
+    // ---- E: reverse layers over (m, mt): (dm, dmt) = (dJ/dm, dJ/dmt)
+    for (int li = 0; li <= P.n_depth; ++li) dual_layer<NT, true>(P, li, lds, m2, dm2, part2, tid, alpha);
+
+    // ---- F: per (slot, part) accumulators, past the basic moments' adjoints
+    double* acc = lds + L.acc;
+    for (int t = tid; t < P.n_parts * KL * 2 * R; t += NT) acc[t] = 0.0;
+    __syncthreads();
+    {
+        const int kk = tid & (kWave - 1), prt = tid / kWave;
+        for (int k = kk; k < nv && prt < P.n_parts; k += kWave) {
             const double ir = 1.0 / rr[k];
             const double ux = vx[k] * ir, uy = vy[k] * ir, uz = vz[k] * ir;
-            double px[kMaxNu + 1], py[kMaxNu + 1], pz[kMaxNu + 1];
-            px[0] = py[0] = pz[0] = 1.0;
-#pragma unroll
-            for (int p = 1; p <= kMaxNu; ++p) { px[p] = px[p - 1] * ux; py[p] = py[p - 1] * uy; pz[p] = pz[p - 1] * uz; }
-            double Am[kMaxR], Apm[kMaxR], Qm[kMaxR];
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu) { Am[mu] = 0.0; Apm[mu] = 0.0; Qm[mu] = 0.0; }
-            const double ddx = Dx[k], ddy = Dy[k], ddz = Dz[k], udk = ud[k];
-            const int b1 = P.part_beg[part + 1];
-            for (int bi = P.part_beg[part]; bi < b1; ++bi) {
-                const I2 bl = P.part_blk[bi];
-                const int mu = bl.x & 0xff, nu = bl.y;
-                const double* d = dm + (bl.x >> 8);
-                const double* dt = dmt + (bl.x >> 8);
-                double a0, a1, q;
-                if (nu == 0) {
-                    a0 = d[0]; a1 = dt[0]; q = 0.0;
-                } else {
-                    double Hx = 0, Hy = 0, Hz = 0, Tx = 0, Ty = 0, Tz = 0;
-                    grad_block_dispatch(nu, d, px, py, pz, Hx, Hy, Hz);
-                    grad_block_dispatch(nu, dt, px, py, pz, Tx, Ty, Tz);
-                    const double uH = ux * Hx + uy * Hy + uz * Hz;
-                    const double uT = ux * Tx + uy * Ty + uz * Tz;
-                    a0 = uH / (double)nu;
-                    a1 = uT / (double)nu;
-                    q = (Tx * ddx + Ty * ddy + Tz * ddz - uT * udk) * ir;
-                }
-#pragma unroll
-                for (int mm = 0; mm < kMaxR; ++mm)
-                    if (mm == mu) { Am[mm] += a0; Apm[mm] += a1; Qm[mm] += q; }
-            }
-            double* o = accs + ((size_t)part * K + k) * 3 * R;
-#pragma unroll
-            for (int mu = 0; mu < kMaxR; ++mu)
-                if (mu < R) { o[mu] = Am[mu]; o[R + mu] = Apm[mu]; o[2 * R + mu] = Qm[mu]; }
+            double* o = acc + ((size_t)prt * KL + k) * 2 * R;
+            const double e0 = ex[k], e1 = ey[k], e2 = ez[k];
+            const int nu_max = P.nu_max;
+            if (nu_max <= 4) dual_slot<4>(P, prt, ux, uy, uz, e0, e1, e2, dm2, o);
+            else if (nu_max <= 6) dual_slot<6>(P, prt, ux, uy, uz, e0, e1, e2, dm2, o);
+            else if (nu_max <= 8) dual_slot<8>(P, prt, ux, uy, uz, e0, e1, e2, dm2, o);
+            else if (nu_max <= 10) dual_slot<10>(P, prt, ux, uy, uz, e0, e1, e2, dm2, o);
+            else dual_slot<kMaxNu>(P, prt, ux, uy, uz, e0, e1, e2, dm2, o);
         }
     }
     __syncthreads();
-    // dJ/dc[ti][tk][mu][n] for this atom, plus alpha for the species gradient
+    // parts summed in order into part 0
+    for (int t = tid; t < nv * 2 * R; t += NT) {
+        double s = acc[t];
+        for (int prt = 1; prt < P.n_parts; ++prt) s += acc[(size_t)prt * KL * 2 * R + t];
+        acc[t] = s;
+    }
+    __syncthreads();
+
+    // dJ/dc[ti][tk][mu][n] of this atom (column (tk R + mu) N + n), slots
+    // split over G thread groups whose sums are added in group order; plus
+    // alpha for the species gradient
     {
         const int ncols = P.Sp * R * N;
         double* out = A.gc_part + (size_t)row * (1 + ncols);
+        double* red = lds + L.red;
         if (tid == 0) out[0] = alpha;
-        for (int c = tid; c < ncols; c += kTB) {
-            const int n = c % N, mu = (c / N) % R, tk = c / (N * R);
+        const int G = ncols >= NT ? 1 : (NT / ncols < 8 ? NT / ncols : 8);
+        for (int c0 = 0; c0 < ncols; c0 += NT / G) {
+            const int c = c0 + tid % (NT / G), g = tid / (NT / G);
             double s = 0.0;
-            for (int k = 0; k < nv; ++k) {
-                if (wrap_species(jtc[k], P.Sp) != tk) continue;
-                double aq = 0.0, ap = 0.0;
-                for (int part = 0; part < P.n_parts; ++part) {
-                    const double* o = accs + ((size_t)part * K + k) * 3 * R;
-                    aq += o[mu] + o[2 * R + mu];
-                    ap += o[R + mu];
+            if (c < ncols && g < G) {
+                const int n = c % N, mu = (c / N) % R, tk = c / (N * R);
+                for (int k = g; k < nv; k += G) {
+                    const double v = acc[k * 2 * R + mu] * ca[n * KL + k] + acc[k * 2 * R + R + mu] * cb[n * KL + k];
+                    s += wrap_species(jtc[k], P.Sp) == tk ? v : 0.0;
                 }
-                s += aq * ca[n * K + k] + ap * ud[k] * cb[n * K + k];
             }
-            out[1 + c] = s;
+            if (G == 1) {
+                if (c < ncols) out[1 + c] = s;
+            } else {
+                if (g < G) red[g * (NT / G) + tid % (NT / G)] = s;
+                __syncthreads();
+                if (tid < NT / G && c0 + tid < ncols) {
+                    double t = red[tid];
+                    for (int q = 1; q < G; ++q) t += red[q * (NT / G) + tid];
+                    out[1 + c0 + tid] = t;
+                }
+                __syncthreads();
+            }
         }
     }
 }
@@ -431,7 +668,7 @@ hipError_t launch_train_grad(const DevPlan& P, const TrainArgs& A, size_t lds_by
                              double* xi_sum, double* typed_sum, double* grad, hipStream_t stream)
 {
     const int ncols = P.Sp * P.R * P.N;
-    hipLaunchKernelGGL(train_kernel, dim3(A.nrows), dim3(kTB), lds_bytes, stream, P, A);
+    hipLaunchKernelGGL(train_kernel, dim3(A.nrows), dim3(kTrainThreads), lds_bytes, stream, P, A);
     const int nchunk = (A.nrows + 1023) / 1024;
     // xi columns (no types)
     hipLaunchKernelGGL(colsum_stage1, dim3((P.S + 63) / 64, nchunk), dim3(256), 0, stream, A.gxi_part, A.nrows, P.S,

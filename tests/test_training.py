@@ -84,7 +84,9 @@ def test_exact_gradient_matches_loss_finite_differences(jm, O):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("level,species", [(10, 1), (16, 2), (20, 2)])
+# (16, 8): 256 radial-coefficient columns per row, one per thread of the
+# kernel's final reduction; the others split slots over thread groups
+@pytest.mark.parametrize("level,species", [(2, 1), (6, 3), (10, 1), (12, 3), (16, 2), (16, 8), (20, 2)])
 def test_gpu_batch_forward_and_param_grad(jm, O, level, species):
     import torch
 

@@ -71,8 +71,9 @@ def traffic_bytes(workload, kernel):
 def head_leg(seed, steps, save_dir=None):
     """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py)."""
     jm = importlib.import_module("jax-mtp_amd")
-    res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, seed=seed, decay=0.35, log=log,
-                         save_dir=save_dir)
+    res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, desc_steps=40, seed=seed,
+                         decay=0.35, log=log, save_dir=save_dir)
+    res["desc_lbfgs_steps"] = 40
     res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
     return res
 

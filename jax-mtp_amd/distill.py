@@ -110,10 +110,11 @@ def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=No
     """Level-12 MTP over the teacher's own low-level invariants.
 
     The level-12 basis uses radial functions mu < 3 (level 2 + 4 mu <= 12);
-    they are taken from the teacher, so the head's inputs are exactly the
-    teacher's level-12 basis values.  Moment coefficients come from linear
-    least squares (then, if steps > 0, L-BFGS over all coefficients).  Alone,
-    this potential is the method without the network."""
+    they start as the teacher's, so the head's inputs start as the teacher's
+    own level-12 basis values.  Moment coefficients come from linear least
+    squares; with steps > 0, L-BFGS then adjusts all coefficients (radial
+    ones included) to the teacher's energies and forces.  Alone, this
+    potential is the method without the network."""
     rng = np.random.default_rng(seed)
     p = configs.random_params(12, teacher.species_count, teacher.radial_basis_size, teacher.min_dist,
                               teacher.max_dist, rng, scaling=teacher.scaling)
@@ -152,7 +153,7 @@ def _time(fn, reps=5, warmup=2):
 
 
 def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
-        save_dir=None):
+        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0)):
     """Teacher, training set, level-12 fit, head training, then evaluate();
     with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
     are written there for infer()."""
@@ -167,9 +168,9 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
     E_site, F, E_c, S_c = label(tpot, batch)
     tpot.close()
     desc, hist_d = fit_descriptor(teacher, batch, E_c, F, S_c, seed=seed, steps=desc_steps, log=log)
-    tr = HeadTrainer(desc, batch, E_site, F, weights=(1.0, 1.0), margin=0.05, seed=seed)
+    tr = HeadTrainer(desc, batch, E_site, F, weights=head_weights, margin=0.05, seed=seed)
     t0 = time.time()
-    hist = tr.fit(steps=head_steps, lr=3e-3, log_every=max(1, head_steps // 10))
+    hist = tr.fit(steps=head_steps, lr=head_lr, log_every=max(1, head_steps // 10))
     torch.cuda.synchronize()
     t_fit = time.time() - t0
     e_rmse, f_rmse = tr.rmse()
@@ -251,6 +252,7 @@ def main(argv=None):
     ap.add_argument("--n-env", type=int, default=100_000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--decay", type=float, default=0.35)
+    ap.add_argument("--desc-steps", type=int, default=40, help="L-BFGS steps of the level-12 descriptor fit")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", default=None, help="directory for head.npz and teacher.mtp")
     ap.add_argument("--infer", default=None, help="only evaluate the head saved in this directory")
@@ -259,7 +261,8 @@ def main(argv=None):
     if a.infer:
         res = infer(a.infer, reps=a.reps)
     else:
-        res = run(n_env=a.n_env, head_steps=a.steps, decay=a.decay, seed=a.seed, save_dir=a.save)
+        res = run(n_env=a.n_env, head_steps=a.steps, desc_steps=a.desc_steps, decay=a.decay, seed=a.seed,
+                  save_dir=a.save)
     print(json.dumps(res), flush=True)
 
 

@@ -9,7 +9,7 @@ adjoint -> per-slot gradients -> Newton-3 force gather -> virial.
 Prints ONE JSON line.  ``training_step`` adds the packed training
 configuration (512 uneven configurations, level 16, eager vs HIP-graph
 evaluation).  ``distilled_head`` adds the head configuration:
-distillation of the level-24 MTP into an MLP over level-12 basis values
+distillation of the level-24 MTP into an MLP over level-14 basis values
 (1e5 training environments) and its atom-steps/s and force RMSE on a
 100,000-atom cell against plain level-24 evaluation.  ``roofline`` is the atom kernel's achieved
 algorithmic FP64 FLOP rate (fixed per-atom FLOP formula)
@@ -72,7 +72,8 @@ def head_leg(seed, steps, save_dir=None):
     """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py)."""
     jm = importlib.import_module("jax-mtp_amd")
     res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, desc_steps=40, seed=seed,
-                         decay=0.35, log=log, save_dir=save_dir)
+                         decay=0.35, log=log, save_dir=save_dir, desc_level=14)
+    res["desc_level"] = 14
     res["desc_lbfgs_steps"] = 40
     res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
     return res

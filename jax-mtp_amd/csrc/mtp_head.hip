@@ -95,56 +95,177 @@ __global__ void prepare_kernel(HeadNet Hn, double* wt)
 
 __global__ void zero_count_kernel(int* c) { *c = 0; }
 
-__global__ void __launch_bounds__(1024) head_eval_kernel(const HeadNet Hn, int n, const int* itypes,
-                                                         const double* basis, double* E, double* xi, int* flags,
-                                                         int* rows, int* count)
+// Inference on FP64 matrix cores: one wavefront per 16 atoms.  Each layer
+// is computed transposed (hidden units on the 16 rows of an MFMA tile,
+// atoms on its 16 columns), out = W^T z with v_mfma_f64_16x16x4f64: the
+// A operand (lane l: row l % 16, k l / 16) is read from the weights in LDS,
+// the B operand (lane l: k l / 16, column l % 16) is the previous layer's
+// output.  The output of a 16x16 tile holds, on lane l, rows 4 q + l / 16
+// (q = 0..3) of column l % 16, which is exactly the B operand of k-step
+// 4 t + q of the next product (t = the tile), so activations never leave
+// the registers.  Backward to the inputs uses the same weights read along
+// the other index (row stride kWS, odd, so both orientations read
+// conflict-free).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kWS = H + 1;
+constexpr int kEvalLdsDoubles = IN * kWS + H * kWS + 3 * H;
+constexpr int kEvalBlock = 256;
+static_assert(IN == kWave && H == kWave, "one hidden unit / input per 16x16 tile row");
+
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(2))) head_eval_kernel(const HeadNet Hn, int n, const int* itypes,
+                                                               const double* basis, double* E, double* xi,
+                                                               int* flags, int* rows, int* count)
 {
     extern __shared__ double w[];
-    const HeadLds L = load_weights(Hn, w);
+    double* W1s = w;               // W1[i][h] at i * kWS + h
+    double* W2s = W1s + IN * kWS;  // W2[l][j] at l * kWS + j
+    double* b1 = W2s + H * kWS;
+    double* b2 = b1 + H;
+    double* w3 = b2 + H;
+    for (int q = threadIdx.x; q < IN * H; q += blockDim.x) W1s[(q / H) * kWS + q % H] = Hn.theta[oW1 + q];
+    for (int q = threadIdx.x; q < H * H; q += blockDim.x) W2s[(q / H) * kWS + q % H] = Hn.theta[oW2 + q];
+    for (int q = threadIdx.x; q < H; q += blockDim.x) {
+        b1[q] = Hn.theta[oB1 + q];
+        b2[q] = Hn.theta[oB2 + q];
+        w3[q] = Hn.theta[oW3 + q];
+    }
+    __syncthreads();
     const double b3 = Hn.theta[oB3];
     const int lane = threadIdx.x & (kWave - 1);
-    const int nw = (gridDim.x * blockDim.x) / kWave;
-    const int i = lane & (IN - 1), half = lane / IN;
-    for (int a = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; a < n; a += nw) {
-        const int t = itypes[a];
+    const int col = lane & 15, rq = lane >> 4;
+    const int S = Hn.S;
+    const int k1 = (S + Hn.Sp + 3) / 4;  // k-steps of the first layer (inputs in use)
+    const int tz = (S + 15) / 16;        // tiles of dE/dz that hold basis inputs
+    const int ngroups = (n + 15) / 16;
+    const int nwb = blockDim.x / kWave;
+    for (int g = blockIdx.x * nwb + (int)(threadIdx.x / kWave); g < ngroups; g += gridDim.x * nwb) {
+        const int a = g * 16 + col;  // this lane's atom (tile column)
+        const bool va = a < n;
+        const int t = va ? itypes[a] : 0;
         const int ts = wrap_species(t, Hn.Sp);
-        const double z = input_of(Hn, basis, a, ts, lane);
-        const bool out = Hn.lo && lane < Hn.S && (z < Hn.lo[ts * Hn.S + lane] || z > Hn.hi[ts * Hn.S + lane]);
-        // forward
-        double a1 = L.b1[lane];
+        // inputs z[a][4 kb + rq] (normalised basis values, species one-hot)
+        double z[16];
+        bool out = false;
+        double lin_p = 0.0;
 #pragma unroll
-        for (int q = 0; q < IN; ++q) a1 += L.W1[q * H + lane] * bcast(z, q);
-        const Silu s1 = silu(a1);
-        double a2 = L.b2[lane];
-#pragma unroll 16
-        for (int l = 0; l < H; ++l) a2 += L.W2[l * H + lane] * bcast(s1.h, l);
-        const Silu s2 = silu(a2);
-        const double net = wave_sum(L.w3[lane] * s2.h) + b3;
-        // backward to the inputs
-        const double da2 = L.w3[lane] * s2.s;
-        double dh1 = 0.0;
-#pragma unroll 16
-        for (int j = 0; j < H; ++j) dh1 += L.W2T[j * H + lane] * bcast(da2, j);
-        const double da1 = dh1 * s1.s;
-        double dz = 0.0;  // lanes i and i + 32 take the two halves of the hidden units
-#pragma unroll 8
-        for (int l = 0; l < H / 2; ++l) {
-            const double d0 = bcast(da1, l), d1 = bcast(da1, l + H / 2);
-            dz += L.W1T[(half * (H / 2) + l) * IN + i] * (half ? d1 : d0);
+        for (int kb = 0; kb < 16; ++kb) {
+            const int i = 4 * kb + rq;
+            double v = 0.0;
+            if (va && i < S) {
+                const double bv = basis[(size_t)a * S + i];
+                v = (bv - Hn.mean[i]) * Hn.inv_std[i];
+                if (Hn.lo) out = out || v < Hn.lo[ts * S + i] || v > Hn.hi[ts * S + i];
+                if (Hn.lin) lin_p += Hn.lin[i] * bv;
+            } else if (va && i == S + ts) {
+                v = 1.0;
+            }
+            z[kb] = v;
         }
-        dz += __shfl_xor(dz, IN);
-        double lin_e = 0.0;
-        if (Hn.lin) {  // residual form: + species energy + lin . b
-            const double bl = lane < Hn.S ? basis[(size_t)a * Hn.S + lane] : 0.0;
-            lin_e = wave_sum(lane < Hn.S ? Hn.lin[lane] * bl : 0.0) + Hn.lin_sc[ts];
+        // layer 1: a1^T = W1^T z^T
+        dbl4 acc[4];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int tp = 0; tp < 4; tp += 2) {
+#pragma unroll
+            for (int kb = 0; kb < 16; ++kb) {
+                if (kb < k1) {
+                    acc[tp] = mfma4(W1s[(4 * kb + rq) * kWS + 16 * tp + col], z[kb], acc[tp]);
+                    acc[tp + 1] = mfma4(W1s[(4 * kb + rq) * kWS + 16 * tp + 16 + col], z[kb], acc[tp + 1]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        if (lane < Hn.S)
-            xi[(size_t)a * Hn.S + lane] = Hn.e_scale * dz * Hn.inv_std[lane] + (Hn.lin ? Hn.lin[lane] : 0.0);
-        const bool any_out = __ballot(out) != 0;
-        if (lane == 0) {
-            E[a] = Hn.e_shift[ts] + Hn.e_scale * net + lin_e;
-            if (flags) flags[a] = any_out ? 1 : 0;
-            if (count && any_out && t >= 0) rows[atomicAdd(count, 1)] = a;
+        double h1[16], s1[16];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const Silu sv = silu(acc[tt][q] + b1[16 * tt + 4 * q + rq]);
+                h1[4 * tt + q] = sv.h;
+                s1[4 * tt + q] = sv.s;
+            }
+        // layer 2: a2^T = W2^T h1^T
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int tp = 0; tp < 4; tp += 2) {
+#pragma unroll
+            for (int kb = 0; kb < 16; ++kb) {
+                acc[tp] = mfma4(W2s[(4 * kb + rq) * kWS + 16 * tp + col], h1[kb], acc[tp]);
+                acc[tp + 1] = mfma4(W2s[(4 * kb + rq) * kWS + 16 * tp + 16 + col], h1[kb], acc[tp + 1]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double da2[16];
+        double net = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int h = 16 * tt + 4 * q + rq;
+                const Silu sv = silu(acc[tt][q] + b2[h]);
+                net += w3[h] * sv.h;
+                da2[4 * tt + q] = w3[h] * sv.s;
+            }
+        net += __shfl_xor(net, 16);
+        net += __shfl_xor(net, 32);
+        // backward: dh1^T = W2 da2^T, da1 = dh1 silu'(a1)
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int tp = 0; tp < 4; tp += 2) {
+#pragma unroll
+            for (int kb = 0; kb < 16; ++kb) {
+                acc[tp] = mfma4(W2s[(16 * tp + col) * kWS + 4 * kb + rq], da2[kb], acc[tp]);
+                acc[tp + 1] = mfma4(W2s[(16 * tp + 16 + col) * kWS + 4 * kb + rq], da2[kb], acc[tp + 1]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double da1[16];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) da1[4 * tt + q] = acc[tt][q] * s1[4 * tt + q];
+        // dz^T = W1 da1^T over the tiles that hold basis inputs
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int tp = 0; tp < 4; tp += 2) {
+            if (tp < tz) {
+#pragma unroll
+                for (int kb = 0; kb < 16; ++kb) {
+                    acc[tp] = mfma4(W1s[(16 * tp + col) * kWS + 4 * kb + rq], da1[kb], acc[tp]);
+                    acc[tp + 1] = mfma4(W1s[(16 * tp + 16 + col) * kWS + 4 * kb + rq], da1[kb], acc[tp + 1]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (va) {
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = 16 * tt + 4 * q + rq;
+                    if (tt < tz && i < S)
+                        xi[(size_t)a * S + i] = Hn.e_scale * acc[tt][q] * Hn.inv_std[i] + (Hn.lin ? Hn.lin[i] : 0.0);
+                }
+        }
+        int o = out ? 1 : 0;
+        o |= __shfl_xor(o, 16);
+        o |= __shfl_xor(o, 32);
+        lin_p += __shfl_xor(lin_p, 16);
+        lin_p += __shfl_xor(lin_p, 32);
+        if (va && rq == 0) {
+            const double lin_e = Hn.lin ? lin_p + Hn.lin_sc[ts] : 0.0;
+            E[a] = Hn.e_shift[ts] + Hn.e_scale * (net + b3) + lin_e;
+            if (flags) flags[a] = o;
+            if (count && o && t >= 0) rows[atomicAdd(count, 1)] = a;
         }
     }
 }
@@ -301,14 +422,16 @@ hipError_t head_eval(const HeadNet& Hn, int n, const int* itypes, const double* 
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)head_eval_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           kLdsDoubles * (int)sizeof(double));
+                                           kEvalLdsDoubles * (int)sizeof(double));
         if (e != hipSuccess) return e;
         attr = true;
     }
-    int grid = (n + 15) / 16;
-    if (grid > 256) grid = 256;
-    hipLaunchKernelGGL(head_eval_kernel, dim3(grid), dim3(1024), kLdsDoubles * sizeof(double), s, Hn, n, itypes,
-                       basis, E, xi, flags, rows, count);
+    const int ngroups = (n + 15) / 16;
+    const int per_block = kEvalBlock / kWave;
+    int grid = (ngroups + per_block - 1) / per_block;
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(head_eval_kernel, dim3(grid), dim3(kEvalBlock), kEvalLdsDoubles * sizeof(double), s, Hn, n,
+                       itypes, basis, E, xi, flags, rows, count);
     return hipGetLastError();
 }
 

@@ -4,14 +4,14 @@
 //
 // Network (one per potential, shared by all species; the species enters as
 // a one-hot input):
-//   z_i  = [(b_i - mean) * inv_std, onehot(t_i)]             (S + Sp <= 32)
+//   z_i  = [(b_i - mean) * inv_std, onehot(t_i)]             (S + Sp <= 64)
 //   a1   = W1^T z + b1,  h1 = silu(a1)                        (64)
 //   a2   = W2^T h1 + b2, h2 = silu(a2)                        (64)
 //   E_i  = e_shift[t_i] + e_scale * (w3 . h2 + b3)
 //          [+ lin_sc[t_i] + lin . b_i   in residual form: the network then
 //           corrects the low-level potential's own linear energy]
 // Flat parameter vector theta (weights stored (in, out), row major):
-//   W1[32][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3
+//   W1[64][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3
 // Rows of W1 beyond S + Sp multiply zero inputs.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -19,7 +19,7 @@
 namespace mtp {
 
 constexpr int kHeadHidden = 64;  // = wavefront width: lane j owns hidden unit j
-constexpr int kHeadIn = 32;      // padded input width
+constexpr int kHeadIn = 64;      // padded input width (= wavefront width: lane i owns input i)
 constexpr int kHeadTheta = kHeadIn * kHeadHidden + kHeadHidden + kHeadHidden * kHeadHidden + 2 * kHeadHidden + 1;
 
 struct HeadNet {

@@ -1,13 +1,13 @@
 """Distillation of a level-24 MTP into the head, and its evaluation.
 
 Workload (the distilled-head configuration): a level-24, 3-species MTP is
-the teacher; the head reads level-12 basis values (29 per atom) through a
+the teacher; the head reads level-14 basis values (52 per atom) through a
 2x64 SiLU network; it is trained on 1e5 site environments of thermally
 displaced 3-species FCC snapshots at 300-1500 K and evaluated on a
 100,000-atom 3-species FCC cell, with exact level-24 re-evaluation of every
 atom outside the training box.  Reported: atom-steps/s of the head (with
 fallback) and of the plain level-24 evaluation, the head's force RMSE
-against the level-24 forces, and the same RMSE for the level-12 MTP alone
+against the level-24 forces, and the same RMSE for the level-14 MTP alone
 (the method without the network).
 
 Synthetic teacher.  All parameters are seeded; there is no trained
@@ -106,22 +106,22 @@ def label(pot: MTPPotential, batch):
     return extra["eatom"].clone(), F.clone(), E_c.clone(), S_c.clone()
 
 
-def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None):
-    """Level-12 MTP over the teacher's own low-level invariants.
+def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None, level=12):
+    """Low-level MTP (level 14 by default) over the teacher's own invariants.
 
-    The level-12 basis uses radial functions mu < 3 (level 2 + 4 mu <= 12);
-    they start as the teacher's, so the head's inputs start as the teacher's
-    own level-12 basis values.  Moment coefficients come from linear least
+    A level-L basis uses the radial functions mu with 2 + 4 mu <= L (mu < 4
+    at level 14); they start as the teacher's, so the head's inputs start as
+    the teacher's own low-level basis values.  Moment coefficients come from linear least
     squares; with steps > 0, L-BFGS then adjusts all coefficients (radial
     ones included) to the teacher's energies and forces.  Alone, this
     potential is the method without the network."""
     rng = np.random.default_rng(seed)
-    p = configs.random_params(12, teacher.species_count, teacher.radial_basis_size, teacher.min_dist,
+    p = configs.random_params(level, teacher.species_count, teacher.radial_basis_size, teacher.min_dist,
                               teacher.max_dist, rng, scaling=teacher.scaling)
     R = p.radial_funcs_count
     p.radial_coeffs = np.array(teacher.radial_coeffs)[:, :, :R, :].copy()
     p.species_coeffs = np.array(teacher.species_coeffs).copy()
-    p.potential_name = "level12_descriptor"
+    p.potential_name = f"level{level}_descriptor"
     pot = MTPPotential(p, max_atoms=batch.natoms, max_nbr=MAX_NBR)
     tr = training.Trainer(pot, batch, E_c, F, S_c, weights=(1.0, 1.0, 0.0))
     hist = [float(tr.loss().detach())]
@@ -132,7 +132,7 @@ def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=No
     sp = p.species_count
     pot.set_params(th[:S], th[S:S + sp], th[S + sp:])
     if log:
-        log(f"[distill] level-12 fit loss {hist[0]:.4e} -> {hist[-1]:.4e}")
+        log(f"[distill] level-{level} fit loss {hist[0]:.4e} -> {hist[-1]:.4e}")
     return pot, hist
 
 
@@ -153,8 +153,8 @@ def _time(fn, reps=5, warmup=2):
 
 
 def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
-        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0)):
-    """Teacher, training set, level-12 fit, head training, then evaluate();
+        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=14):
+    """Teacher, training set, descriptor fit, head training, then evaluate();
     with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
     are written there for infer()."""
     import torch
@@ -167,7 +167,8 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
     tpot = MTPPotential(teacher, max_atoms=batch.natoms, max_nbr=MAX_NBR)
     E_site, F, E_c, S_c = label(tpot, batch)
     tpot.close()
-    desc, hist_d = fit_descriptor(teacher, batch, E_c, F, S_c, seed=seed, steps=desc_steps, log=log)
+    desc, hist_d = fit_descriptor(teacher, batch, E_c, F, S_c, seed=seed, steps=desc_steps, log=log,
+                                  level=desc_level)
     tr = HeadTrainer(desc, batch, E_site, F, weights=head_weights, margin=0.05, seed=seed)
     t0 = time.time()
     hist = tr.fit(steps=head_steps, lr=head_lr, log_every=max(1, head_steps // 10))
@@ -210,7 +211,7 @@ def evaluate(model, teacher: MTPData, infer_ncell=(25, 25, 40), seed=2, reps=5):
     exact.autotune(it, js, rij, jt)
     head = DistilledHead(model, exact, max_atoms=A, max_nbr=MAX_NBR)
     head.desc.autotune(it, js, rij, jt)
-    lin = head.desc  # the fitted level-12 MTP itself
+    lin = head.desc  # the fitted descriptor MTP itself
     Ex, Fx, _ = exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
     Eh, Fh, _, extra = head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
     El, Fl, _ = lin.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
@@ -225,10 +226,10 @@ def evaluate(model, teacher: MTPData, infer_ncell=(25, 25, 40), seed=2, reps=5):
         "speedup": ms_exact / ms_head, "ms_head": ms_head, "ms_exact": ms_exact,
         "n_fallback": extra["n_fallback"],
         "force_rmse_head_meV_per_A": rm(Fh, Fx) * 1e3,
-        "force_rmse_level12_meV_per_A": rm(Fl, Fx) * 1e3,
+        "force_rmse_desc_meV_per_A": rm(Fl, Fx) * 1e3,
         "force_rms_exact_eV_per_A": float(torch.sqrt((Fx ** 2).mean())),
         "energy_err_head_meV_per_atom": abs(float(Eh - Ex)) / A * 1e3,
-        "energy_err_level12_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
+        "energy_err_desc_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
         "variant_exact": exact.variant, "variant_desc": head.desc.variant,
     }
     head.close()
@@ -252,7 +253,8 @@ def main(argv=None):
     ap.add_argument("--n-env", type=int, default=100_000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--decay", type=float, default=0.35)
-    ap.add_argument("--desc-steps", type=int, default=40, help="L-BFGS steps of the level-12 descriptor fit")
+    ap.add_argument("--desc-level", type=int, default=14, help="MTP level of the descriptor")
+    ap.add_argument("--desc-steps", type=int, default=40, help="L-BFGS steps of the descriptor fit")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", default=None, help="directory for head.npz and teacher.mtp")
     ap.add_argument("--infer", default=None, help="only evaluate the head saved in this directory")
@@ -262,7 +264,7 @@ def main(argv=None):
         res = infer(a.infer, reps=a.reps)
     else:
         res = run(n_env=a.n_env, head_steps=a.steps, desc_steps=a.desc_steps, decay=a.decay, seed=a.seed,
-                  save_dir=a.save)
+                  save_dir=a.save, desc_level=a.desc_level)
     print(json.dumps(res), flush=True)
 
 

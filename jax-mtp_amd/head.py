@@ -31,7 +31,7 @@ from .mtp_io import MTPData, format_mtp, parse_mtp_text
 from .potential import MTPPotential, _f64, _i32
 
 HIDDEN = 64
-N_IN = 32
+N_IN = 64
 N_THETA = N_IN * HIDDEN + HIDDEN + HIDDEN * HIDDEN + 2 * HIDDEN + 1
 
 

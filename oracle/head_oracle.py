@@ -7,8 +7,8 @@ The reference has no learned head, so nothing here restates a reference
 file; the head's definition is the network documented in
 include/mtp_hip.h (mtp_head_*) and jax-mtp_amd/csrc/mtp_head.hpp:
 
-  z   = [(b - mean) * inv_std, onehot(type)]  padded to 32
-  a1  = z W1 + b1,  h1 = silu(a1)             W1 [32, 64]
+  z   = [(b - mean) * inv_std, onehot(type)]  padded to 64
+  a1  = z W1 + b1,  h1 = silu(a1)             W1 [64, 64]
   a2  = h1 W2 + b2, h2 = silu(a2)             W2 [64, 64]
   E   = e_shift[type] + e_scale * (h2 . w3 + b3)
         [+ lin_sc[type] + lin . b   in residual form]
@@ -30,7 +30,7 @@ from __future__ import annotations
 import numpy as np
 
 H = 64
-IN = 32
+IN = 64
 N_THETA = IN * H + H + H * H + H + H + 1
 
 

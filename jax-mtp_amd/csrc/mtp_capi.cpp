@@ -871,12 +871,13 @@ AtomArgs row_args(int nrows, int nslots, const int* it, const double* rij, const
 }
 
 int slot_grad_xi(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs, const int* d_jtypes,
-                 const double* d_xi, int xi_stride, double* d_g, double* d_Wi, hipStream_t s)
+                 const double* d_xi, int xi_stride, double* d_g, double* d_Wi, hipStream_t s,
+                 const double* d_mom_in = nullptr)
 {
     const int v = mode_variant(ctx, true);
     if (v < 0) return ctx->fail(MTP_ERR_UNSUPPORTED, "per-row coefficients need a balanced-item kernel variant");
     AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
-    A.xi_atom = d_xi; A.xi_stride = xi_stride; A.g = d_g; A.Wi = d_Wi;
+    A.xi_atom = d_xi; A.xi_stride = xi_stride; A.g = d_g; A.Wi = d_Wi; A.mom_in = d_mom_in;
     hipError_t e = launch_atoms(ctx->vdev[v], A, ctx->vlds_xi[v], ctx->radial_fp32, s, v, nrows);
     if (e != hipSuccess) return ctx->hip_fail(e, "slot-gradient launch");
     return MTP_OK;
@@ -891,6 +892,7 @@ struct mtp_head {
     std::string err;
     std::vector<void*> dptrs;
     double *d_theta = nullptr, *d_wt = nullptr, *d_basis = nullptr, *d_xi = nullptr, *d_E = nullptr;
+    double* d_mom = nullptr;  // [max_atoms][n_live] moments of the descriptor pass, reread by the adjoint pass
     int *d_rows = nullptr, *d_count = nullptr;
     double* d_part = nullptr;
 
@@ -1048,6 +1050,7 @@ int mtp_head_open(mtp_ctx* desc, mtp_ctx* exact, const double* theta, const doub
     double* d_hi = hi ? h->alloc<double>((size_t)Sp * S, e) : nullptr;
     double* d_shift = h->alloc<double>(Sp, e);
     h->d_basis = h->alloc<double>(A * S, e);
+    h->d_mom = h->alloc<double>(A * (size_t)std::max(1, desc->plan.n_live), e);
     h->d_xi = h->alloc<double>(A * S, e);
     h->d_E = h->alloc<double>(A, e);
     h->d_rows = h->alloc<int>(A, e);
@@ -1147,15 +1150,21 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out; F.tot = D->d_tot;
     if (nrows > 0) {
         // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
-        rc = mtp_descriptor_device(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_basis, s);
-        if (rc) return h->fail(rc, D->err);
+        {
+            const int v = mode_variant(D, false);
+            AtomArgs Bd = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+            Bd.basis = h->d_basis; Bd.fwd_only = 1; Bd.mom_out = h->d_mom;
+            e = launch_atoms(D->vdev[v], Bd, D->vlds[v], D->radial_fp32, s, v, nrows);
+            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("descriptor launch: ") + hipGetErrorString(e));
+        }
         e = head_zero_count(h->d_count, s);
         if (e == hipSuccess)
             e = head_eval(h->net, nrows, d_itypes, h->d_basis, E, h->d_xi, d_flags, X ? h->d_rows : nullptr,
                           X ? h->d_count : nullptr, s);
         if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
         // 3. slot gradients of the head energy through the descriptor
-        rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g, D->d_Wi, s);
+        rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g, D->d_Wi, s,
+                          h->d_mom);
         if (rc) return h->fail(rc, D->err);
         // 4. exact energies and slot gradients of the out-of-box rows
         if (X) {

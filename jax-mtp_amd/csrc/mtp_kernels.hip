@@ -593,50 +593,62 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     // thread per monomial; the mu range is the largest of the monomial's
     // wavefront, so the slot loop is a wave-uniform unrolled instance (lanes
     // of lower degree carry unused mu).
-    const int np1 = P.nu_max + 1;
-    const int Kp = P.k_layout | 1;
-    double* pw = dm;
-    for (int i = tid; i < nv * P.R; i += NT) {  // radial values: one thread per (slot, mu)
-        const int c = i / P.R, mu = i - c * P.R;
-        double fv, dfv;
-        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
-        f[i] = fv;
-    }
-    for (int t = tid; t < nv * 3; t += NT) {
-        const int k = t / 3, q = t - 3 * k;
-        const double inv = 1.0 / rr[k];
-        const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
-        double* col = pw + (size_t)q * np1 * Kp + k;
-        double v = 1.0;
-        col[0] = 1.0;
-        for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
-    }
-    __syncthreads();
-    basic_moments<NW>(P, tid / kWave, tid & (kWave - 1), nv, pw, Kp, np1, f, m, 1);
-    // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
-    // region, free once the power table and f are consumed)
-    for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
-    __syncthreads();
-
-    // ---- C: contraction layers (depth 1 .. n_depth)
-    for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, lds, dm, part, tid, P.xi_of);
-
-    // ---- D: site energy (and the basis values m[mapping] when asked for)
-    const double* xrow = A.xi_atom ? A.xi_atom + (size_t)row * A.xi_stride : P.moment_coeffs;
-    {
-        double acc = 0.0;
-        for (int s = tid; s < P.S; s += NT) {
-            const double b = m[P.mapping[s]];
-            acc += xrow[s] * b;
-            if (A.basis) A.basis[(size_t)row * P.S + s] = b;
-        }
-        red[tid] = acc;
+    if (NT <= 256 && A.mom_in) {
+        // moments [0, n_live) of this row as the descriptor pass left them:
+        // no radial basis, basic moments, forward layers or energy here.
+        // (The 512-thread kernel recomputes them: in it this branch costs
+        // registers; the results are the same.)
+        const double* mi = A.mom_in + (size_t)row * P.n_live;
+        for (int x = tid; x < P.n_live; x += NT) m[x] = mi[x];
         __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] += red[tid + w];
-            __syncthreads();
+    } else {
+        const int np1 = P.nu_max + 1;
+        const int Kp = P.k_layout | 1;
+        double* pw = dm;
+        for (int i = tid; i < nv * P.R; i += NT) {  // radial values: one thread per (slot, mu)
+            const int c = i / P.R, mu = i - c * P.R;
+            double fv, dfv;
+            radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+            f[i] = fv;
         }
-        if (tid == 0 && A.Ei) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
+        for (int t = tid; t < nv * 3; t += NT) {
+            const int k = t / 3, q = t - 3 * k;
+            const double inv = 1.0 / rr[k];
+            const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
+            double* col = pw + (size_t)q * np1 * Kp + k;
+            double v = 1.0;
+            col[0] = 1.0;
+            for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
+        }
+        __syncthreads();
+        basic_moments<NW>(P, tid / kWave, tid & (kWave - 1), nv, pw, Kp, np1, f, m, 1);
+        // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
+        // region, free once the power table and f are consumed)
+        for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
+        __syncthreads();
+
+        // ---- C: contraction layers (depth 1 .. n_depth)
+        for (int d = 0; d < P.n_depth; ++d) contraction_layer<NT, false>(P, d, lds, dm, part, tid, P.xi_of);
+
+        // ---- D: site energy (and the basis values m[mapping] when asked for)
+        {
+            const double* xrow = A.xi_atom ? A.xi_atom + (size_t)row * A.xi_stride : P.moment_coeffs;
+            double acc = 0.0;
+            for (int s = tid; s < P.S; s += NT) {
+                const double b = m[P.mapping[s]];
+                acc += xrow[s] * b;
+                if (A.basis) A.basis[(size_t)row * P.S + s] = b;
+            }
+            if (A.mom_out)
+                for (int x = tid; x < P.n_live; x += NT) A.mom_out[(size_t)row * P.n_live + x] = m[x];
+            red[tid] = acc;
+            __syncthreads();
+            for (int w = NT / 2; w > 0; w >>= 1) {
+                if (tid < w) red[tid] += red[tid + w];
+                __syncthreads();
+            }
+            if (tid == 0 && A.Ei) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
+        }
     }
     if (A.fwd_only) return;
 
@@ -646,6 +658,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     // Jacobians).
     const double* xi = P.xi_of;
     if (A.xi_atom) {
+        const double* xrow = A.xi_atom + (size_t)row * A.xi_stride;
         double* xl = lds + L.xl;
         for (int x = tid; x < P.amc; x += NT) xl[x] = 0.0;
         __syncthreads();

@@ -25,6 +25,14 @@ struct AtomArgs {
     int fwd_only;          // stop after the basis and the energy (g, Wi untouched)
     const int* rows;       // rows to evaluate (null: 0 .. nrows-1)
     const int* row_count;  // device-side length of `rows` (null: nrows)
+    // Moment cache of a descriptor pass followed by a per-row-coefficient
+    // pass over the same rows (streams variants only): the first writes
+    // m[0, n_live) of every row to mom_out [rows][n_live]; the second reads
+    // them from mom_in and skips the radial basis, basic moments, forward
+    // layers and energy (Ei and basis must be null then; kernels of more
+    // than 256 threads recompute instead).
+    double* mom_out;
+    const double* mom_in;
 };
 
 struct ForceArgs {

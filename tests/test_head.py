@@ -242,12 +242,15 @@ def test_gpu_head_network_and_gradient(jm, O, residual, level):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("exact_level", [16, 24])
-def test_gpu_head_compute_with_fallback(jm, O, exact_level):
+@pytest.mark.parametrize("exact_level,desc_variant", [(16, 3), (24, 3), (24, 1), (24, 2)])
+def test_gpu_head_compute_with_fallback(jm, O, exact_level, desc_variant):
+    """Descriptor variants 64 and 256 threads reuse the descriptor pass's
+    moments in the adjoint pass; 512 threads recompute them."""
     import head_oracle as HO
 
     w, nb = _level12_case(jm, seed=7, natoms=48, cell=8.6)
-    print(f"[case] head compute natoms={w.natoms} exact level {exact_level}", flush=True)
+    print(f"[case] head compute natoms={w.natoms} exact level {exact_level} descriptor variant {desc_variant}",
+          flush=True)
     rng = np.random.default_rng(7)
     model = _model(rng, box=False)
     orc_d = oracle_for(O, w.params)
@@ -271,6 +274,7 @@ def test_gpu_head_compute_with_fallback(jm, O, exact_level):
     hm = jm.head.HeadModel(w.params, model.theta, model.mean, model.inv_std, lo, hi, model.e_shift, model.e_scale,
                            True)
     head = jm.head.DistilledHead(hm, exact, max_atoms=A, max_nbr=K)
+    head.desc.set_variant(desc_variant)
     E, F, St, extra = head.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
                                                             w.volume, n_e, n_f, return_all=True)
     assert float(E) == pytest.approx(E_r, rel=1e-10)

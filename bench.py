@@ -9,10 +9,10 @@ adjoint -> per-slot gradients -> Newton-3 force gather -> virial.
 Prints ONE JSON line.  ``training_step`` adds the packed training
 configuration (512 uneven configurations, level 16, eager vs HIP-graph
 evaluation).  ``distilled_head`` adds the head configuration:
-distillation of the level-24 MTP into an MLP over level-14 basis values
+distillation of the level-24 MTP into an MLP over level-12 basis values
 (1e5 training environments) and its atom-steps/s and force RMSE on a
 100,000-atom cell against plain level-24 evaluation.  ``roofline`` is the atom kernel's achieved
-algorithmic FP64 FLOP rate (fixed per-atom FLOP formula)
+algorithmic FP64 FLOP rate (the fixed per-atom FLOP formula of ``algorithmic_flops``)
 over its average launch time, measured with HIP events on its own stream.
 ``cpu_baseline`` times the FP64 CPU restatement of the reference's
 full-tensor algorithm on a bounded sample of the same workload on the host
@@ -68,12 +68,13 @@ def traffic_bytes(workload, kernel):
     return float(kv["traffic_bytes"])
 
 
-def head_leg(seed, steps, save_dir=None):
-    """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py)."""
+def head_leg(seed, steps, save_dir=None, desc_level=12):
+    """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py); the
+    head's inputs are the basis values of a level-``desc_level`` MTP."""
     jm = importlib.import_module("jax-mtp_amd")
     res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, desc_steps=40, seed=seed,
-                         decay=0.35, log=log, save_dir=save_dir, desc_level=14)
-    res["desc_level"] = 14
+                         decay=0.35, log=log, save_dir=save_dir, desc_level=desc_level)
+    res["desc_level"] = desc_level
     res["desc_lbfgs_steps"] = 40
     res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
     return res
@@ -307,7 +308,9 @@ def main():
         log(f"[bench] training: {line['training_step']}")
     if not args.no_head:
         torch.cuda.empty_cache()
-        line["distilled_head"] = head_leg(args.seed, args.head_steps, args.head_save)
+        hs = args.head_save
+        line["distilled_head"] = head_leg(args.seed, args.head_steps, hs and f"{hs}/level12", 12)
+        line["distilled_head_level14"] = head_leg(args.seed, args.head_steps, hs and f"{hs}/level14", 14)
     print(json.dumps(line), flush=True)
 
 

@@ -1,13 +1,14 @@
 """Distillation of a level-24 MTP into the head, and its evaluation.
 
 Workload (the distilled-head configuration): a level-24, 3-species MTP is
-the teacher; the head reads level-14 basis values (52 per atom) through a
-2x64 SiLU network; it is trained on 1e5 site environments of thermally
+the teacher; the head reads level-12 basis values (29 per atom; 52 at the
+optional level 14) through a 2x64 SiLU network; it is trained on 1e5 site
+environments of thermally
 displaced 3-species FCC snapshots at 300-1500 K and evaluated on a
 100,000-atom 3-species FCC cell, with exact level-24 re-evaluation of every
 atom outside the training box.  Reported: atom-steps/s of the head (with
 fallback) and of the plain level-24 evaluation, the head's force RMSE
-against the level-24 forces, and the same RMSE for the level-14 MTP alone
+against the level-24 forces, and the same RMSE for the descriptor MTP alone
 (the method without the network).
 
 Synthetic teacher.  All parameters are seeded; there is no trained
@@ -107,10 +108,10 @@ def label(pot: MTPPotential, batch):
 
 
 def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None, level=12):
-    """Low-level MTP (level 14 by default) over the teacher's own invariants.
+    """Low-level MTP (level 12 by default) over the teacher's own invariants.
 
-    A level-L basis uses the radial functions mu with 2 + 4 mu <= L (mu < 4
-    at level 14); they start as the teacher's, so the head's inputs start as
+    A level-L basis uses the radial functions mu with 2 + 4 mu <= L (mu < 3
+    at level 12, mu < 4 at level 14); they start as the teacher's, so the head's inputs start as
     the teacher's own low-level basis values.  Moment coefficients come from linear least
     squares; with steps > 0, L-BFGS then adjusts all coefficients (radial
     ones included) to the teacher's energies and forces.  Alone, this
@@ -153,7 +154,7 @@ def _time(fn, reps=5, warmup=2):
 
 
 def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
-        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=14):
+        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=12):
     """Teacher, training set, descriptor fit, head training, then evaluate();
     with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
     are written there for infer()."""
@@ -253,7 +254,7 @@ def main(argv=None):
     ap.add_argument("--n-env", type=int, default=100_000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--decay", type=float, default=0.35)
-    ap.add_argument("--desc-level", type=int, default=14, help="MTP level of the descriptor")
+    ap.add_argument("--desc-level", type=int, default=12, help="MTP level of the descriptor")
     ap.add_argument("--desc-steps", type=int, default=40, help="L-BFGS steps of the descriptor fit")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", default=None, help="directory for head.npz and teacher.mtp")

@@ -3,13 +3,12 @@
 Workload (the distilled-head configuration): a level-24, 3-species MTP is
 the teacher; the head reads level-12 basis values (29 per atom; 52 at the
 optional level 14) through a 2x64 SiLU network; it is trained on 1e5 site
-environments of thermally
-displaced 3-species FCC snapshots at 300-1500 K and evaluated on a
-100,000-atom 3-species FCC cell, with exact level-24 re-evaluation of every
-atom outside the training box.  Reported: atom-steps/s of the head (with
-fallback) and of the plain level-24 evaluation, the head's force RMSE
-against the level-24 forces, and the same RMSE for the descriptor MTP alone
-(the method without the network).
+environments of thermally displaced 3-species FCC snapshots at 300-1500 K
+and evaluated on a 100,000-atom 3-species FCC cell, with exact level-24
+re-evaluation of every atom outside the training box.  Reported:
+atom-steps/s of the head (with fallback) and of the plain level-24
+evaluation, the head's force RMSE against the level-24 forces, and the same
+RMSE for the descriptor MTP alone (the method without the network).
 
 Synthetic teacher.  All parameters are seeded; there is no trained
 level-24 potential to start from.  Coefficients drawn N(0, 0.1) for every

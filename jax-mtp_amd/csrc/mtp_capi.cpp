@@ -801,7 +801,7 @@ int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const i
         hipError_t e = set_train_kernel_lds(ctx->train_lds);
         const size_t A = (size_t)ctx->max_atoms;
         const size_t ncols = 1 + (size_t)f.species_count * f.radial_funcs_count * f.radial_basis_size;
-        const size_t nchunk = (A + 1023) / 1024;
+        const size_t nchunk = (A + kColsumChunk - 1) / kColsumChunk;
         const size_t part = nchunk * std::max((size_t)f.S(), (size_t)f.species_count * ncols);
         ctx->d_gxi = ctx->alloc<double>(A * (size_t)std::max(1, f.S()), e);
         ctx->d_gc = ctx->alloc<double>(A * ncols, e);

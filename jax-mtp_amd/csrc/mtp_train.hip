@@ -549,7 +549,7 @@ train_kernel(const DevPlan P, TrainArgs A)
 }
 
 // Column sums over row chunks; with rowtype the sums are split by type.
-// partial[chunk][t][col], chunk = 1024 rows, fixed summation order.
+// partial[chunk][t][col], chunk = kColsumChunk rows, fixed summation order.
 __global__ void __launch_bounds__(256) colsum_stage1(const double* X, int rows, int cols, const int* rowtype, int ntypes,
                                                      int Sp, double* partial)
 {
@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(256) colsum_stage1(const double* X, int rows, 
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int lane = threadIdx.x >> 6;
     const int chunk = blockIdx.y;
-    const int r0 = chunk * 1024, r1 = min(rows, r0 + 1024);
+    const int r0 = chunk * kColsumChunk, r1 = min(rows, r0 + kColsumChunk);
     for (int t = 0; t < ntypes; ++t) {
         double s = 0.0;
         if (col < cols)
@@ -575,13 +575,20 @@ __global__ void __launch_bounds__(256) colsum_stage1(const double* X, int rows, 
     }
 }
 
-__global__ void colsum_stage2(const double* partial, int nchunk, int width, double* out)
+__global__ void __launch_bounds__(256) colsum_stage2(const double* partial, int nchunk, int width, double* out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= width) return;
+    // one block per column: strided per-thread sums, then a tree in LDS
+    __shared__ double buf[256];
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int q = 0; q < nchunk; ++q) s += partial[(size_t)q * width + c];
-    out[c] = s;
+    for (int q = threadIdx.x; q < nchunk; q += 256) s += partial[(size_t)q * width + c];
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) buf[threadIdx.x] += buf[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = buf[0];
 }
 
 // grad = [dxi (S) | dspecies (Sp) | dradial (Sp*Sp*R*N)] from the reduced parts.
@@ -669,18 +676,17 @@ hipError_t launch_train_grad(const DevPlan& P, const TrainArgs& A, size_t lds_by
 {
     const int ncols = P.Sp * P.R * P.N;
     hipLaunchKernelGGL(train_kernel, dim3(A.nrows), dim3(kTrainThreads), lds_bytes, stream, P, A);
-    const int nchunk = (A.nrows + 1023) / 1024;
+    const int nchunk = (A.nrows + kColsumChunk - 1) / kColsumChunk;
     // xi columns (no types)
     hipLaunchKernelGGL(colsum_stage1, dim3((P.S + 63) / 64, nchunk), dim3(256), 0, stream, A.gxi_part, A.nrows, P.S,
                        (const int*)nullptr, 1, P.Sp, scratch_partial);
-    hipLaunchKernelGGL(colsum_stage2, dim3((P.S + 255) / 256), dim3(256), 0, stream, scratch_partial, nchunk, P.S,
-                       xi_sum);
+    hipLaunchKernelGGL(colsum_stage2, dim3(P.S), dim3(256), 0, stream, scratch_partial, nchunk, P.S, xi_sum);
     // alpha + radial columns, split by the row's species
     const int w = 1 + ncols;
     hipLaunchKernelGGL(colsum_stage1, dim3((w + 63) / 64, nchunk), dim3(256), 0, stream, A.gc_part, A.nrows, w,
                        A.itypes, P.Sp, P.Sp, scratch_partial);
-    hipLaunchKernelGGL(colsum_stage2, dim3((P.Sp * w + 255) / 256), dim3(256), 0, stream, scratch_partial, nchunk,
-                       P.Sp * w, typed_sum);
+    hipLaunchKernelGGL(colsum_stage2, dim3(P.Sp * w), dim3(256), 0, stream, scratch_partial, nchunk, P.Sp * w,
+                       typed_sum);
     hipLaunchKernelGGL(assemble_grad, dim3(8), dim3(256), 0, stream, xi_sum, typed_sum, P.S, P.Sp, ncols, grad);
     return hipGetLastError();
 }

@@ -22,6 +22,7 @@ struct TrainArgs {
 };
 
 constexpr int kTrainThreads = 256;  // threads per row of the training kernel
+constexpr int kColsumChunk = 128;   // rows per partial sum of the per-atom gradient columns
 
 // LDS of the training kernel.  A moment travels with its tangent and an
 // adjoint with the adjoint of the tangent, as 16-byte pairs, so one entry of

@@ -330,11 +330,20 @@ train_kernel(const DevPlan P, TrainArgs A)
     {
         const int k = tid;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        double dx = gFx, dy = gFy, dz = gFz;
+        int jt = -1;
         bool valid = false;
         if (k < K) {
+            // every load of the slot (the neighbor's gF included) issues
+            // before the compaction's barrier
             v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            jt = jt_row[k];
+            const int j = js_row[k];
+            if (j >= 0 && j < A.nrows) {
+                dx -= A.gF[3 * (size_t)j]; dy -= A.gF[3 * (size_t)j + 1]; dz -= A.gF[3 * (size_t)j + 2];
+            }
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+            valid = (jt >= 0) && (r < P.max_dist);
         }
         const uint64_t mask = __ballot(valid);
         const int wave = tid / kWave;
@@ -344,12 +353,7 @@ train_kernel(const DevPlan P, TrainArgs A)
         for (int w = 0; w < wave; ++w) off += wave_cnt[w];
         const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
         if (valid) {
-            jtc[pos] = jt_row[k];
-            const int j = js_row[k];
-            double dx = gFx, dy = gFy, dz = gFz;
-            if (j >= 0 && j < A.nrows) {
-                dx -= A.gF[3 * (size_t)j]; dy -= A.gF[3 * (size_t)j + 1]; dz -= A.gF[3 * (size_t)j + 2];
-            }
+            jtc[pos] = jt;
             dx += Lxx * v0 + Lxy * v1 + Lxz * v2;
             dy += Lxy * v0 + Lyy * v1 + Lyz * v2;
             dz += Lxz * v0 + Lyz * v1 + Lzz * v2;

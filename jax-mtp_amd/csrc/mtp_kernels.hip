@@ -532,14 +532,30 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     // ---- A: compaction of valid slots in slot order (radial basis below)
     int nv = 0;
     if constexpr (NT < kAtomBlock) {  // slots in chunks of NT (K may exceed NT)
-        for (int k0 = 0; k0 < K; k0 += NT) {
+        // every chunk's loads issue before the first compaction barrier
+        constexpr int MC = kAtomBlock / NT;
+        double c0[MC], c1[MC], c2[MC];
+        int ct[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            const int k = c * NT + tid;
+            c0[c] = 0.0; c1[c] = 0.0; c2[c] = 0.0; ct[c] = -1;
+            if (k < K) {
+                c0[c] = rij_row[3 * k]; c1[c] = rij_row[3 * k + 1]; c2[c] = rij_row[3 * k + 2];
+                ct[c] = jt_row[k];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            const int k0 = c * NT;
+            if (k0 >= K) break;
             const int k = k0 + tid;
-            double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+            const double v0 = c0[c], v1 = c1[c], v2 = c2[c];
+            double r = 1.0;
             bool valid = false;
             if (k < K) {
-                v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
                 r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-                valid = (jt_row[k] >= 0) && (r < P.max_dist);
+                valid = (ct[c] >= 0) && (r < P.max_dist);
             }
             const uint64_t mask = __ballot(valid);
             const int lane = tid & (kWave - 1), wave = tid / kWave;

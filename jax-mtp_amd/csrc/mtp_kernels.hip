@@ -528,6 +528,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int ti = A.itypes[row];
     const double* rij_row = A.rij + (size_t)row * K * 3;
     const int* jt_row = A.jtypes + (size_t)row * K;
+    if (NT <= 256 && A.mom_in) {  // cached moments; visible after the compaction's barriers
+        const double* mi = A.mom_in + (size_t)row * P.n_live;
+        for (int x = tid; x < P.n_live; x += NT) m[x] = mi[x];
+    }
 
     // ---- A: compaction of valid slots in slot order (radial basis below)
     int nv = 0;
@@ -610,13 +614,11 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     // wavefront, so the slot loop is a wave-uniform unrolled instance (lanes
     // of lower degree carry unused mu).
     if (NT <= 256 && A.mom_in) {
-        // moments [0, n_live) of this row as the descriptor pass left them:
-        // no radial basis, basic moments, forward layers or energy here.
-        // (The 512-thread kernel recomputes them: in it this branch costs
-        // registers; the results are the same.)
-        const double* mi = A.mom_in + (size_t)row * P.n_live;
-        for (int x = tid; x < P.n_live; x += NT) m[x] = mi[x];
-        __syncthreads();
+        // moments [0, n_live) of this row as the descriptor pass left them
+        // (copied before the compaction): no radial basis, basic moments,
+        // forward layers or energy here.  (The 512-thread kernel recomputes
+        // them: in it this branch costs registers; the results are the
+        // same.)
     } else {
         const int np1 = P.nu_max + 1;
         const int Kp = P.k_layout | 1;

@@ -279,7 +279,8 @@ __device__ __forceinline__ void dual_slot(const DevPlan& P, int prt, double ux, 
     }
 }
 
-// One row (atom) per workgroup.
+// One row (atom) per workgroup, one thread per slot in the compaction.
+static_assert(kTrainThreads >= kAtomBlock, "rows hold at most kAtomBlock slots");
 __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(3)))
 train_kernel(const DevPlan P, TrainArgs A)
 {

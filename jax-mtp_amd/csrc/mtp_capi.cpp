@@ -50,6 +50,8 @@ struct mtp_ctx {
     DevPlan vdev[kNumVariants]{};         // per-variant device plans
     size_t vlds[kNumVariants] = {};          // LDS bytes per variant, 0 = unavailable
     size_t vlds_xi[kNumVariants] = {};       // the same with per-row coefficient vectors
+    DevPlan vdev_cache[kNumVariants]{};   // per-row pass over cached moments (layout without tables)
+    size_t vlds_cache[kNumVariants] = {};    // its LDS bytes, 0 = unavailable
     int n_cu = 0;
     double* d_eye = nullptr;              // [S][S] identity (basis Jacobians), allocated on first use
     int variant = kVariantV1;
@@ -293,6 +295,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     for (int q = 0; q < 9; ++q) d.part_beg[q] = p.part_beg[q];
     d.part_blk = part_blk;
     d.k_layout = max_nbr;
+    d.lds_tables = 1;
     d.n_mapped = (int)p.map_x.size(); d.map_x = map_x; d.map_beg = map_beg; d.map_s = map_s;
 
     // --- kernel variants ------------------------------------------------
@@ -359,6 +362,31 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                                                    p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
                 if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
                     c->vlds_xi[v] = lxi;
+                // the per-row pass over cached moments (kernels of up to 256
+                // threads take that branch): no power table, its own streams
+                if (c->vlds_xi[v] && nt <= 256) {
+                    const AtomLds2 Lc = atom_lds2_layout(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
+                                                         f.radial_funcs_count, p.nu_max, dv.n_parts, nt, 0,
+                                                         f.alpha_moments_count, false);
+                    StreamLayout layc;
+                    layc.dm_off = Lc.dm;
+                    layc.part_off = Lc.part;
+                    const int budc = std::max(64, (int)(lds_max / 2 / sizeof(double)) - Lc.total_doubles);
+                    ItemPlan ipc;
+                    if (build_items(*pv, nt, layc, budc, ipc, ierr)) {
+                        DevPlan dvc = dv;
+                        upload_streams(c, ipc, dvc, e);
+                        dvc.lds_tables = 0;
+                        const size_t lc = atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
+                                                          f.radial_funcs_count, p.nu_max, dvc.n_parts, nt,
+                                                          ipc.max_partials, f.alpha_moments_count, false);
+                        if (e == hipSuccess && (long long)lc <= (long long)lds_max &&
+                            set_atom_kernel_lds(v, lc) == hipSuccess) {
+                            c->vdev_cache[v] = dvc;
+                            c->vlds_cache[v] = lc;
+                        }
+                    }
+                }
             }
         }
         if (e != hipSuccess) {
@@ -878,7 +906,9 @@ int slot_grad_xi(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const
     if (v < 0) return ctx->fail(MTP_ERR_UNSUPPORTED, "per-row coefficients need a balanced-item kernel variant");
     AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
     A.xi_atom = d_xi; A.xi_stride = xi_stride; A.g = d_g; A.Wi = d_Wi; A.mom_in = d_mom_in;
-    hipError_t e = launch_atoms(ctx->vdev[v], A, ctx->vlds_xi[v], ctx->radial_fp32, s, v, nrows);
+    const bool cached = d_mom_in && ctx->vlds_cache[v];  // the table-less layout needs the cache
+    hipError_t e = launch_atoms(cached ? ctx->vdev_cache[v] : ctx->vdev[v], A,
+                                cached ? ctx->vlds_cache[v] : ctx->vlds_xi[v], ctx->radial_fp32, s, v, nrows);
     if (e != hipSuccess) return ctx->hip_fail(e, "slot-gradient launch");
     return MTP_OK;
 }

@@ -271,10 +271,11 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
 size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
-                       int max_partials, int xi_len)
+                       int max_partials, int xi_len, bool tables)
 {
-    return sizeof(double) *
-           (size_t)atom_lds2_layout(amc, n_live, B, nslots, R, nu_max, n_parts, nt, max_partials, xi_len).total_doubles;
+    return sizeof(double) * (size_t)atom_lds2_layout(amc, n_live, B, nslots, R, nu_max, n_parts, nt, max_partials,
+                                                     xi_len, tables)
+                                .total_doubles;
 }
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
@@ -508,7 +509,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
     const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, P.B, P.k_layout, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
-                                        A.xi_atom ? P.amc : 0);
+                                        A.xi_atom ? P.amc : 0, P.lds_tables != 0);
     if (tid < kLdsM) lds[tid] = 0.0;  // operand of the padding entries, sink of idle lanes
     double* m = lds + L.m;
     double* dm = lds + L.dm;

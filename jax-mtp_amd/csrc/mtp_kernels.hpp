@@ -70,15 +70,18 @@ struct AtomLds2 {
 //               recomputed f, f'; after F, dm[0, 3K) holds the slot
 //               gradients
 //   part, vx, vy, vz, r, xl (per-row coefficients), ints
+// Without tables (a per-row pass over cached moments, which skips phases
+// A's radial values and B) the power table and f are left out.
 __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B, int K, int R, int nu_max,
-                                                    int n_parts, int nt, int max_partials, int xi_len)
+                                                    int n_parts, int nt, int max_partials, int xi_len,
+                                                    bool tables = true)
 {
     AtomLds2 L;
-    const int pw_len = (K | 1) * 3 * (nu_max + 1);
+    const int pw_len = tables ? (K | 1) * 3 * (nu_max + 1) : 0;
     const int mlen = n_live > 0 ? n_live : 1;
     const int g0 = B > 3 * K ? B : 3 * K;
     int dlen = amc;
-    if (dlen < pw_len + R * K) dlen = pw_len + R * K;
+    if (tables && dlen < pw_len + R * K) dlen = pw_len + R * K;
     if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
     if (dlen < g0 + n_parts * K * 3 + 2 * R * K) dlen = g0 + n_parts * K * 3 + 2 * R * K;
     L.m = kLdsM;
@@ -162,7 +165,7 @@ constexpr int variant_parts(int v)
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
 size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
-                       int max_partials, int xi_len = 0);
+                       int max_partials, int xi_len = 0, bool tables = true);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.

@@ -96,6 +96,8 @@ struct DevPlan {
     const int* rev_bo;               // moments without entries (dm = xi)
     const int* rev_xa;               // moments of unsplit rows whose dm gets + xi after the sums
     int k_layout;                    // slots the LDS layout is sized for (>= the slots of any call)
+    int lds_tables;                  // 1: the LDS layout holds the power table and radial values; 0 only
+                                     // for the plans of a per-row pass that reads a moment cache
     int max_partials;
     // inverse of alpha_moment_mapping: distinct mapped moments (increasing)
     // and, per moment, the scalar indices s mapped to it (increasing)

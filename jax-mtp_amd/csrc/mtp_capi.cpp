@@ -68,6 +68,7 @@ struct mtp_ctx {
     // per-call buffers
     double *d_rij = nullptr, *d_Ei = nullptr, *d_g = nullptr, *d_Wi = nullptr, *d_F = nullptr, *d_out = nullptr;
     int *d_it = nullptr, *d_js = nullptr, *d_jt = nullptr, *d_cnt = nullptr, *d_off = nullptr, *d_rev = nullptr;
+    int *d_csr_state = nullptr, *d_js_copy = nullptr;  // reuse of the reverse-neighbor CSR (launch_gather)
     double* d_tot = nullptr;
     // pinned staging
     double* h_rij = nullptr;
@@ -251,6 +252,13 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->d_tot = c->alloc<double>(kTotalsBlocks * 10, e);
     c->d_off = c->alloc<int>(A + 1, e);
     c->d_rev = c->alloc<int>(A * K, e);
+    c->d_csr_state = c->alloc<int>(8, e);
+    c->d_js_copy = c->alloc<int>(A * K, e);
+    if (e == hipSuccess) {  // no previous build yet (valid flag 0)
+        static const int zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        e = hipMemcpyAsync(c->d_csr_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rij, A * K * 3 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_it, A * sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_js, A * K * sizeof(int), hipHostMallocDefault);
@@ -524,6 +532,7 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
     ForceArgs F{};
     F.n_e = natoms_energy; F.n_f = natoms_force; F.cell_rank = cell_rank; F.volume = volume;
     F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.out = d_out; F.tot = ctx->d_tot;
+    F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy;
     hipEvent_t eb = nullptr, ee = nullptr;
     if (ctx->profiling) {
         if (ctx->ev_used == ctx->ev_pool.size()) {
@@ -575,6 +584,7 @@ int mtp_autotune(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const
     ForceArgs F{};
     F.n_e = nrows; F.n_f = nrows; F.cell_rank = 3; F.volume = 1.0;
     F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = ctx->d_F; F.out = ctx->d_out; F.tot = ctx->d_tot;
+    F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy;
     int chosen = ctx->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumVariants; ++v) {
@@ -1035,6 +1045,7 @@ int mtp_gather_forces_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_j
     A.nrows = nrows; A.nslots = nslots; A.js = d_js; A.g = const_cast<double*>(d_g);
     ForceArgs F{};
     F.n_f = n_f; F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.tot = ctx->d_tot;
+    F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy;
     hipError_t e = launch_gather(A, F, stream ? (hipStream_t)stream : ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "force gather");
     return MTP_OK;
@@ -1178,6 +1189,7 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     ForceArgs F{};
     F.n_e = n_e; F.n_f = n_f; F.cell_rank = cell_rank; F.volume = volume;
     F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out; F.tot = D->d_tot;
+    F.csr_state = D->d_csr_state; F.js_copy = D->d_js_copy;
     if (nrows > 0) {
         // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
         {

@@ -1045,13 +1045,40 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
 // a reverse-neighbor CSR is built per call; each atom sums its incoming
 // slots in increasing slot order, so results are bitwise reproducible.
 // ---------------------------------------------------------------------------
-__global__ void zero_ints_kernel(int* p, int n)
+// The reverse-neighbor CSR (below) depends only on js, n_f and the array
+// shape.  csr_key_kernel and csr_compare_kernel set state[0] = 1 when all
+// of them equal those of the previous build (kept in state[1..4] and a copy
+// of js), and the build kernels then return at once: in an MD run js stays
+// the same between neighbor-list rebuilds.  No host synchronisation.
+__global__ void csr_key_kernel(int* state, int nrows, int nslots, int n_f)
 {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        state[0] = (state[4] == 1 && state[1] == nrows && state[2] == nslots && state[3] == n_f) ? 1 : 0;
+        state[1] = nrows; state[2] = nslots; state[3] = n_f; state[4] = 1;
+    }
+}
+
+__global__ void csr_compare_kernel(const int* js, int* js_copy, long long total, int* state)
+{
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < total;
+         s += (long long)gridDim.x * blockDim.x) {
+        const int v = js[s];
+        if (js_copy[s] != v) {
+            js_copy[s] = v;
+            state[0] = 0;
+        }
+    }
+}
+
+__global__ void zero_ints_kernel(int* p, int n, const int* skip)
+{
+    if (skip && *skip) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-__global__ void rev_count_kernel(const int* js, int nrows, int nslots, int n_f, int* cnt)
+__global__ void rev_count_kernel(const int* js, int nrows, int nslots, int n_f, int* cnt, const int* skip)
 {
+    if (skip && *skip) return;
     const long long total = (long long)nrows * nslots;
     for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < total;
          s += (long long)gridDim.x * blockDim.x) {
@@ -1083,8 +1110,9 @@ __device__ __forceinline__ int block_excl_scan_256(int v, int* sh, int& total)
     return incl - v;
 }
 
-__global__ void __launch_bounds__(256) scan_tiles_kernel(const int* cnt, int n, int* tile_sum)
+__global__ void __launch_bounds__(256) scan_tiles_kernel(const int* cnt, int n, int* tile_sum, const int* skip)
 {
+    if (skip && *skip) return;
     __shared__ int sh[256];
     const int base = blockIdx.x * kScanTile;
     int s = 0;
@@ -1097,8 +1125,10 @@ __global__ void __launch_bounds__(256) scan_tiles_kernel(const int* cnt, int n, 
     if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(256) scan_top_kernel(const int* tile_sum, int ntiles, int n, int* tile_off, int* off)
+__global__ void __launch_bounds__(256) scan_top_kernel(const int* tile_sum, int ntiles, int n, int* tile_off, int* off,
+                                                       const int* skip)
 {
+    if (skip && *skip) return;
     __shared__ int sh[256];
     int carry = 0;
     for (int b0 = 0; b0 < ntiles; b0 += 256) {
@@ -1112,8 +1142,10 @@ __global__ void __launch_bounds__(256) scan_top_kernel(const int* tile_sum, int 
     if (threadIdx.x == 0) off[n] = carry;
 }
 
-__global__ void __launch_bounds__(256) scan_apply_kernel(const int* cnt, int n, const int* tile_off, int* off)
+__global__ void __launch_bounds__(256) scan_apply_kernel(const int* cnt, int n, const int* tile_off, int* off,
+                                                         const int* skip)
 {
+    if (skip && *skip) return;
     __shared__ int tile[kScanTile];
     __shared__ int sh[256];
     const int base = blockIdx.x * kScanTile;
@@ -1141,8 +1173,10 @@ __global__ void __launch_bounds__(256) scan_apply_kernel(const int* cnt, int n, 
     }
 }
 
-__global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, const int* off, int* cur, int* rev)
+__global__ void rev_fill_kernel(const int* js, int nrows, int nslots, int n_f, const int* off, int* cur, int* rev,
+                                const int* skip)
 {
+    if (skip && *skip) return;
     const long long total = (long long)nrows * nslots;
     for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < total;
          s += (long long)gridDim.x * blockDim.x) {
@@ -1407,18 +1441,23 @@ hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
     const long long slots = (long long)A.nrows * A.nslots;
     int gs = (int)((slots + 255) / 256);
     if (gs > 65536) gs = 65536;
-    hipLaunchKernelGGL(zero_ints_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, F.cnt, 2 * n);
-    hipLaunchKernelGGL(rev_count_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.cnt);
+    const int* skip = F.csr_state;  // null: always rebuild
+    if (F.csr_state) {
+        hipLaunchKernelGGL(csr_key_kernel, dim3(1), dim3(64), 0, stream, F.csr_state, A.nrows, A.nslots, F.n_f);
+        hipLaunchKernelGGL(csr_compare_kernel, dim3(gs), dim3(256), 0, stream, A.js, F.js_copy, slots, F.csr_state);
+    }
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((2 * n + 255) / 256), dim3(256), 0, stream, F.cnt, 2 * n, skip);
+    hipLaunchKernelGGL(rev_count_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.cnt, skip);
     {
         const int ntiles = (n + kScanTile - 1) / kScanTile;
         int* tile_sum = F.cnt + 2 * n;
         int* tile_off = tile_sum + ntiles;
-        hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_sum);
-        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, stream, tile_sum, ntiles, n, tile_off, F.off);
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_off, F.off);
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_sum, skip);
+        hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, stream, tile_sum, ntiles, n, tile_off, F.off, skip);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(ntiles), dim3(256), 0, stream, F.cnt, n, tile_off, F.off, skip);
     }
     hipLaunchKernelGGL(rev_fill_kernel, dim3(gs), dim3(256), 0, stream, A.js, A.nrows, A.nslots, F.n_f, F.off,
-                       F.cnt + n, F.rev);
+                       F.cnt + n, F.rev, skip);
     hipLaunchKernelGGL(force_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, A.g, A.nrows, A.nslots, F.n_f,
                        F.off, F.rev, F.F);
     return hipGetLastError();

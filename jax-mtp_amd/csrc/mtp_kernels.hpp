@@ -44,6 +44,10 @@ struct ForceArgs {
     double* F;             // [A][3]
     double* out;           // [14]: E, stress6, virial6, status
     double* tot;           // [kTotalsBlocks * 10] scratch of the totals reduction
+    // Optional reuse of the reverse-neighbor CSR (cnt, off, rev) of the
+    // previous call with the same js, n_f and shape (null: rebuild always).
+    int* csr_state;        // [8]: same flag, last nrows, nslots, n_f, valid
+    int* js_copy;          // [A][K] js of the last build
 };
 constexpr int kTotalsBlocks = 128;
 

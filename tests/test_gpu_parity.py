@@ -93,6 +93,29 @@ def test_masks_and_padding_rows(jm, O, rows, variant):
     _check(_run_gpu(jm, w, nb, n_e=17, n_f=15, variant=variant), _run_oracle(O, w, nb, n_e=17, n_f=15))
 
 
+def test_reverse_neighbor_reuse(jm, O):
+    """One potential over a sequence of calls: the reverse-neighbor CSR of the
+    force gather is reused when js, natoms_force and the shape repeat, and
+    rebuilt when any of them changes (a different neighbor list of the same
+    shape, a smaller natoms_force)."""
+    wa = jm.configs.small_cluster(12, 2, 24, seed=31, cell=8.0, cutoff=5.0, max_nbr=64)
+    wb = jm.configs.small_cluster(12, 2, 24, seed=32, cell=8.0, cutoff=5.0, max_nbr=64)
+    wb.params = wa.params
+    na = jm.neighbors.padded_neighbors(wa.pos, wa.types, wa.cell, wa.cutoff, wa.max_nbr, rows=24)
+    nb = jm.neighbors.padded_neighbors(wb.pos, wb.types, wb.cell, wb.cutoff, wb.max_nbr, rows=24)
+    assert na["js"].shape == nb["js"].shape and not np.array_equal(na["js"], nb["js"])
+    pot = jm.MTPPotential(wa.params, max_atoms=24, max_nbr=64)
+    seq = [(wa, na, 24), (wa, na, 24), (wb, nb, 24), (wa, na, 24), (wa, na, 19), (wa, na, 19), (wb, nb, 19)]
+    for q, (w, nbr, n_f) in enumerate(seq):
+        print(f"[case] reuse step {q} n_f={n_f}", flush=True)
+        E, F, s6, extra = pot.calc_energy_forces_stress_padded(nbr["itypes"], nbr["js"], nbr["rij"], nbr["jtypes"], 3,
+                                                               w.volume, 24, n_f, return_all=True)
+        ref = _run_oracle(O, w, nbr, n_e=24, n_f=n_f)
+        assert _rel(F.cpu().numpy(), ref["F"]) <= TOL, q
+        assert abs(float(E.item()) - ref["E"]) <= TOL * np.max(np.abs(ref["eatom"])), q
+    pot.close()
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_fcc32k_level24_full_size(jm, O, variant):
     """The 32,000-atom level-24 workload at full size against the oracle."""

@@ -1440,7 +1440,7 @@ hipError_t launch_gather(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
     if (n <= 0) return hipSuccess;
     const long long slots = (long long)A.nrows * A.nslots;
     int gs = (int)((slots + 255) / 256);
-    if (gs > 65536) gs = 65536;
+    if (gs > 4096) gs = 4096;  // grid-stride loops; keeps the launch of a skipped build cheap
     const int* skip = F.csr_state;  // null: always rebuild
     if (F.csr_state) {
         hipLaunchKernelGGL(csr_key_kernel, dim3(1), dim3(64), 0, stream, F.csr_state, A.nrows, A.nslots, F.n_f);

@@ -341,6 +341,21 @@ def test_gpu_head_training_gradient_and_fit(jm, O):
 
 
 @pytest.mark.gpu
+def test_gpu_head_input_width_limit(jm):
+    """A descriptor whose basis values and species exceed the 64 input lanes
+    (level 16, 3 species: 92 + 3) is refused at open."""
+    rng = np.random.default_rng(11)
+    desc = jm.configs.random_params(16, SP, 8, 2.0, 5.0, rng)
+    S = len(desc.moment_coeffs)
+    assert S + SP > 64
+    hm = jm.head.HeadModel(desc, jm.head.init_theta(min(S, 60), SP, rng), np.zeros(S), np.ones(S), None, None,
+                           np.zeros(SP), 1.0)
+    with pytest.raises(jm.capi.MTPError) as e:
+        jm.head.DistilledHead(hm, None, max_atoms=8, max_nbr=16)
+    assert e.value.code == jm.capi.MTP_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
 def test_gpu_distillation_gain(jm):
     """The trained network against the method without it: on a held-out cell
     the head's force RMSE (vs the level-24 teacher) is well below that of the

@@ -44,6 +44,56 @@ T* dalloc(size_t n, hipError_t& e)
 
 }  // namespace
 
+// Replay of a call's kernel sequence as a HIP graph.  The key is every
+// argument that shapes the sequence (array pointers, sizes, scalars, the
+// kernel variant).  A call whose key equals the previous call's is
+// captured (the first repeat) and then replayed; any other call runs
+// directly and becomes the new reference.  Calls made while the caller's
+// stream is being captured, and profiled calls, run directly.
+struct GraphCache {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    void drop()
+    {
+        if (exec) hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    ~GraphCache() { drop(); }
+};
+
+template <class Issue>
+int run_cached(GraphCache& gc, const void* key, size_t key_len, hipStream_t s, bool direct, Issue issue)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (direct || hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return issue();
+    const unsigned char* k = static_cast<const unsigned char*>(key);
+    const bool same = gc.key.size() == key_len && std::memcmp(gc.key.data(), k, key_len) == 0;
+    if (!same) {
+        gc.drop();
+        gc.key.assign(k, k + key_len);
+        return issue();
+    }
+    if (!gc.exec) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return issue();
+        const int rc = issue();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != MTP_OK || ec != hipSuccess || !g) {
+            if (g) hipGraphDestroy(g);
+            gc.key.clear();
+            return rc != MTP_OK ? rc : issue();
+        }
+        const hipError_t ei = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            gc.exec = nullptr;
+            gc.key.clear();
+            return issue();
+        }
+    }
+    return hipGraphLaunch(gc.exec, s) == hipSuccess ? MTP_OK : MTP_ERR_HIP;
+}
+
 struct mtp_ctx {
     HostPlan plan;
     DevPlan dev{};                        // variant 0 (and the base of the others)
@@ -80,6 +130,8 @@ struct mtp_ctx {
     DevPlan tstreams{};                   // contraction streams of the training kernel (stream fields only)
     double *d_gxi = nullptr, *d_gc = nullptr, *d_part = nullptr, *d_xisum = nullptr, *d_tsum = nullptr;
     int lds_max = 0;
+    GraphCache graph;                     // replay of mtp_compute_padded_device
+    unsigned gen = 0;                     // parameter generation (part of the graph keys)
     // live per-kernel timing (events recorded around each atom-kernel launch)
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -171,6 +223,8 @@ int upload_params(mtp_ctx* c)
     if (e != hipSuccess) return c->hip_fail(e, "parameter upload");
     c->dev.scaling = f.scaling;
     for (auto& v : c->vdev) v.scaling = f.scaling;
+    for (auto& v : c->vdev_cache) v.scaling = f.scaling;
+    ++c->gen;  // kernel arguments changed: recorded graphs are stale
     return MTP_OK;
 }
 
@@ -545,10 +599,26 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
         ee = ctx->ev_pool[ctx->ev_used].second;
         ++ctx->ev_used;
     }
-    hipError_t e = launch_forces(ctx->vdev[ctx->variant], A, F, ctx->vlds[ctx->variant], ctx->radial_fp32 != 0, s,
-                                 eb, ee, ctx->variant);
-    if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
-    return MTP_OK;
+    struct {
+        const void* p[7];
+        int n[7];
+        double volume;
+        hipStream_t s;
+    } key;
+    std::memset(&key, 0, sizeof(key));
+    key.p[0] = d_itypes; key.p[1] = d_js; key.p[2] = d_rijs; key.p[3] = d_jtypes; key.p[4] = d_out;
+    key.p[5] = d_forces; key.p[6] = d_eatom;
+    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = natoms_energy; key.n[4] = natoms_force;
+    key.n[5] = ctx->variant * 2 + (ctx->radial_fp32 != 0);
+    key.n[6] = (int)ctx->gen;
+    key.volume = volume;
+    key.s = s;
+    return run_cached(ctx->graph, &key, sizeof(key), s, ctx->profiling, [&]() -> int {
+        const hipError_t e = launch_forces(ctx->vdev[ctx->variant], A, F, ctx->vlds[ctx->variant],
+                                           ctx->radial_fp32 != 0, s, eb, ee, ctx->variant);
+        if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
+        return MTP_OK;
+    });
 }
 
 int mtp_set_variant(mtp_ctx* ctx, int variant)
@@ -933,6 +1003,7 @@ struct mtp_head {
     std::vector<void*> dptrs;
     double *d_theta = nullptr, *d_wt = nullptr, *d_basis = nullptr, *d_xi = nullptr, *d_E = nullptr;
     double* d_mom = nullptr;  // [max_atoms][n_live] moments of the descriptor pass, reread by the adjoint pass
+    GraphCache graph;         // replay of mtp_head_compute_padded_device
     int *d_rows = nullptr, *d_count = nullptr;
     double* d_part = nullptr;
 
@@ -1190,44 +1261,65 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     F.n_e = n_e; F.n_f = n_f; F.cell_rank = cell_rank; F.volume = volume;
     F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out; F.tot = D->d_tot;
     F.csr_state = D->d_csr_state; F.js_copy = D->d_js_copy;
-    if (nrows > 0) {
-        // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
-        {
-            const int v = mode_variant(D, false);
-            AtomArgs Bd = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
-            Bd.basis = h->d_basis; Bd.fwd_only = 1; Bd.mom_out = h->d_mom;
-            e = launch_atoms(D->vdev[v], Bd, D->vlds[v], D->radial_fp32, s, v, nrows);
-            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("descriptor launch: ") + hipGetErrorString(e));
+    const int vd = nrows > 0 ? mode_variant(D, false) : -1, vx = nrows > 0 ? mode_variant(D, true) : -1;
+    const int vf = X && nrows > 0 ? mode_variant(X, false) : -1;
+    struct {
+        const void* p[9];
+        int n[11];
+        double volume;
+        hipStream_t s;
+    } key;
+    std::memset(&key, 0, sizeof(key));
+    key.p[0] = d_itypes; key.p[1] = d_js; key.p[2] = d_rijs; key.p[3] = d_jtypes; key.p[4] = d_out;
+    key.p[5] = d_forces; key.p[6] = d_eatom; key.p[7] = d_flags; key.p[8] = d_n_fallback;
+    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = n_e; key.n[4] = n_f;
+    key.n[5] = vd; key.n[6] = vx; key.n[7] = vf; key.n[8] = D->radial_fp32;
+    key.n[9] = (int)D->gen; key.n[10] = X ? (int)X->gen : 0;
+    key.volume = volume;
+    key.s = s;
+    return run_cached(h->graph, &key, sizeof(key), s, false, [&]() -> int {
+        if (nrows > 0) {
+            // 1. invariants  2. network: E_i, dE_i/db_i, out-of-box rows
+            {
+                const int v = mode_variant(D, false);
+                AtomArgs Bd = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+                Bd.basis = h->d_basis; Bd.fwd_only = 1; Bd.mom_out = h->d_mom;
+                e = launch_atoms(D->vdev[v], Bd, D->vlds[v], D->radial_fp32, s, v, nrows);
+                if (e != hipSuccess)
+                    return h->fail(MTP_ERR_HIP, std::string("descriptor launch: ") + hipGetErrorString(e));
+            }
+            e = head_zero_count(h->d_count, s);
+            if (e == hipSuccess)
+                e = head_eval(h->net, nrows, d_itypes, h->d_basis, E, h->d_xi, d_flags, X ? h->d_rows : nullptr,
+                              X ? h->d_count : nullptr, s);
+            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
+            // 3. slot gradients of the head energy through the descriptor
+            rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g,
+                              D->d_Wi, s, h->d_mom);
+            if (rc) return h->fail(rc, D->err);
+            // 4. exact energies and slot gradients of the out-of-box rows
+            if (X) {
+                const int v = mode_variant(X, false);
+                AtomArgs B = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
+                B.Ei = E; B.g = D->d_g; B.Wi = D->d_Wi; B.rows = h->d_rows; B.row_count = h->d_count;
+                const int grid = nrows < X->n_cu ? nrows : X->n_cu;
+                e = launch_atoms(X->vdev[v], B, X->vlds[v], false, s, v, grid);
+                if (e != hipSuccess)
+                    return h->fail(MTP_ERR_HIP, std::string("fallback launch: ") + hipGetErrorString(e));
+            }
+            // 5. Newton-3 gather
+            e = launch_gather(A, F, s);
+            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("force gather: ") + hipGetErrorString(e));
         }
-        e = head_zero_count(h->d_count, s);
-        if (e == hipSuccess)
-            e = head_eval(h->net, nrows, d_itypes, h->d_basis, E, h->d_xi, d_flags, X ? h->d_rows : nullptr,
-                          X ? h->d_count : nullptr, s);
-        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
-        // 3. slot gradients of the head energy through the descriptor
-        rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g, D->d_Wi, s,
-                          h->d_mom);
-        if (rc) return h->fail(rc, D->err);
-        // 4. exact energies and slot gradients of the out-of-box rows
-        if (X) {
-            const int v = mode_variant(X, false);
-            AtomArgs B = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
-            B.Ei = E; B.g = D->d_g; B.Wi = D->d_Wi; B.rows = h->d_rows; B.row_count = h->d_count;
-            const int grid = nrows < X->n_cu ? nrows : X->n_cu;
-            e = launch_atoms(X->vdev[v], B, X->vlds[v], false, s, v, grid);
-            if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("fallback launch: ") + hipGetErrorString(e));
+        e = launch_totals(A, F, s);
+        if (e == hipSuccess && d_n_fallback) {
+            if (X && nrows > 0)
+                e = hipMemcpyAsync(d_n_fallback, h->d_count, sizeof(int), hipMemcpyDeviceToDevice, s);
+            else e = head_zero_count(d_n_fallback, s);
         }
-        // 5. Newton-3 gather
-        e = launch_gather(A, F, s);
-        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("force gather: ") + hipGetErrorString(e));
-    }
-    e = launch_totals(A, F, s);
-    if (e == hipSuccess && d_n_fallback) {
-        if (X && nrows > 0) e = hipMemcpyAsync(d_n_fallback, h->d_count, sizeof(int), hipMemcpyDeviceToDevice, s);
-        else e = head_zero_count(d_n_fallback, s);
-    }
-    if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head totals: ") + hipGetErrorString(e));
-    return MTP_OK;
+        if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head totals: ") + hipGetErrorString(e));
+        return MTP_OK;
+    });
 }
 
 }  // extern "C"

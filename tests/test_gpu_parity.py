@@ -93,6 +93,64 @@ def test_masks_and_padding_rows(jm, O, rows, variant):
     _check(_run_gpu(jm, w, nb, n_e=17, n_f=15, variant=variant), _run_oracle(O, w, nb, n_e=17, n_f=15))
 
 
+def test_repeated_calls_graph_replay(jm, O):
+    """Repeated calls with the same arrays run as a recorded HIP graph (the
+    second call records it, later ones replay it).  Replays read the arrays'
+    current contents and parameters: positions changed in place, new moment
+    coefficients and a new scaling all show up, each against the oracle."""
+    import copy
+    import ctypes
+
+    import torch
+
+    w = jm.configs.small_cluster(14, 2, 24, seed=41, cell=8.0, cutoff=5.0, max_nbr=64)
+    nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, w.cutoff, w.max_nbr, rows=24)
+    A, K = nb["js"].shape
+    dev = torch.device("cuda")
+    it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
+    js = torch.as_tensor(nb["js"], dtype=torch.int32, device=dev)
+    rij = torch.as_tensor(nb["rij"], dtype=torch.float64, device=dev)
+    jt = torch.as_tensor(nb["jtypes"], dtype=torch.int32, device=dev)
+    out = torch.empty(16, dtype=torch.float64, device=dev)
+    F = torch.empty((A, 3), dtype=torch.float64, device=dev)
+    pot = jm.MTPPotential(w.params, max_atoms=A, max_nbr=K)
+    L = jm.capi.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def call():
+        jm.capi.check(L.mtp_compute_padded_device(pot.ctx, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(),
+                                                  jt.data_ptr(), 3, w.volume, A, A, out.data_ptr(), F.data_ptr(),
+                                                  None, stream), pot.ctx)
+        torch.cuda.synchronize()
+        return float(out[0].item()), F.cpu().numpy().copy()
+
+    def check(res, params, rij_np, tag):
+        n2 = dict(nb, rij=rij_np)
+        ref = _run_oracle(O, jm.configs.Workload(w.name, params, w.pos, w.types, w.cell, w.cutoff, w.max_nbr), n2)
+        print(f"[case] graph replay {tag}: rel F {_rel(res[1], ref['F']):.3e}", flush=True)
+        assert _rel(res[1], ref["F"]) <= TOL, tag
+        assert abs(res[0] - ref["E"]) <= TOL * np.max(np.abs(ref["eatom"])), tag
+
+    first = call()
+    check(first, w.params, nb["rij"], "eager")
+    for q in range(3):  # records, then replays
+        again = call()
+        assert again[0] == first[0] and np.array_equal(again[1], first[1]), q
+    rng = np.random.default_rng(3)
+    moved = nb["rij"] + np.where(nb["jtypes"][..., None] >= 0, rng.normal(0, 0.02, nb["rij"].shape), 0.0)
+    rij.copy_(torch.as_tensor(moved, dtype=torch.float64, device=dev))
+    check(call(), w.params, moved, "positions changed in place")
+    p2 = copy.deepcopy(w.params)
+    p2.moment_coeffs = np.asarray(p2.moment_coeffs) * 1.5 + 0.01
+    pot.set_params(moment_coeffs=p2.moment_coeffs)
+    check(call(), p2, moved, "moment coefficients")
+    p2.scaling = 1.3
+    pot.set_params(scaling=1.3)
+    check(call(), p2, moved, "scaling")
+    check(call(), p2, moved, "scaling, replayed")
+    pot.close()
+
+
 def test_reverse_neighbor_reuse(jm, O):
     """One potential over a sequence of calls: the reverse-neighbor CSR of the
     force gather is reused when js, natoms_force and the shape repeat, and

@@ -289,6 +289,70 @@ def test_gpu_head_compute_with_fallback(jm, O, exact_level, desc_variant):
 
 
 @pytest.mark.gpu
+def test_gpu_head_graph_replay(jm, O):
+    """Repeated head evaluations with the same arrays replay a recorded HIP
+    graph; results stay equal to the first (eager) call and follow positions
+    changed in place, against the oracle composition."""
+    import torch
+    import head_oracle as HO
+
+    w, nb = _level12_case(jm, seed=7, natoms=48, cell=8.6)
+    rng = np.random.default_rng(7)
+    model = _model(rng, box=False)
+    orc_d = oracle_for(O, w.params)
+    _, _, b = orc_d.compact_xi(nb["itypes"], nb["rij"], nb["jtypes"], np.zeros(S12), grad=False)
+    z = (b - model.mean) * model.inv_std
+    lo = np.tile(np.quantile(z, 0.03, axis=0), (SP, 1))
+    hi = np.tile(np.quantile(z, 0.97, axis=0), (SP, 1))
+    model.lo, model.hi = lo, hi
+    model.lin, model.lin_sc = orc_d._keep["moment"], orc_d._keep["species"]
+    exact_p = jm.configs.random_params(24, SP, 8, 2.0, 5.0, rng)
+    orc_x = oracle_for(O, exact_p)
+    A, K = nb["js"].shape
+    exact = jm.MTPPotential(exact_p, max_atoms=A, max_nbr=K)
+    hm = jm.head.HeadModel(w.params, model.theta, model.mean, model.inv_std, lo, hi, model.e_shift, model.e_scale,
+                           True)
+    head = jm.head.DistilledHead(hm, exact, max_atoms=A, max_nbr=K)
+    dev = torch.device("cuda")
+    it = torch.as_tensor(nb["itypes"], dtype=torch.int32, device=dev)
+    js = torch.as_tensor(nb["js"], dtype=torch.int32, device=dev)
+    rij = torch.as_tensor(nb["rij"], dtype=torch.float64, device=dev)
+    jt = torch.as_tensor(nb["jtypes"], dtype=torch.int32, device=dev)
+    out = torch.empty(14, dtype=torch.float64, device=dev)
+    F = torch.empty((A, 3), dtype=torch.float64, device=dev)
+    nfb = torch.zeros(1, dtype=torch.int32, device=dev)
+    L = jm.capi.lib()
+    st = torch.cuda.current_stream().cuda_stream
+
+    def call():
+        jm.capi.check(L.mtp_head_compute_padded_device(head.h, A, K, it.data_ptr(), js.data_ptr(), rij.data_ptr(),
+                                                       jt.data_ptr(), 3, w.volume, A, A, out.data_ptr(),
+                                                       F.data_ptr(), None, None, nfb.data_ptr(), st), head=head.h)
+        torch.cuda.synchronize()
+        return float(out[0].item()), F.cpu().numpy().copy(), int(nfb.item())
+
+    def check(res, rij_np, tag):
+        E_r, F_r, S_r, eat_r, fl_r, _ = HO.head_compute(model, orc_d, orc_x, nb["itypes"], nb["js"], rij_np,
+                                                         nb["jtypes"], 3, w.volume, A, A)
+        print(f"[case] head graph replay {tag}: rel F {_rel(res[1], F_r):.3e}", flush=True)
+        assert _rel(res[1], F_r) <= 1e-10, tag
+        assert res[0] == pytest.approx(E_r, rel=1e-10), tag
+        assert res[2] == int(((fl_r != 0) & (nb["itypes"] >= 0)).sum()), tag
+
+    first = call()
+    check(first, nb["rij"], "eager")
+    for q in range(3):
+        again = call()
+        assert again[0] == first[0] and np.array_equal(again[1], first[1]) and again[2] == first[2], q
+    moved = nb["rij"] + np.where(nb["jtypes"][..., None] >= 0, rng.normal(0, 0.02, nb["rij"].shape), 0.0)
+    rij.copy_(torch.as_tensor(moved, dtype=torch.float64, device=dev))
+    check(call(), moved, "positions changed in place")
+    check(call(), moved, "positions changed in place, replayed")
+    head.close()
+    exact.close()
+
+
+@pytest.mark.gpu
 def test_gpu_head_training_gradient_and_fit(jm, O):
     """The autograd Function's gradient of the distillation loss against the
     oracle composition, then a short fit that lowers the loss."""

@@ -235,6 +235,50 @@ def test_deterministic(jm):
     np.testing.assert_array_equal(a["s6"], b["s6"])
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_widest_rows(jm, O, variant):
+    """K = 256 slots (the largest row the kernels take), 195-237 neighbors per
+    atom: four slot chunks for the 64-thread kernel, one slot per thread for
+    256.  (The paired-row variants do not fit this row width in LDS and are
+    reported unavailable.)"""
+    rng = np.random.default_rng(12)
+    n, cell = 150, 10.0
+    pos = rng.uniform(0, cell, (n, 3))
+    types = rng.integers(0, 2, n)
+    params = jm.configs.random_params(12, 2, 8, 0.5, 7.0, rng, sigma=0.02)
+    w = jm.configs.Workload("dense150", params, pos, types, np.array([cell] * 3), 7.0, 256)
+    nb = jm.neighbors.padded_neighbors(pos, types, w.cell, 7.0, 256)
+    print(f"[case] widest rows K=256 maxn={nb['counts'].max()} variant={variant}", flush=True)
+    assert nb["counts"].max() > 192
+    _check(_run_gpu(jm, w, nb, variant=variant), _run_oracle(O, w, nb))
+
+
+def test_empty_and_single_slot(jm, O):
+    """No rows (energy = 0, nothing launched but the totals) and rows with a single neighbor."""
+    import torch
+
+    w = jm.configs.small_cluster(10, 1, 6, seed=5, cell=6.0, cutoff=5.0, max_nbr=40)
+    pot = jm.MTPPotential(w.params, max_atoms=8, max_nbr=8)
+    dev = torch.device("cuda")
+    z = lambda shape, dt: torch.zeros(shape, dtype=dt, device=dev)
+    E, F, s6 = pot.calc_energy_forces_stress_padded(z((0,), torch.int32), z((0, 1), torch.int32),
+                                                    z((0, 1, 3), torch.float64), z((0, 1), torch.int32), 3,
+                                                    w.volume, 0, 0)
+    assert float(E.item()) == 0.0 and F.shape[0] == 0
+    pot.close()
+    # rows of two slots (the smallest width the pair style allows), one neighbor and one empty slot
+    nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, 5.0, 40)
+    assert nb["counts"].max() < 40  # slot 39 is empty in every row
+    keep = np.argmax(nb["jtypes"] >= 0, axis=1)
+    A = nb["js"].shape[0]
+    cols = np.stack([keep, np.full(A, 39)], axis=1)
+    rows = np.arange(A)[:, None]
+    one = dict(nb, js=nb["js"][rows, cols].copy(), jtypes=nb["jtypes"][rows, cols].copy(),
+               rij=nb["rij"][rows, cols].copy())
+    w1 = jm.configs.Workload(w.name, w.params, w.pos, w.types, w.cell, w.cutoff, 2)
+    _check(_run_gpu(jm, w1, one), _run_oracle(O, w1, one))
+
+
 def test_capacity_errors(jm):
     w = jm.configs.small_cluster(10, 1, 8, seed=1, cell=6.0, cutoff=5.0, max_nbr=40)
     nb = w.neighbors()

@@ -980,12 +980,13 @@ AtomArgs row_args(int nrows, int nslots, const int* it, const double* rij, const
 
 int slot_grad_xi(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const double* d_rijs, const int* d_jtypes,
                  const double* d_xi, int xi_stride, double* d_g, double* d_Wi, hipStream_t s,
-                 const double* d_mom_in = nullptr)
+                 const double* d_mom_in = nullptr, const int* d_skip_row = nullptr)
 {
     const int v = mode_variant(ctx, true);
     if (v < 0) return ctx->fail(MTP_ERR_UNSUPPORTED, "per-row coefficients need a balanced-item kernel variant");
     AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
     A.xi_atom = d_xi; A.xi_stride = xi_stride; A.g = d_g; A.Wi = d_Wi; A.mom_in = d_mom_in;
+    A.skip_row = d_skip_row;
     const bool cached = d_mom_in && ctx->vlds_cache[v];  // the table-less layout needs the cache
     hipError_t e = launch_atoms(cached ? ctx->vdev_cache[v] : ctx->vdev[v], A,
                                 cached ? ctx->vlds_cache[v] : ctx->vlds_xi[v], ctx->radial_fp32, s, v, nrows);
@@ -1004,6 +1005,9 @@ struct mtp_head {
     double *d_theta = nullptr, *d_wt = nullptr, *d_basis = nullptr, *d_xi = nullptr, *d_E = nullptr;
     double* d_mom = nullptr;  // [max_atoms][n_live] moments of the descriptor pass, reread by the adjoint pass
     GraphCache graph;         // replay of mtp_head_compute_padded_device
+    int* d_exact_mask = nullptr;      // [max_atoms] rows re-evaluated exactly (skipped by the adjoint pass)
+    hipStream_t side = nullptr;       // the exact re-evaluation runs here, beside the adjoint pass
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int *d_rows = nullptr, *d_count = nullptr;
     double* d_part = nullptr;
 
@@ -1023,6 +1027,11 @@ struct mtp_head {
     ~mtp_head()
     {
         if (desc && desc->stream) hipStreamSynchronize(desc->stream);
+        if (side) hipStreamSynchronize(side);
+        graph.drop();
+        if (ev_fork) hipEventDestroy(ev_fork);
+        if (ev_join) hipEventDestroy(ev_join);
+        if (side) hipStreamDestroy(side);
         for (void* p : dptrs) hipFree(p);
     }
 };
@@ -1163,6 +1172,10 @@ int mtp_head_open(mtp_ctx* desc, mtp_ctx* exact, const double* theta, const doub
     double* d_shift = h->alloc<double>(Sp, e);
     h->d_basis = h->alloc<double>(A * S, e);
     h->d_mom = h->alloc<double>(A * (size_t)std::max(1, desc->plan.n_live), e);
+    h->d_exact_mask = h->alloc<int>(A, e);
+    if (e == hipSuccess && exact) e = hipStreamCreate(&h->side);  // default priority
+    if (e == hipSuccess && exact) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess && exact) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
     h->d_xi = h->alloc<double>(A * S, e);
     h->d_E = h->alloc<double>(A, e);
     h->d_rows = h->alloc<int>(A, e);
@@ -1291,21 +1304,29 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
             e = head_zero_count(h->d_count, s);
             if (e == hipSuccess)
                 e = head_eval(h->net, nrows, d_itypes, h->d_basis, E, h->d_xi, d_flags, X ? h->d_rows : nullptr,
-                              X ? h->d_count : nullptr, s);
+                              X ? h->d_count : nullptr, s, X ? h->d_exact_mask : nullptr);
             if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("head launch: ") + hipGetErrorString(e));
-            // 3. slot gradients of the head energy through the descriptor
-            rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g,
-                              D->d_Wi, s, h->d_mom);
-            if (rc) return h->fail(rc, D->err);
-            // 4. exact energies and slot gradients of the out-of-box rows
+            // 3. exact energies and slot gradients of the out-of-box rows, on
+            // the side stream (a handful of workgroups) while 4. the
+            // descriptor's adjoint gives the slot gradients of every other row
             if (X) {
+                e = hipEventRecord(h->ev_fork, s);
+                if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
                 const int v = mode_variant(X, false);
                 AtomArgs B = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
                 B.Ei = E; B.g = D->d_g; B.Wi = D->d_Wi; B.rows = h->d_rows; B.row_count = h->d_count;
                 const int grid = nrows < X->n_cu ? nrows : X->n_cu;
-                e = launch_atoms(X->vdev[v], B, X->vlds[v], false, s, v, grid);
+                if (e == hipSuccess) e = launch_atoms(X->vdev[v], B, X->vlds[v], false, h->side, v, grid);
+                if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);
                 if (e != hipSuccess)
                     return h->fail(MTP_ERR_HIP, std::string("fallback launch: ") + hipGetErrorString(e));
+            }
+            rc = slot_grad_xi(D, nrows, nslots, d_itypes, d_rijs, d_jtypes, h->d_xi, D->plan.file.S(), D->d_g,
+                              D->d_Wi, s, h->d_mom, X ? h->d_exact_mask : nullptr);
+            if (rc) return h->fail(rc, D->err);
+            if (X) {
+                e = hipStreamWaitEvent(s, h->ev_join, 0);
+                if (e != hipSuccess) return h->fail(MTP_ERR_HIP, std::string("join: ") + hipGetErrorString(e));
             }
             // 5. Newton-3 gather
             e = launch_gather(A, F, s);

@@ -119,7 +119,7 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c)
 
 __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu(2))) head_eval_kernel(const HeadNet Hn, int n, const int* itypes,
                                                                const double* basis, double* E, double* xi,
-                                                               int* flags, int* rows, int* count)
+                                                               int* flags, int* rows, int* count, int* exact_mask)
 {
     extern __shared__ double w[];
     double* W1s = w;               // W1[i][h] at i * kWS + h
@@ -266,6 +266,7 @@ __global__ void __launch_bounds__(kEvalBlock) __attribute__((amdgpu_waves_per_eu
             E[a] = Hn.e_shift[ts] + Hn.e_scale * (net + b3) + lin_e;
             if (flags) flags[a] = o;
             if (count && o && t >= 0) rows[atomicAdd(count, 1)] = a;
+            if (exact_mask) exact_mask[a] = (o && t >= 0) ? 1 : 0;
         }
     }
 }
@@ -416,7 +417,7 @@ hipError_t head_zero_count(int* count, hipStream_t s)
 }
 
 hipError_t head_eval(const HeadNet& Hn, int n, const int* itypes, const double* basis, double* E, double* xi,
-                     int* flags, int* rows, int* count, hipStream_t s)
+                     int* flags, int* rows, int* count, hipStream_t s, int* exact_mask)
 {
     if (n <= 0) return hipSuccess;
     static bool attr = false;
@@ -431,7 +432,7 @@ hipError_t head_eval(const HeadNet& Hn, int n, const int* itypes, const double* 
     int grid = (ngroups + per_block - 1) / per_block;
     if (grid > 1024) grid = 1024;
     hipLaunchKernelGGL(head_eval_kernel, dim3(grid), dim3(kEvalBlock), kEvalLdsDoubles * sizeof(double), s, Hn, n,
-                       itypes, basis, E, xi, flags, rows, count);
+                       itypes, basis, E, xi, flags, rows, count, exact_mask);
     return hipGetLastError();
 }
 

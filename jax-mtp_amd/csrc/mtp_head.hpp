@@ -42,8 +42,10 @@ hipError_t head_prepare(const HeadNet& H, double* wt, hipStream_t s);
 // E_i, dE_i/db_i and the out-of-box flag of rows [0, n).  flags/count may be
 // null; with count, flagged rows (itype >= 0 only) are appended to `rows`
 // (order unspecified; *count must be zero on entry: head_zero_count).
+// exact_mask (may be null): per row 1 if the row is appended to rows (its
+// inputs leave the training box and its type is valid), else 0.
 hipError_t head_eval(const HeadNet& H, int n, const int* itypes, const double* basis, double* E, double* xi,
-                     int* flags, int* rows, int* count, hipStream_t s);
+                     int* flags, int* rows, int* count, hipStream_t s, int* exact_mask = nullptr);
 hipError_t head_zero_count(int* count, hipStream_t s);
 
 // d/dtheta of sum_i [alpha_i E_i + v_i . dE_i/db_i]  (v_i in basis units).

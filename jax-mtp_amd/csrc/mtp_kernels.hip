@@ -505,6 +505,7 @@ __device__ __forceinline__ void basic_moments(const DevPlan& P, int w, int lane,
 template <typename RadialReal, int NT>
 __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A, const int row, double* lds)
 {
+    if (A.skip_row && A.skip_row[row]) return;  // the whole workgroup: no barrier reached yet
     const int tid = threadIdx.x;
     const int K = A.nslots;
     constexpr int NW = NT / kWave;

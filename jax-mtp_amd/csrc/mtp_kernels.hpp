@@ -33,6 +33,7 @@ struct AtomArgs {
     // than 256 threads recompute instead).
     double* mom_out;
     const double* mom_in;
+    const int* skip_row;   // rows with skip_row[row] != 0 are left to another kernel (null: none)
 };
 
 struct ForceArgs {

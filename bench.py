@@ -68,13 +68,15 @@ def traffic_bytes(workload, kernel):
     return float(kv["traffic_bytes"])
 
 
-def head_leg(seed, steps, save_dir=None, desc_level=12):
+def head_leg(seed, steps, save_dir=None, desc_level=12, pair_mus=()):
     """Distilled head vs plain level-24 on the 100,000-atom cell (see jax-mtp_amd/distill.py); the
-    head's inputs are the basis values of a level-``desc_level`` MTP."""
+    head's inputs are the basis values of a level-``desc_level`` MTP, plus the pair moments
+    sum_k f_mu(r_k) of ``pair_mus``."""
     jm = importlib.import_module("jax-mtp_amd")
     res = jm.distill.run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=steps, desc_steps=40, seed=seed,
-                         decay=0.35, log=log, save_dir=save_dir, desc_level=desc_level)
+                         decay=0.35, log=log, save_dir=save_dir, desc_level=desc_level, pair_mus=pair_mus)
     res["desc_level"] = desc_level
+    res["desc_pair_mus"] = list(pair_mus)
     res["desc_lbfgs_steps"] = 40
     res["teacher"] = "synthetic level-24, 3 species, basis functions RMS-normalised and damped 0.35^((level-2)/2)"
     return res
@@ -319,6 +321,8 @@ def main():
         torch.cuda.empty_cache()
         hs = args.head_save
         line["distilled_head"] = head_leg(args.seed, args.head_steps, hs and f"{hs}/level12", 12)
+        line["distilled_head_level12_pairs"] = head_leg(args.seed, args.head_steps, hs and f"{hs}/level12_pairs", 12,
+                                                        (3, 4, 5))
         line["distilled_head_level14"] = head_leg(args.seed, args.head_steps, hs and f"{hs}/level14", 14)
     print(json.dumps(line), flush=True)
 

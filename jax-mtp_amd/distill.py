@@ -106,7 +106,38 @@ def label(pot: MTPPotential, batch):
     return extra["eatom"].clone(), F.clone(), E_c.clone(), S_c.clone()
 
 
-def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None, level=12):
+def with_pair_invariants(p: MTPData, mus) -> MTPData:
+    """p with the pair moments M_{mu,0} = sum_k f_mu(r_k) of the given mu added
+    as basic moments and scalar basis functions (inserted after the basic
+    moments; later moment indices shift)."""
+    mus = [int(m) for m in mus]
+    k = len(mus)
+    if k == 0:
+        return p
+    B = int(p.alpha_index_basic_count)
+    basic = np.asarray(p.alpha_index_basic, dtype=np.int64).reshape(-1, 4)
+    times = np.asarray(p.alpha_index_times, dtype=np.int64).reshape(-1, 4).copy()
+    for col in (0, 1, 3):
+        times[:, col] = np.where(times[:, col] >= B, times[:, col] + k, times[:, col])
+    mapping = np.asarray(p.alpha_moment_mapping, dtype=np.int64)
+    mapping = np.concatenate([np.where(mapping >= B, mapping + k, mapping), B + np.arange(k)])
+    p.alpha_index_basic = np.concatenate([basic, np.array([[m, 0, 0, 0] for m in mus], dtype=np.int64)])
+    p.alpha_index_basic_count = B + k
+    p.alpha_moments_count = int(p.alpha_moments_count) + k
+    p.alpha_index_times = times
+    p.alpha_moment_mapping = mapping
+    p.alpha_scalar_moments = int(p.alpha_scalar_moments) + k
+    p.moment_coeffs = np.concatenate([np.asarray(p.moment_coeffs, dtype=np.float64), np.zeros(k)])
+    R0, R = int(p.radial_funcs_count), max(int(p.radial_funcs_count), max(mus) + 1)
+    if R > R0:
+        rc = np.asarray(p.radial_coeffs, dtype=np.float64)
+        pad = np.zeros(rc.shape[:2] + (R - R0,) + rc.shape[3:])
+        p.radial_coeffs = np.concatenate([rc, pad], axis=2)
+    p.radial_funcs_count = R
+    return p
+
+
+def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=None, level=12, pair_mus=()):
     """Low-level MTP (level 12 by default) over the teacher's own invariants.
 
     A level-L basis uses the radial functions mu with 2 + 4 mu <= L (mu < 3
@@ -118,6 +149,7 @@ def fit_descriptor(teacher: MTPData, batch, E_c, F, S_c, seed=0, steps=0, log=No
     rng = np.random.default_rng(seed)
     p = configs.random_params(level, teacher.species_count, teacher.radial_basis_size, teacher.min_dist,
                               teacher.max_dist, rng, scaling=teacher.scaling)
+    p = with_pair_invariants(p, pair_mus)
     R = p.radial_funcs_count
     p.radial_coeffs = np.array(teacher.radial_coeffs)[:, :, :R, :].copy()
     p.species_coeffs = np.array(teacher.species_coeffs).copy()
@@ -153,7 +185,7 @@ def _time(fn, reps=5, warmup=2):
 
 
 def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
-        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=12):
+        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=12, pair_mus=()):
     """Teacher, training set, descriptor fit, head training, then evaluate();
     with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
     are written there for infer()."""
@@ -168,7 +200,7 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
     E_site, F, E_c, S_c = label(tpot, batch)
     tpot.close()
     desc, hist_d = fit_descriptor(teacher, batch, E_c, F, S_c, seed=seed, steps=desc_steps, log=log,
-                                  level=desc_level)
+                                  level=desc_level, pair_mus=pair_mus)
     tr = HeadTrainer(desc, batch, E_site, F, weights=head_weights, margin=0.05, seed=seed)
     t0 = time.time()
     hist = tr.fit(steps=head_steps, lr=head_lr, log_every=max(1, head_steps // 10))
@@ -254,6 +286,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--decay", type=float, default=0.35)
     ap.add_argument("--desc-level", type=int, default=12, help="MTP level of the descriptor")
+    ap.add_argument("--pair-mus", default="", help="comma-separated mu of pair moments added to the descriptor")
     ap.add_argument("--desc-steps", type=int, default=40, help="L-BFGS steps of the descriptor fit")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", default=None, help="directory for head.npz and teacher.mtp")
@@ -264,7 +297,8 @@ def main(argv=None):
         res = infer(a.infer, reps=a.reps)
     else:
         res = run(n_env=a.n_env, head_steps=a.steps, desc_steps=a.desc_steps, decay=a.decay, seed=a.seed,
-                  save_dir=a.save, desc_level=a.desc_level)
+                  save_dir=a.save, desc_level=a.desc_level,
+                  pair_mus=tuple(int(m) for m in a.pair_mus.split(",") if m))
     print(json.dumps(res), flush=True)
 
 

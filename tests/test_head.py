@@ -158,6 +158,29 @@ def test_oracle_per_row_coefficients(jm, O):
     np.testing.assert_allclose(np.einsum("as,askc->akc", xi, J), g, rtol=1e-11, atol=1e-12 * np.abs(g).max())
 
 
+def test_pair_invariants_descriptor(jm, O):
+    """Descriptor with pair moments M_{mu,0} = sum_k f_mu(r_k) (mu = 3, 4, 5)
+    appended to the level-12 basis: its first 29 basis values are those of the
+    plain level-12 descriptor (the moment renumbering is right), and each new
+    one equals the single basis value of a level-2 potential whose radial
+    function 0 is f_mu (a pair sum by construction)."""
+    rng = np.random.default_rng(21)
+    base = jm.configs.random_params(12, SP, 8, 2.0, 5.0, rng)
+    aug = jm.distill.with_pair_invariants(jm.parse_mtp_text(jm.format_mtp(base)), (3, 4, 5))
+    aug.radial_coeffs = rng.normal(0, 0.3, np.asarray(aug.radial_coeffs).shape)
+    base.radial_coeffs = aug.radial_coeffs[:, :, :3, :].copy()
+    w, nb = _level12_case(jm, seed=21, natoms=20, cell=8.0)
+    args = (nb["itypes"], nb["rij"], nb["jtypes"])
+    _, _, b_base = oracle_for(O, base).compact_xi(*args, np.zeros(S12), grad=False)
+    _, _, b_aug = oracle_for(O, aug).compact_xi(*args, np.zeros(S12 + 3), grad=False)
+    np.testing.assert_allclose(b_aug[:, :S12], b_base, rtol=1e-13, atol=1e-13 * np.abs(b_base).max())
+    for q, mu in enumerate((3, 4, 5)):
+        pair = jm.configs.random_params(2, SP, 8, 2.0, 5.0, rng)
+        pair.radial_coeffs = aug.radial_coeffs[:, :, mu:mu + 1, :].copy()
+        _, _, b_pair = oracle_for(O, pair).compact_xi(*args, np.zeros(1), grad=False)
+        np.testing.assert_allclose(b_aug[:, S12 + q], b_pair[:, 0], rtol=1e-13, atol=1e-13 * np.abs(b_pair).max())
+
+
 def test_head_model_roundtrip(jm, tmp_path):
     rng = np.random.default_rng(4)
     desc = jm.configs.random_params(12, SP, 8, 2.0, 5.0, rng)

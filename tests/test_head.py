@@ -205,16 +205,23 @@ def _rel(a, r):
 
 
 @pytest.mark.gpu
-def test_gpu_descriptor_jacobian_gather(jm, O):
+@pytest.mark.parametrize("pairs", [False, True])
+def test_gpu_descriptor_jacobian_gather(jm, O, pairs):
+    """Level-12 descriptor; with pairs, the level-12 basis plus the pair moments of mu = 3, 4, 5 (six
+    radial functions, a non-standard alpha table)."""
     import torch
 
     w, nb = _level12_case(jm, seed=5, natoms=40, cell=8.2)
-    print(f"[case] level-12 descriptor natoms={w.natoms}", flush=True)
+    if pairs:
+        w.params = jm.distill.with_pair_invariants(w.params, (3, 4, 5))
+        w.params.radial_coeffs = np.random.default_rng(55).normal(0, 0.2, np.asarray(w.params.radial_coeffs).shape)
+    S = len(w.params.moment_coeffs)
+    print(f"[case] level-12 descriptor natoms={w.natoms} pairs={pairs} S={S}", flush=True)
     orc = oracle_for(O, w.params)
     pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=w.max_nbr)
     A = w.natoms
     rng = np.random.default_rng(5)
-    xi = rng.normal(0, 1, (A, S12))
+    xi = rng.normal(0, 1, (A, S))
     _, g_ref, b_ref = orc.compact_xi(nb["itypes"], nb["rij"], nb["jtypes"], xi)
     b = pot.descriptor(nb["itypes"], nb["rij"], nb["jtypes"]).cpu().numpy()
     assert _rel(b, b_ref) <= 1e-12

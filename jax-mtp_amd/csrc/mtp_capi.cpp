@@ -358,6 +358,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.part_blk = part_blk;
     d.k_layout = max_nbr;
     d.lds_tables = 1;
+    d.coef_lds = 0;
     d.n_mapped = (int)p.map_x.size(); d.map_x = map_x; d.map_beg = map_beg; d.map_s = map_s;
 
     // --- kernel variants ------------------------------------------------
@@ -410,26 +411,42 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) continue;  // variant unavailable
             upload_streams(c, ip, dv, e);
             dv.k_layout = max_nbr;
+            if (!paired && nt <= 256) {
+                // kernels of up to 256 threads take several rounds of their
+                // radial loops; they stage the row species' radial
+                // coefficients in LDS when that costs no workgroup per CU
+                const int coef = f.species_count * f.radial_funcs_count * f.radial_basis_size;
+                const auto bytes = [&](int cl) {
+                    return atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr, f.radial_funcs_count,
+                                           p.nu_max, dv.n_parts, nt, ip.max_partials, 0, true, cl);
+                };
+                set_atom_kernel_lds(v, bytes(coef));  // the occupancy query needs the LDS size allowed
+                const int occ0 = atom_kernel_blocks_per_cu(v, bytes(0));
+                if (occ0 > 0 && atom_kernel_blocks_per_cu(v, bytes(coef)) >= occ0) dv.coef_lds = coef;
+            }
             const size_t lds = paired ? sizeof(double) * (size_t)atom_lds_pair_layout(
                                                 f.alpha_moments_count, p.n_live, max_nbr, f.radial_funcs_count,
                                                 p.nu_max, dv.n_parts, nt, ip.max_partials).total_doubles
                                       : atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
                                                         f.radial_funcs_count, p.nu_max, dv.n_parts, nt,
-                                                        ip.max_partials);
+                                                        ip.max_partials, 0, true, dv.coef_lds);
             if (e == hipSuccess && (long long)lds <= (long long)lds_max && set_atom_kernel_lds(v, lds) == hipSuccess) {
                 c->vdev[v] = dv;
                 c->vlds[v] = lds;
                 if (paired) continue;  // plain force evaluation only
                 const size_t lxi = atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr, f.radial_funcs_count,
-                                                   p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count);
+                                                   p.nu_max, dv.n_parts, nt, ip.max_partials, f.alpha_moments_count,
+                                                   true, dv.coef_lds);
                 if ((long long)lxi <= (long long)lds_max && set_atom_kernel_lds(v, lxi) == hipSuccess)
                     c->vlds_xi[v] = lxi;
                 // the per-row pass over cached moments (kernels of up to 256
                 // threads take that branch): no power table, its own streams
                 if (c->vlds_xi[v] && nt <= 256) {
+                    // (radial coefficients from global memory: staging them
+                    // made this pass slower in measurement)
                     const AtomLds2 Lc = atom_lds2_layout(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
                                                          f.radial_funcs_count, p.nu_max, dv.n_parts, nt, 0,
-                                                         f.alpha_moments_count, false);
+                                                         f.alpha_moments_count, false, 0);
                     StreamLayout layc;
                     layc.dm_off = Lc.dm;
                     layc.part_off = Lc.part;
@@ -439,9 +456,11 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                         DevPlan dvc = dv;
                         upload_streams(c, ipc, dvc, e);
                         dvc.lds_tables = 0;
+                        dvc.coef_lds = 0;
                         const size_t lc = atom_lds2_bytes(f.alpha_moments_count, p.n_live, f.B(), max_nbr,
                                                           f.radial_funcs_count, p.nu_max, dvc.n_parts, nt,
-                                                          ipc.max_partials, f.alpha_moments_count, false);
+                                                          ipc.max_partials, f.alpha_moments_count, false,
+                                                          dvc.coef_lds);
                         if (e == hipSuccess && (long long)lc <= (long long)lds_max &&
                             set_atom_kernel_lds(v, lc) == hipSuccess) {
                             c->vdev_cache[v] = dvc;

@@ -66,13 +66,13 @@ __device__ __forceinline__ void radial_slot(const DevPlan& P, double r, int ti, 
     }
 }
 
-// One radial function of one slot: f_mu(r) and d f_mu / d r, with the same
-// operations as radial_slot for that mu (so the results are identical).  The
-// coefficient loads of a block of eight Chebyshev terms are independent and
+// One radial function of one slot from its N Chebyshev coefficients c:
+// f_mu(r) and d f_mu / d r, with the same operations as radial_slot for that
+// mu (so the results are identical).  The coefficient loads of a block of eight Chebyshev terms are independent and
 // issue together; one thread per (slot, mu) spreads the basis over a
 // workgroup.
 template <typename Real>
-__device__ __forceinline__ void radial_one(const DevPlan& P, double r, int ti, int tk, int mu, double& f, double& df)
+__device__ __forceinline__ void radial_one_c(const DevPlan& P, double r, const double* c, double& f, double& df)
 {
     const Real rmin = (Real)P.min_dist, rmax = (Real)P.max_dist;
     const Real rr = (Real)r;
@@ -81,8 +81,6 @@ __device__ __forceinline__ void radial_one(const DevPlan& P, double r, int ti, i
     const Real dxdr = (Real)2 * range_inv;
     const Real s = (rmax - rr) * (rmax - rr);
     const Real ds = (Real)-2 * (rmax - rr);
-    const double* c =
-        P.radial_coeffs + ((size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R + mu) * P.N;
     Real af = (Real)c[0], ad = 0;
     if (P.N > 1) {
         af += (Real)c[1] * x;
@@ -100,6 +98,16 @@ __device__ __forceinline__ void radial_one(const DevPlan& P, double r, int ti, i
     const Real sc = (Real)P.scaling;
     f = (double)(sc * s * af);
     df = (double)(sc * (ds * af + s * ad * dxdr));
+}
+
+// The same with the coefficients of (ti, tk, mu) read from the plan's global table.
+template <typename Real>
+__device__ __forceinline__ void radial_one(const DevPlan& P, double r, int ti, int tk, int mu, double& f, double& df)
+{
+    radial_one_c<Real>(P, r,
+                       P.radial_coeffs +
+                           ((size_t)(wrap_species(ti, P.Sp) * P.Sp + wrap_species(tk, P.Sp)) * P.R + mu) * P.N,
+                       f, df);
 }
 
 // FP64 radial basis of one slot with the per-coefficient factors used by the

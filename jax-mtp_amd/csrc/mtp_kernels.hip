@@ -91,11 +91,13 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
     {
         const int k = tid;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        int tk = -1;
         bool valid = false;
         if (k < K) {
             v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            tk = jt_row[k];
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+            valid = (tk >= 0) && (r < P.max_dist);
         }
         const uint64_t mask = __ballot(valid);
         const int lane = tid & (kWave - 1), wave = tid / kWave;
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
         if (k < K) cidx[k] = valid ? pos : -1;
         if (valid) {
             double fv[kMaxR], dfv[kMaxR];
-            radial_slot<RadialReal>(P, r, ti, jt_row[k], fv, dfv);
+            radial_slot<RadialReal>(P, r, ti, tk, fv, dfv);
 #pragma unroll
             for (int mu = 0; mu < kMaxR; ++mu)
                 if (mu < P.R) { f[mu * K + pos] = fv[mu]; fp[mu * K + pos] = dfv[mu]; }
@@ -271,10 +273,10 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
 // entry packs both operand indices and the multiplier.
 // ---------------------------------------------------------------------------
 size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
-                       int max_partials, int xi_len, bool tables)
+                       int max_partials, int xi_len, bool tables, int coef_len)
 {
     return sizeof(double) * (size_t)atom_lds2_layout(amc, n_live, B, nslots, R, nu_max, n_parts, nt, max_partials,
-                                                     xi_len, tables)
+                                                     xi_len, tables, coef_len)
                                 .total_doubles;
 }
 
@@ -510,7 +512,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int K = A.nslots;
     constexpr int NW = NT / kWave;
     const AtomLds2 L = atom_lds2_layout(P.amc, P.n_live, P.B, P.k_layout, P.R, P.nu_max, P.n_parts, NT, P.max_partials,
-                                        A.xi_atom ? P.amc : 0, P.lds_tables != 0);
+                                        A.xi_atom ? P.amc : 0, P.lds_tables != 0, P.coef_lds);
     if (tid < kLdsM) lds[tid] = 0.0;  // operand of the padding entries, sink of idle lanes
     double* m = lds + L.m;
     double* dm = lds + L.dm;
@@ -522,7 +524,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* rr = lds + L.r;
     double* red = lds + L.red;
     int* ints = reinterpret_cast<int*>(lds + L.ints);
-    int* slot_of = ints;
+    int* nbr_t = ints;  // [K] compact index -> neighbour species (the radial basis's only input from the slot)
     int* cidx = ints + K;
     int* wave_cnt = ints + 2 * K;
     int* nvp = wave_cnt + NW;
@@ -530,6 +532,13 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     const int ti = A.itypes[row];
     const double* rij_row = A.rij + (size_t)row * K * 3;
     const int* jt_row = A.jtypes + (size_t)row * K;
+    // radial coefficients of this row's species, [tk][mu][n]; visible after
+    // the compaction's barriers
+    double* coef = lds + L.coef;
+    if (P.coef_lds) {
+        const double* src = P.radial_coeffs + (size_t)wrap_species(ti, P.Sp) * P.coef_lds;
+        for (int x = tid; x < P.coef_lds; x += NT) coef[x] = src[x];
+    }
     if (NT <= 256 && A.mom_in) {  // cached moments; visible after the compaction's barriers
         const double* mi = A.mom_in + (size_t)row * P.n_live;
         for (int x = tid; x < P.n_live; x += NT) m[x] = mi[x];
@@ -573,7 +582,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             if (k < K) cidx[k] = valid ? pos : -1;
             if (valid) {
                 vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
-                slot_of[pos] = k;
+                nbr_t[pos] = ct[c];
             }
             for (int w = 0; w < NW; ++w) nv += wave_cnt[w];
             nv = __builtin_amdgcn_readfirstlane(nv);
@@ -582,11 +591,13 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     } else {  // K <= NT: one slot per thread
         const int k = tid;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        int tk = -1;
         bool valid = false;
         if (k < K) {
             v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            tk = jt_row[k];
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+            valid = (tk >= 0) && (r < P.max_dist);
         }
         const uint64_t mask = __ballot(valid);
         const int lane = tid & (kWave - 1), wave = tid / kWave;
@@ -598,7 +609,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         if (k < K) cidx[k] = valid ? pos : -1;
         if (valid) {
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
-            slot_of[pos] = k;
+            nbr_t[pos] = tk;
         }
         if (tid == 0) {
             int tot = 0;
@@ -625,11 +636,21 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         const int np1 = P.nu_max + 1;
         const int Kp = P.k_layout | 1;
         double* pw = dm;
-        for (int i = tid; i < nv * P.R; i += NT) {  // radial values: one thread per (slot, mu)
-            const int c = i / P.R, mu = i - c * P.R;
-            double fv, dfv;
-            radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
-            f[i] = fv;
+        // radial values: one thread per (slot, mu)
+        if (P.coef_lds) {
+            for (int i = tid; i < nv * P.R; i += NT) {
+                const int c = i / P.R, mu = i - c * P.R;
+                double fv, dfv;
+                radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N, fv, dfv);
+                f[i] = fv;
+            }
+        } else {
+            for (int i = tid; i < nv * P.R; i += NT) {
+                const int c = i / P.R, mu = i - c * P.R;
+                double fv, dfv;
+                radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
+                f[i] = fv;
+            }
         }
         for (int t = tid; t < nv * 3; t += NT) {
             const int k = t / 3, q = t - 3 * k;
@@ -698,12 +719,22 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* gpart = lds + L.gp;
     double* gf = lds + L.gf;
     double* gfp = lds + L.gfp;
-    for (int i = tid; i < nv * P.R; i += NT) {
-        const int c = i / P.R, mu = i - c * P.R;
-        double fv, dfv;
-        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
-        gf[mu * K + c] = fv;
-        gfp[mu * K + c] = dfv;
+    if (P.coef_lds) {
+        for (int i = tid; i < nv * P.R; i += NT) {
+            const int c = i / P.R, mu = i - c * P.R;
+            double fv, dfv;
+            radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N, fv, dfv);
+            gf[mu * K + c] = fv;
+            gfp[mu * K + c] = dfv;
+        }
+    } else {
+        for (int i = tid; i < nv * P.R; i += NT) {
+            const int c = i / P.R, mu = i - c * P.R;
+            double fv, dfv;
+            radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
+            gf[mu * K + c] = fv;
+            gfp[mu * K + c] = dfv;
+        }
     }
     __syncthreads();
     {
@@ -893,7 +924,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
     double* vz = vy + Kl;
     double* rr = vz + Kl;
     int* ints = reinterpret_cast<int*>(lds + L.ints) + h * L.ints_per_row;
-    int* slot_of = ints;
+    int* nbr_t = ints;  // [Kl] compact index -> neighbour species
     int* cidx = ints + Kl;
     int* wave_cnt = ints + 2 * Kl;
     int* nvp = wave_cnt + NWH;
@@ -911,11 +942,13 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
     {
         const int k = ht;
         double v0 = 0, v1 = 0, v2 = 0, r = 1.0;
+        int tk = -1;
         bool valid = false;
         if (active && k < K) {
             v0 = rij_row[3 * k]; v1 = rij_row[3 * k + 1]; v2 = rij_row[3 * k + 2];
+            tk = jt_row[k];
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-            valid = (jt_row[k] >= 0) && (r < P.max_dist);
+            valid = (tk >= 0) && (r < P.max_dist);
         }
         const uint64_t mask = __ballot(valid);
         if (lane == 0) wave_cnt[wr] = __popcll(mask);
@@ -926,7 +959,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
         if (k < K) cidx[k] = valid ? pos : -1;
         if (valid) {
             vx[pos] = v0; vy[pos] = v1; vz[pos] = v2; rr[pos] = r;
-            slot_of[pos] = k;
+            nbr_t[pos] = tk;
         }
         if (ht == 0) {
             int tot = 0;
@@ -946,7 +979,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
         for (int i = ht; i < nv * P.R; i += NH) {  // radial values: one thread per (slot, mu)
             const int c = i / P.R, mu = i - c * P.R;
             double fv, dfv;
-            radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+            radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
             f[i] = fv;
         }
         for (int t = ht; t < nv * 3; t += NH) {
@@ -996,7 +1029,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
     for (int i = ht; i < nv * P.R; i += NH) {
         const int c = i / P.R, mu = i - c * P.R;
         double fv, dfv;
-        radial_one<RadialReal>(P, rr[c], ti, jt_row[slot_of[c]], mu, fv, dfv);
+        radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
         gf[mu * Kl + c] = fv;
         gfp[mu * Kl + c] = dfv;
     }
@@ -1317,6 +1350,21 @@ __global__ void __launch_bounds__(64) totals_kernel(const double* tot, int cell_
 // ---------------------------------------------------------------------------
 // Raises (never lowers) the dynamic-LDS limit of a variant's kernels: the
 // limit belongs to the kernel function, which every open potential shares.
+int atom_kernel_blocks_per_cu(int variant, size_t bytes)
+{
+    const void* fn = nullptr;
+    int nt = 0;
+    switch (variant) {
+        case kVariantV2x256: fn = (const void*)atom_kernel_v2<double, 256>; nt = 256; break;
+        case kVariantV2x512: fn = (const void*)atom_kernel_v2<double, 512>; nt = 512; break;
+        case kVariantV2x64: fn = (const void*)atom_kernel_v2<double, 64>; nt = 64; break;
+        default: return -1;
+    }
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, bytes) != hipSuccess) return -1;
+    return n;
+}
+
 hipError_t set_atom_kernel_lds(int variant, size_t bytes)
 {
     static size_t cur[kNumVariants] = {};

@@ -57,7 +57,7 @@ struct AtomLds {
 };
 
 struct AtomLds2 {
-    int m, dm, part, f, gp, gf, gfp, vx, vy, vz, r, red, xl, ints, total_doubles;
+    int m, dm, part, f, gp, gf, gfp, vx, vy, vz, r, red, xl, ints, coef, total_doubles;
 };
 
 // LDS of the balanced-item kernels (doubles).  Regions are reused across
@@ -74,12 +74,13 @@ struct AtomLds2 {
 //               max(B, 3K) on, F keeps the per-part gradients and the
 //               recomputed f, f'; after F, dm[0, 3K) holds the slot
 //               gradients
-//   part, vx, vy, vz, r, xl (per-row coefficients), ints
+//   part, vx, vy, vz, r, xl (per-row coefficients), ints, coef (the radial
+//               coefficients of the row's species, when staged)
 // Without tables (a per-row pass over cached moments, which skips phases
 // A's radial values and B) the power table and f are left out.
 __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B, int K, int R, int nu_max,
                                                     int n_parts, int nt, int max_partials, int xi_len,
-                                                    bool tables = true)
+                                                    bool tables = true, int coef_len = 0)
 {
     AtomLds2 L;
     const int pw_len = tables ? (K | 1) * 3 * (nu_max + 1) : 0;
@@ -103,7 +104,8 @@ __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B,
     L.r = L.vz + K;
     L.xl = L.r + K;
     L.ints = L.xl + xi_len;
-    L.total_doubles = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
+    L.coef = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
+    L.total_doubles = L.coef + coef_len;
     return L;
 }
 
@@ -170,8 +172,11 @@ constexpr int variant_parts(int v)
 
 size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
 size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
-                       int max_partials, int xi_len = 0, bool tables = true);
+                       int max_partials, int xi_len = 0, bool tables = true, int coef_len = 0);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);
+// Resident workgroups per CU of a balanced-item variant's FP64 kernel at
+// `bytes` of dynamic LDS (-1 for other variants or on error).
+int atom_kernel_blocks_per_cu(int variant, size_t bytes);
 // ev_begin / ev_end (may be null) are recorded on `stream` around the atom
 // kernel launch, for live per-kernel timing.
 // The atom kernel alone over `grid` workgroups (rows, or the persistent

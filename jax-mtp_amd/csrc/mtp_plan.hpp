@@ -98,6 +98,8 @@ struct DevPlan {
     int k_layout;                    // slots the LDS layout is sized for (>= the slots of any call)
     int lds_tables;                  // 1: the LDS layout holds the power table and radial values; 0 only
                                      // for the plans of a per-row pass that reads a moment cache
+    int coef_lds;                    // > 0: the radial coefficients of the row's species (Sp * R * N
+                                     // doubles) are staged in LDS; 0: read from global memory
     int max_partials;
     // inverse of alpha_moment_mapping: distinct mapped moments (increasing)
     // and, per moment, the scalar indices s mapped to it (increasing)

@@ -100,6 +100,16 @@ int mtp_compute_padded_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_
                               int natoms_energy, int natoms_force, double *d_out, double *d_forces,
                               double *d_eatom, void *stream);
 
+/* Per-atom energies and virials of the last mtp_compute on ctx, for the pair
+ * style's eflag_atom / vflag_atom (no reference counterpart: the reference
+ * returns totals only, pair_jax_mtp.cpp:195-205).  eatom[natoms] = E_i;
+ * vatom6[natoms][6] = -(W_i + W_i^T) / 2 with W_i = sum_k r_ik (x) dE_i/dr_ik
+ * (atom i's row), in LAMMPS order (xx, yy, zz, xy, xz, yz, energy units), so
+ * that the vatom6 rows sum to mtp_compute's virial6.  Either output may be
+ * NULL.  natoms must not exceed that call's natoms (MTP_ERR_ARG otherwise,
+ * also after a device entry point has run on ctx since).  Synchronous. */
+int mtp_last_atom_values(mtp_ctx *ctx, int natoms, double *eatom, double *vatom6);
+
 /* Device pointer of the per-slot gradients dE_i/dr_ik [nrows][nslots][3] of
  * the last device evaluation (valid until the next call on ctx). */
 const double *mtp_last_slot_gradients(const mtp_ctx *ctx);

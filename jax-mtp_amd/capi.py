@@ -37,6 +37,7 @@ SIGNATURES = {
     "mtp_compute": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]),
     "mtp_compute_padded_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _I, _D, _I, _I, _P, _P, _P, _P]),
     "mtp_last_slot_gradients": (_P, [_P]),
+    "mtp_last_atom_values": (_I, [_P, _I, _P, _P]),
     "mtp_sync": (_I, [_P, _P]),
     "mtp_set_variant": (_I, [_P, _I]),
     "mtp_get_variant": (_I, [_P, _P, _P]),

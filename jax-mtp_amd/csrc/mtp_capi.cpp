@@ -125,6 +125,8 @@ struct mtp_ctx {
     int *h_it = nullptr, *h_js = nullptr, *h_jt = nullptr;
     double* h_F = nullptr;
     double* h_out = nullptr;
+    int last_host_natoms = 0;             // rows of the last mtp_compute whose E_i, W_i are in d_Ei, d_Wi
+    std::vector<double> h_wi;             // staging of mtp_last_atom_values
     // training buffers (allocated at the first training call)
     size_t train_lds = 0;
     DevPlan tstreams{};                   // contraction streams of the training kernel (stream fields only)
@@ -588,6 +590,7 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
                               void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     if (nrows < 0 || nrows > ctx->max_atoms)
         return ctx->fail(MTP_ERR_CAPACITY, "rows " + std::to_string(nrows) + " exceed max_atoms " +
                                                std::to_string(ctx->max_atoms));
@@ -661,6 +664,7 @@ int mtp_autotune(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const
                  const int* d_jtypes, int reps, int* best, double* ms_per_variant)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     if (reps < 1) reps = 1;
     if (nrows <= 0 || nrows > ctx->max_atoms || nslots <= 0 || nslots > ctx->max_nbr)
         return ctx->fail(MTP_ERR_CAPACITY, "autotune shape outside the context's capacity");
@@ -735,6 +739,35 @@ int mtp_profile_read(mtp_ctx* ctx, double* atom_kernel_ms, int* launches)
 
 const double* mtp_last_slot_gradients(const mtp_ctx* ctx) { return ctx ? ctx->d_g : nullptr; }
 
+int mtp_last_atom_values(mtp_ctx* ctx, int natoms, double* eatom, double* vatom6)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (natoms < 0 || natoms > ctx->last_host_natoms)
+        return ctx->fail(MTP_ERR_ARG, "natoms " + std::to_string(natoms) + " exceeds the last mtp_compute's " +
+                                          std::to_string(ctx->last_host_natoms));
+    if (natoms == 0 || (!eatom && !vatom6)) return MTP_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && eatom) e = hipMemcpy(eatom, ctx->d_Ei, natoms * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vatom6) {
+        ctx->h_wi.resize((size_t)natoms * 9);
+        e = hipMemcpy(ctx->h_wi.data(), ctx->d_Wi, (size_t)natoms * 9 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            for (int i = 0; i < natoms; ++i) {
+                const double* W = ctx->h_wi.data() + (size_t)i * 9;  // W[a * 3 + b]
+                double* v = vatom6 + (size_t)i * 6;
+                v[0] = -W[0];
+                v[1] = -W[4];
+                v[2] = -W[8];
+                v[3] = -0.5 * (W[1] + W[3]);
+                v[4] = -0.5 * (W[2] + W[6]);
+                v[5] = -0.5 * (W[5] + W[7]);
+            }
+        }
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "per-atom values");
+    return MTP_OK;
+}
+
 int mtp_sync(mtp_ctx* ctx, void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
@@ -806,6 +839,7 @@ int mtp_compute(mtp_ctx* ctx, int natoms, int nneigh_total, const double* const*
     if (stress6) std::memcpy(stress6, ctx->h_out + 1, 6 * sizeof(double));
     if (virial6) std::memcpy(virial6, ctx->h_out + 7, 6 * sizeof(double));
     if (ctx->h_out[13] != 0.0) return ctx->fail(MTP_ERR_NONFINITE, "non-finite energy or virial");
+    ctx->last_host_natoms = natoms;
     return MTP_OK;
 }
 
@@ -866,6 +900,7 @@ int mtp_batch_forward(mtp_ctx* ctx, int nrows, int nslots, int nconf, const int*
 {
     (void)d_conf_of;
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     if (nconf <= 0 || !d_conf_off || !d_volumes || !d_E || !d_stress) return ctx->fail(MTP_ERR_ARG, "bad batch");
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     // one evaluation over all rows: per-atom energies, gradients, virials, forces
@@ -884,6 +919,7 @@ int mtp_batch_param_grad(mtp_ctx* ctx, int nrows, int nslots, int nconf, const i
 {
     (void)d_conf_off;
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     if (nrows <= 0 || nrows > ctx->max_atoms)
         return ctx->fail(MTP_ERR_CAPACITY, "batch rows " + std::to_string(nrows) + " outside 1..max_atoms");
     if (nslots <= 0 || nslots > ctx->max_nbr) return ctx->fail(MTP_ERR_CAPACITY, "batch slots exceed max_neighbors");
@@ -1061,6 +1097,7 @@ int mtp_descriptor_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_ityp
                           const int* d_jtypes, double* d_basis, void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     int rc = check_rows(ctx, nrows, nslots);
     if (rc || nrows == 0) return rc;
     if (!d_itypes || !d_rijs || !d_jtypes || !d_basis) return ctx->fail(MTP_ERR_ARG, "null descriptor array");
@@ -1079,6 +1116,7 @@ int mtp_slot_grad_xi_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_it
                             void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     int rc = check_rows(ctx, nrows, nslots);
     if (rc || nrows == 0) return rc;
     if (!d_itypes || !d_rijs || !d_jtypes || !d_xi || !d_g || xi_stride < 0)
@@ -1091,6 +1129,7 @@ int mtp_basis_jacobian_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
                               const int* d_jtypes, double* d_J, void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     int rc = check_rows(ctx, nrows, nslots);
     if (rc || nrows == 0) return rc;
     if (!d_itypes || !d_rijs || !d_jtypes || !d_J) return ctx->fail(MTP_ERR_ARG, "null Jacobian array");
@@ -1137,6 +1176,7 @@ int mtp_gather_forces_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_j
                              double* d_forces, void* stream)
 {
     if (!ctx) return MTP_ERR_ARG;
+    ctx->last_host_natoms = 0;  // d_Ei, d_Wi no longer hold an mtp_compute's rows
     int rc = check_rows(ctx, nrows, nslots);
     if (rc || nrows == 0) return rc;
     if (!d_js || !d_g || !d_forces) return ctx->fail(MTP_ERR_ARG, "null gather array");
@@ -1277,6 +1317,8 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     if (!h) return MTP_ERR_ARG;
     mtp_ctx* D = h->desc;
     mtp_ctx* X = h->exact;
+    D->last_host_natoms = 0;  // their d_Ei, d_Wi no longer hold an mtp_compute's rows
+    if (X) X->last_host_natoms = 0;
     int rc = check_rows(D, nrows, nslots);
     if (rc) return h->fail(rc, D->err);
     if (X && (nrows > X->max_atoms || nslots > X->max_nbr))

@@ -139,6 +139,21 @@ void PairJaxMTP::compute(int eflag, int vflag)
     const double *v = virial_compat ? stress6 : virial6;
     for (int k = 0; k < 6; k++) virial[k] += v[k];
   }
+  // per-atom energy and virial (compute pe/atom, stress/atom): E_i and the symmetrised -W_i of
+  // atom i's row, which sum to the totals above (the reference has no per-atom values)
+  if (eflag_atom || vflag_atom) {
+    atom_e.resize(nlocal);
+    atom_v.resize((size_t) nlocal * 6);
+    if (mtp_last_atom_values(ctx, nlocal, eflag_atom ? atom_e.data() : nullptr,
+                             vflag_atom ? atom_v.data() : nullptr) != MTP_OK)
+      error->all(FLERR, "jax/mtp per-atom values failed: {}", mtp_last_error(ctx));
+    for (int ii = 0; ii < inum; ii++) {
+      const int i = list->ilist[ii];
+      if (eflag_atom) eatom[i] += atom_e[i];
+      if (vflag_atom)
+        for (int k = 0; k < 6; k++) vatom[i][k] += atom_v[6 * (size_t) i + k];
+    }
+  }
   total_calls++;
 }
 

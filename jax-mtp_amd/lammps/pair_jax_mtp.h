@@ -53,6 +53,7 @@ class PairJaxMTP : public Pair {
   std::vector<const double *> pos_rows;
   std::vector<const int *> nbr_rows, ntype_rows;
   std::vector<double> forces;     // [max_atoms][3]
+  std::vector<double> atom_e, atom_v;  // per-atom energy [nlocal], virial [nlocal][6]
 
   void allocate();
   void initialize_persistent_arrays();

@@ -272,3 +272,13 @@ class MTPPotential:
             counts.ctypes.data_as(vp), ctypes.cast(pt, vp), float(volume), ctypes.byref(E),
             F.ctypes.data_as(vp), s6.ctypes.data_as(vp), v6.ctypes.data_as(vp)), self.ctx)
         return E.value, F, s6, v6
+
+    def last_atom_values(self, natoms):
+        """Per-atom energies [natoms] and LAMMPS virials [natoms, 6] (xx yy zz xy xz yz) of the last
+        compute_rows call; the virial rows sum to its virial6."""
+        eatom = np.zeros(natoms)
+        vatom = np.zeros((natoms, 6))
+        vp = ctypes.c_void_p
+        capi.check(capi.lib().mtp_last_atom_values(self.ctx, natoms, eatom.ctypes.data_as(vp),
+                                                   vatom.ctypes.data_as(vp)), self.ctx)
+        return eatom, vatom

@@ -225,6 +225,36 @@ def test_host_rows_api_matches_device_api(jm):
     pot.close()
 
 
+def test_per_atom_energy_and_virial(jm, O):
+    """mtp_last_atom_values (the pair style's eatom / vatom): E_i against the oracle, the per-atom
+    virial against -sym(sum_k r_ik (x) g_ik) of the oracle's slot gradients, and both summing to the
+    totals of the same mtp_compute."""
+    w = jm.configs.small_cluster(24, 3, 30, seed=6, cell=8.5, cutoff=5.0, max_nbr=64)
+    nb = w.neighbors()
+    pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=64)
+    E, F, s6, v6 = pot.compute_rows(w.natoms, nb["rij"].reshape(w.natoms, -1), nb["itypes"], nb["js"], nb["counts"],
+                                    nb["jtypes"], w.volume)
+    eatom, vatom = pot.last_atom_values(w.natoms)
+    part_e, _ = pot.last_atom_values(w.natoms // 2)  # a prefix of the same rows
+    np.testing.assert_array_equal(part_e, eatom[: w.natoms // 2])
+    ref = _run_oracle(O, w, nb)
+    assert _rel(eatom, ref["eatom"]) <= TOL
+    W = np.einsum("ika,ikb->iab", nb["rij"], ref["g"])  # per-row virial of the oracle's gradients
+    vref = -np.stack([W[:, 0, 0], W[:, 1, 1], W[:, 2, 2], 0.5 * (W[:, 0, 1] + W[:, 1, 0]),
+                      0.5 * (W[:, 0, 2] + W[:, 2, 0]), 0.5 * (W[:, 1, 2] + W[:, 2, 1])], axis=1)
+    assert _rel(vatom, vref) <= TOL
+    np.testing.assert_allclose(eatom.sum(), E, rtol=1e-12, atol=1e-12 * np.abs(eatom).max())
+    np.testing.assert_allclose(vatom.sum(axis=0), v6, rtol=1e-12, atol=1e-12 * np.abs(vatom).max())
+    with pytest.raises(RuntimeError):
+        pot.last_atom_values(w.natoms + 1)
+    # a device entry point overwrites the per-row buffers: the values are no longer offered
+    pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3, w.volume,
+                                         w.natoms, w.natoms)
+    with pytest.raises(RuntimeError):
+        pot.last_atom_values(1)
+    pot.close()
+
+
 def test_deterministic(jm):
     w = jm.configs.small_cluster(20, 2, 40, seed=8, cell=9.0, cutoff=5.0, max_nbr=80)
     nb = w.neighbors()

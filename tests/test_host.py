@@ -85,6 +85,11 @@ def test_malformed_files(jm, tmp_path):
         assert rc == jm.capi.MTP_ERR_FILE, name
 
 
+def test_per_atom_values_need_a_handle(jm):
+    """mtp_last_atom_values rejects a null handle before touching the GPU."""
+    assert jm.capi.lib().mtp_last_atom_values(None, 1, None, None) == -1
+
+
 def test_open_argument_checks_without_gpu(jm, tmp_path):
     """pair_jax_mtp.cpp:241-243, 268-269: positive sizes, max_neighbors > 1."""
     p = jm.configs.random_params(10, 1, 8, 2.0, 5.0, np.random.default_rng(0))

@@ -25,7 +25,7 @@ extern "C" {
 #define MTP_ERR_FILE -2     /* potential file missing or malformed       */
 #define MTP_ERR_CAPACITY -3 /* natoms > max_atoms or count > max_nbr     */
 #define MTP_ERR_HIP -4      /* HIP runtime error                         */
-#define MTP_ERR_NONFINITE -5 /* NaN/Inf in energy or virial             */
+#define MTP_ERR_NONFINITE -5 /* NaN/Inf in energy or virial, or in r_ij */
 #define MTP_ERR_UNSUPPORTED -6 /* basis layout or size not supported     */
 
 typedef struct mtp_ctx mtp_ctx;
@@ -62,7 +62,7 @@ int mtp_info(const mtp_ctx *ctx, int *level, int *species_count, int *radial_fun
  * order, species_coeffs[Sp], radial_coeffs[Sp][Sp][R][N]); NULL keeps a
  * group.  Mirrors MTPData.parameters' setter (data.py:74-102). */
 int mtp_set_params(mtp_ctx *ctx, const double *moment_coeffs, const double *species_coeffs,
-                   const double *radial_coeffs, double scaling);
+                   const double *radial_coeffs, double scaling); /* scaling NaN or <= 0 keeps the current one */
 
 /* Same from device arrays, enqueued on stream (no host synchronisation,
  * so it can be captured in a graph with the evaluation that follows).  The
@@ -91,10 +91,17 @@ int mtp_compute(mtp_ctx *ctx, int natoms, int nneigh_total, const double *const 
  * compile_cuda.py:225-232 but FP64): d_itypes[nrows], d_js[nrows*nslots],
  * d_rijs[nrows*nslots*3], d_jtypes[nrows*nslots], cell_rank, volume,
  * natoms_energy, natoms_force.  nrows <= max_atoms, nslots <= max_nbr.
+ * Masks as the reference wires them: the energy sums E_i for
+ * i < natoms_energy (jax_backend.py:447-452), and the forces are masked by
+ * natoms_energy too (rows i and Newton-3 targets j below it), because
+ * jax_backend.py:414-418 passes natoms_energy into the natoms_force slot of
+ * calc_energy_forces_stress; natoms_force is accepted and unused.
  * Device outputs: d_out[14] = {E, stress6 (reference), virial6 (LAMMPS),
  * status}, d_forces[nrows*3]; d_eatom[nrows] (per-atom energies) may be
- * NULL.  stream is a hipStream_t (NULL = the context's stream).  Returns
- * after enqueueing; call mtp_sync(ctx, stream) before reading results. */
+ * NULL.  status bit 1: non-finite energy or virial; bit 2: a slot vector
+ * r_ij with a NaN/Inf component (the slot is dropped and E set to NaN).
+ * stream is a hipStream_t (NULL = the context's stream).  Returns after
+ * enqueueing; call mtp_sync(ctx, stream) before reading results. */
 int mtp_compute_padded_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js,
                               const double *d_rijs, const int *d_jtypes, int cell_rank, double volume,
                               int natoms_energy, int natoms_force, double *d_out, double *d_forces,
@@ -119,14 +126,16 @@ int mtp_sync(mtp_ctx *ctx, void *stream);
 
 /* Atom-kernel variants (no reference counterpart; XLA chose its own
  * fusion): 0 = one thread per target moment, 256 threads per atom;
- * 1 / 2 / 3 = balanced work items with 256 / 512 / 64 threads per atom.
- * mtp_open picks variant 3 for small contraction tables (<= 400 entries,
- * levels up to 14) and otherwise the widest variant whose LDS fits (env
- * MTP_KERNEL_VARIANT overrides).  mtp_autotune times every available variant
- * on the given device arrays (reps launches each, after one warm-up),
- * selects the fastest for ctx and reports the per-variant ms in
+ * 1 / 2 / 3 = contraction streams with 256 / 512 / 64 threads per atom;
+ * 4 / 5 = two atoms per workgroup sharing the contraction streams, 1024 /
+ * 512 threads (plain force evaluation only).  mtp_open picks variant 3 for
+ * small contraction tables (<= 400 entries, levels up to 14) and otherwise
+ * the widest single-row variant whose LDS fits (env MTP_KERNEL_VARIANT
+ * overrides).  mtp_autotune times every available variant on the given
+ * device arrays (reps launches each, after one warm-up), selects the
+ * fastest for ctx and reports the per-variant ms in
  * ms_per_variant[MTP_NUM_VARIANTS] (-1 = unavailable). */
-#define MTP_NUM_VARIANTS 4
+#define MTP_NUM_VARIANTS 6
 int mtp_set_variant(mtp_ctx *ctx, int variant);
 int mtp_get_variant(const mtp_ctx *ctx, int *variant, long long *lds_bytes);
 int mtp_autotune(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js, const double *d_rijs,
@@ -225,8 +234,8 @@ const char *mtp_head_last_error(const mtp_head *h);
 /* theta from host (on_device 0, synchronous) or device memory (stream-ordered). */
 int mtp_head_set_theta(mtp_head *h, const double *theta, int on_device, void *stream);
 
-/* Same outputs as mtp_compute_padded_device (d_out[14]: E, stress6,
- * virial6, status), plus d_flags[nrows] (1 = out of box; may be NULL) and
+/* Same outputs and masks as mtp_compute_padded_device (d_out[14]: E,
+ * stress6, virial6, status; n_f accepted and unused), plus d_flags[nrows] (1 = out of box; may be NULL) and
  * d_n_fallback (device int: rows re-evaluated exactly; may be NULL). */
 int mtp_head_compute_padded_device(mtp_head *h, int nrows, int nslots, const int *d_itypes, const int *d_js,
                                    const double *d_rijs, const int *d_jtypes, int cell_rank, double volume, int n_e,

@@ -23,6 +23,8 @@
 
 using namespace mtp;
 
+static_assert(MTP_NUM_VARIANTS == kNumVariants, "mtp_hip.h must size ms_per_variant for every kernel variant");
+
 namespace {
 
 std::mutex g_err_mu;
@@ -119,6 +121,7 @@ struct mtp_ctx {
     double *d_rij = nullptr, *d_Ei = nullptr, *d_g = nullptr, *d_Wi = nullptr, *d_F = nullptr, *d_out = nullptr;
     int *d_it = nullptr, *d_js = nullptr, *d_jt = nullptr, *d_cnt = nullptr, *d_off = nullptr, *d_rev = nullptr;
     int *d_csr_state = nullptr, *d_js_copy = nullptr;  // reuse of the reverse-neighbor CSR (launch_gather)
+    int* d_status = nullptr;              // non-finite slot vectors seen by the atom kernels (cleared by the totals)
     double* d_tot = nullptr;
     // pinned staging
     double* h_rij = nullptr;
@@ -310,9 +313,11 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->d_rev = c->alloc<int>(A * K, e);
     c->d_csr_state = c->alloc<int>(8, e);
     c->d_js_copy = c->alloc<int>(A * K, e);
-    if (e == hipSuccess) {  // no previous build yet (valid flag 0)
+    c->d_status = c->alloc<int>(8, e);
+    if (e == hipSuccess) {  // no previous build yet (valid flag 0); status clear
         static const int zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         e = hipMemcpyAsync(c->d_csr_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->d_status, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rij, A * K * 3 * sizeof(double), hipHostMallocDefault);
@@ -604,11 +609,16 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
     AtomArgs A{};
     A.nrows = nrows; A.nslots = nslots; A.rij = d_rijs; A.itypes = d_itypes; A.jtypes = d_jtypes; A.js = d_js;
     A.Ei = d_eatom ? d_eatom : ctx->d_Ei;
-    A.g = ctx->d_g; A.Wi = ctx->d_Wi;
+    A.g = ctx->d_g; A.Wi = ctx->d_Wi; A.status = ctx->d_status;
     ForceArgs F{};
-    F.n_e = natoms_energy; F.n_f = natoms_force; F.cell_rank = cell_rank; F.volume = volume;
+    // jax_backend.py:414-418 passes natoms_energy positionally into
+    // calc_energy_forces_stress's natoms_force slot, which becomes the mask of
+    // accumulate_forces (:279-301): forces and their Newton-3 targets are
+    // masked by natoms_energy, and natoms_force has no effect (as nneigh_total).
+    (void)natoms_force;
+    F.n_e = natoms_energy; F.n_f = natoms_energy; F.cell_rank = cell_rank; F.volume = volume;
     F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = d_forces; F.out = d_out; F.tot = ctx->d_tot;
-    F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy;
+    F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy; F.status = ctx->d_status;
     hipEvent_t eb = nullptr, ee = nullptr;
     if (ctx->profiling) {
         if (ctx->ev_used == ctx->ev_pool.size()) {
@@ -630,7 +640,7 @@ int mtp_compute_padded_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_
     std::memset(&key, 0, sizeof(key));
     key.p[0] = d_itypes; key.p[1] = d_js; key.p[2] = d_rijs; key.p[3] = d_jtypes; key.p[4] = d_out;
     key.p[5] = d_forces; key.p[6] = d_eatom;
-    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = natoms_energy; key.n[4] = natoms_force;
+    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = natoms_energy;
     key.n[5] = ctx->variant * 2 + (ctx->radial_fp32 != 0);
     key.n[6] = (int)ctx->gen;
     key.volume = volume;
@@ -678,6 +688,7 @@ int mtp_autotune(mtp_ctx* ctx, int nrows, int nslots, const int* d_itypes, const
     F.n_e = nrows; F.n_f = nrows; F.cell_rank = 3; F.volume = 1.0;
     F.cnt = ctx->d_cnt; F.off = ctx->d_off; F.rev = ctx->d_rev; F.F = ctx->d_F; F.out = ctx->d_out; F.tot = ctx->d_tot;
     F.csr_state = ctx->d_csr_state; F.js_copy = ctx->d_js_copy;
+    A.status = ctx->d_status; F.status = ctx->d_status;
     int chosen = ctx->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumVariants; ++v) {
@@ -838,7 +849,11 @@ int mtp_compute(mtp_ctx* ctx, int natoms, int nneigh_total, const double* const*
     if (natoms > 0) std::memcpy(forces, ctx->h_F, A * 3 * sizeof(double));
     if (stress6) std::memcpy(stress6, ctx->h_out + 1, 6 * sizeof(double));
     if (virial6) std::memcpy(virial6, ctx->h_out + 7, 6 * sizeof(double));
-    if (ctx->h_out[13] != 0.0) return ctx->fail(MTP_ERR_NONFINITE, "non-finite energy or virial");
+    if (ctx->h_out[13] != 0.0) {
+        const int st = (int)ctx->h_out[13];
+        return ctx->fail(MTP_ERR_NONFINITE, (st & 2) ? "non-finite neighbor vector r_ij in the input rows"
+                                                     : "non-finite energy or virial");
+    }
     ctx->last_host_natoms = natoms;
     return MTP_OK;
 }
@@ -1332,9 +1347,11 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     AtomArgs A = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
     A.js = d_js; A.Ei = E; A.g = D->d_g; A.Wi = D->d_Wi;
     ForceArgs F{};
-    F.n_e = n_e; F.n_f = n_f; F.cell_rank = cell_rank; F.volume = volume;
+    // forces masked by natoms_energy, as the plain entry (jax_backend.py:414-418)
+    (void)n_f;
+    F.n_e = n_e; F.n_f = n_e; F.cell_rank = cell_rank; F.volume = volume;
     F.cnt = D->d_cnt; F.off = D->d_off; F.rev = D->d_rev; F.F = d_forces; F.out = d_out; F.tot = D->d_tot;
-    F.csr_state = D->d_csr_state; F.js_copy = D->d_js_copy;
+    F.csr_state = D->d_csr_state; F.js_copy = D->d_js_copy; F.status = D->d_status;
     const int vd = nrows > 0 ? mode_variant(D, false) : -1, vx = nrows > 0 ? mode_variant(D, true) : -1;
     const int vf = X && nrows > 0 ? mode_variant(X, false) : -1;
     struct {
@@ -1346,7 +1363,7 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
     std::memset(&key, 0, sizeof(key));
     key.p[0] = d_itypes; key.p[1] = d_js; key.p[2] = d_rijs; key.p[3] = d_jtypes; key.p[4] = d_out;
     key.p[5] = d_forces; key.p[6] = d_eatom; key.p[7] = d_flags; key.p[8] = d_n_fallback;
-    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = n_e; key.n[4] = n_f;
+    key.n[0] = nrows; key.n[1] = nslots; key.n[2] = cell_rank; key.n[3] = n_e;
     key.n[5] = vd; key.n[6] = vx; key.n[7] = vf; key.n[8] = D->radial_fp32;
     key.n[9] = (int)D->gen; key.n[10] = X ? (int)X->gen : 0;
     key.volume = volume;
@@ -1357,7 +1374,7 @@ int mtp_head_compute_padded_device(mtp_head* h, int nrows, int nslots, const int
             {
                 const int v = mode_variant(D, false);
                 AtomArgs Bd = row_args(nrows, nslots, d_itypes, d_rijs, d_jtypes);
-                Bd.basis = h->d_basis; Bd.fwd_only = 1; Bd.mom_out = h->d_mom;
+                Bd.basis = h->d_basis; Bd.fwd_only = 1; Bd.mom_out = h->d_mom; Bd.status = D->d_status;
                 e = launch_atoms(D->vdev[v], Bd, D->vlds[v], D->radial_fp32, s, v, nrows);
                 if (e != hipSuccess)
                     return h->fail(MTP_ERR_HIP, std::string("descriptor launch: ") + hipGetErrorString(e));

@@ -163,6 +163,14 @@ bool validate_mtp(const MtpFile& f, std::string& err)
     if (f.species_count <= 0) { err = "species_count must be positive"; return false; }
     if (f.radial_funcs_count <= 0 || f.radial_basis_size <= 0) { err = "radial sizes must be positive"; return false; }
     if (!f.has_radial) { err = "missing radial_coeffs"; return false; }
+    // the counts may be declared after the radial block: check the table against the final ones
+    const size_t radial_len =
+        (size_t)f.species_count * f.species_count * f.radial_funcs_count * f.radial_basis_size;
+    if (f.radial_coeffs.size() != radial_len) {
+        err = "radial_coeffs holds " + std::to_string(f.radial_coeffs.size()) + " values, species_count^2 * " +
+              "radial_funcs_count * radial_basis_size = " + std::to_string(radial_len);
+        return false;
+    }
     if (!f.has_species || (int)f.species_coeffs.size() != f.species_count) { err = "species_coeffs missing or wrong length"; return false; }
     if (!f.has_moment || (int)f.moment_coeffs.size() != f.S()) { err = "moment_coeffs missing or wrong length"; return false; }
     if (!(f.max_dist > f.min_dist)) { err = "max_dist must exceed min_dist"; return false; }

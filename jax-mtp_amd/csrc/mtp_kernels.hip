@@ -32,6 +32,18 @@ namespace mtp {
 
 constexpr int kBlock = kAtomBlock;
 
+// A slot vector with a NaN or Inf component turns the reference's energy and
+// forces into NaN (r, u = r_ij / r and the Chebyshev terms of such a slot are
+// not finite, and s = 0 does not mask them, jax_backend.py:83-95, 232).  The
+// kernels drop such a slot like an invalid one and raise the status word
+// instead, which the totals turn into a NaN energy and status bit 2 (error
+// MTP_ERR_NONFINITE at the C ABI).  Called by every lane of the wave.
+__device__ __forceinline__ void flag_nonfinite(int* status, bool bad)
+{
+    const uint64_t m = __ballot(bad);
+    if (status && m != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(status, 1);
+}
+
 // LDS layout (in doubles) shared by host sizing and the kernel.
 __host__ __device__ inline AtomLds atom_lds_layout(int amc, int nslots, int R, int nu_max, int n_parts)
 {
@@ -99,6 +111,7 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
             valid = (tk >= 0) && (r < P.max_dist);
         }
+        flag_nonfinite(A.status, k < K && !isfinite(r));
         const uint64_t mask = __ballot(valid);
         const int lane = tid & (kWave - 1), wave = tid / kWave;
         if (lane == 0) wave_cnt[wave] = __popcll(mask);
@@ -572,6 +585,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
                 r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
                 valid = (ct[c] >= 0) && (r < P.max_dist);
             }
+            flag_nonfinite(A.status, k < K && !isfinite(r));
             const uint64_t mask = __ballot(valid);
             const int lane = tid & (kWave - 1), wave = tid / kWave;
             if (lane == 0) wave_cnt[wave] = __popcll(mask);
@@ -599,6 +613,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
             valid = (tk >= 0) && (r < P.max_dist);
         }
+        flag_nonfinite(A.status, !isfinite(r));
         const uint64_t mask = __ballot(valid);
         const int lane = tid & (kWave - 1), wave = tid / kWave;
         if (lane == 0) wave_cnt[wave] = __popcll(mask);
@@ -950,6 +965,7 @@ atom_kernel_pair(const DevPlan P, AtomArgs A)
             r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
             valid = (tk >= 0) && (r < P.max_dist);
         }
+        flag_nonfinite(A.status, !isfinite(r));
         const uint64_t mask = __ballot(valid);
         if (lane == 0) wave_cnt[wr] = __popcll(mask);
         __syncthreads();
@@ -1283,7 +1299,9 @@ __global__ void __launch_bounds__(256) force_kernel(const double* g, int nrows, 
 
 // out[0] = E, out[1..6] = stress (reference Voigt xx yy zz yz xz xy, /V),
 // out[7..12] = LAMMPS virial (xx yy zz xy xz yz, energy units), out[13] =
-// status (0 ok, 1 non-finite energy or virial).
+// status: bit 1 a non-finite energy or virial, bit 2 a non-finite slot
+// vector r_ij (the atom kernels' flag, which this kernel clears; the energy
+// is then NaN, as the reference's).
 // Stage 1: block b sums rows b, b + G, ... (E_i for i < n_e, and the 9
 // virial components) into tot[b][10]; stage 2 adds the kTotalsBlocks partials
 // in block order.  Deterministic for a given row count.
@@ -1316,7 +1334,8 @@ __global__ void __launch_bounds__(256) totals_partial_kernel(const double* Ei, c
     }
 }
 
-__global__ void __launch_bounds__(64) totals_kernel(const double* tot, int cell_rank, double volume, double* out)
+__global__ void __launch_bounds__(64) totals_kernel(const double* tot, int cell_rank, double volume, double* out,
+                                                    int* in_status)
 {
     __shared__ double acc[10];
     const int tid = threadIdx.x;
@@ -1341,9 +1360,14 @@ __global__ void __launch_bounds__(64) totals_kernel(const double* tot, int cell_
         out[10] = -0.5 * (W[1] + W[3]);
         out[11] = -0.5 * (W[2] + W[6]);
         out[12] = -0.5 * (W[5] + W[7]);
+        const int bad_in = in_status ? *in_status : 0;
+        if (bad_in) {
+            out[0] = __builtin_nan("");
+            *in_status = 0;
+        }
         bool bad = !isfinite(out[0]);
         for (int q = 0; q < 9; ++q) bad = bad || !isfinite(W[q]);
-        out[13] = bad ? 1.0 : 0.0;
+        out[13] = (double)((bad ? 1 : 0) | (bad_in ? 2 : 0));
     }
 }
 
@@ -1516,7 +1540,7 @@ hipError_t launch_totals(const AtomArgs& A, const ForceArgs& F, hipStream_t stre
 {
     hipLaunchKernelGGL(totals_partial_kernel, dim3(kTotalsBlocks), dim3(256), 0, stream, A.Ei, A.Wi,
                        A.nrows > 0 ? A.nrows : 0, A.nrows > 0 ? F.n_e : 0, F.tot);
-    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, stream, F.tot, F.cell_rank, F.volume, F.out);
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, stream, F.tot, F.cell_rank, F.volume, F.out, F.status);
     return hipGetLastError();
 }
 

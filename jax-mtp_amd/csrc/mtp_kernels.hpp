@@ -34,6 +34,7 @@ struct AtomArgs {
     double* mom_out;
     const double* mom_in;
     const int* skip_row;   // rows with skip_row[row] != 0 are left to another kernel (null: none)
+    int* status;           // non-null: set to 1 when a slot vector is not finite (read and cleared by the totals)
 };
 
 struct ForceArgs {
@@ -49,6 +50,7 @@ struct ForceArgs {
     // previous call with the same js, n_f and shape (null: rebuild always).
     int* csr_state;        // [8]: same flag, last nrows, nslots, n_f, valid
     int* js_copy;          // [A][K] js of the last build
+    int* status;           // the atom kernels' non-finite-input flag (null: none); the totals clear it
 };
 constexpr int kTotalsBlocks = 128;
 

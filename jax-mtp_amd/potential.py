@@ -4,6 +4,9 @@
 entry point of the same name (jax_backend.py:374-460): the same 8 dynamic
 arguments, the same padding/masking semantics, and the same outputs
 (energy, forces[A,3], stress6 in the reference's Voigt order), in FP64.
+As in the reference, both the energy and the forces are masked by
+natoms_energy (jax_backend.py:414-418 passes it into the force mask's slot);
+natoms_force is accepted and has no effect.
 PyTorch only supplies device memory and the stream; all arithmetic runs in
 the HIP kernels of libmtp_hip.so.
 """
@@ -112,7 +115,9 @@ class MTPPotential:
         m, mp = ptr(moment_coeffs)
         s, sp = ptr(species_coeffs)
         r, rp = ptr(radial_coeffs)
-        capi.check(capi.lib().mtp_set_params(self.ctx, mp, sp, rp, float(scaling) if scaling else float("nan")),
+        if scaling is not None and not (np.isfinite(scaling) and scaling > 0):
+            raise ValueError("scaling must be a positive finite number (None keeps the current one)")
+        capi.check(capi.lib().mtp_set_params(self.ctx, mp, sp, rp, float("nan") if scaling is None else float(scaling)),
                    self.ctx)
         if m is not None:
             self.params.moment_coeffs = m.copy()
@@ -120,7 +125,7 @@ class MTPPotential:
             self.params.species_coeffs = s.copy()
         if r is not None:
             self.params.radial_coeffs = r.reshape(self.params.radial_coeffs.shape).copy()
-        if scaling:
+        if scaling is not None:
             self.params.scaling = float(scaling)
 
     def set_params_device(self, theta, stream=None):
@@ -148,8 +153,9 @@ class MTPPotential:
         Inputs may be numpy arrays or torch tensors (device tensors are used
         in place).  Returns (energy, forces[A,3], stress6) as torch float64
         tensors on the device; with ``return_all`` also a dict holding
-        ``virial6`` (LAMMPS order), ``eatom`` [A], ``status`` and the per-slot
-        gradients ``g`` [A,K,3].
+        ``virial6`` (LAMMPS order), ``eatom`` [A], ``status`` (bit 1: non-finite
+        energy or virial, bit 2: a non-finite r_ij slot, energy then NaN) and the
+        per-slot gradients ``g`` [A,K,3].
         """
         import torch
 

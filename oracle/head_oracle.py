@@ -52,7 +52,9 @@ def pack(W1, b1, W2, b2, w3, b3):
 
 
 def silu(a):
-    g = 1.0 / (1.0 + np.exp(-a))
+    # logistic function without overflow: exp of a non-positive argument only
+    e = np.exp(-np.abs(a))
+    g = np.where(a >= 0, 1.0 / (1.0 + e), e / (1.0 + e))
     return a * g, g * (1.0 + a * (1.0 - g)), g * (1.0 - g) * (2.0 + a * (1.0 - 2.0 * g))
 
 
@@ -149,7 +151,8 @@ def head_compute(model, orc_desc, orc_exact, itypes, js, rij, jtypes, cell_rank,
         Ee, ge = orc_exact.compact(np.asarray(itypes)[rows], np.asarray(rij)[rows], np.asarray(jtypes)[rows])
         eatom[rows] = Ee
         g[rows] = ge
-    F = Oracle.forces(g, js, n_f)
+    # forces masked by natoms_energy; n_f is unused (jax_backend.py:414-418 wiring)
+    F = Oracle.forces(g, js, n_e)
     stress, _ = Oracle.stress(rij, g, volume, cell_rank)
     E = Oracle.energy(eatom, n_e)
     return E, F, stress, eatom, flags, g

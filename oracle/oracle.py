@@ -422,12 +422,17 @@ class Oracle:
 
     def calc_energy_forces_stress_padded(self, itypes, all_js, all_rijs, all_jtypes, cell_rank, volume,
                                          natoms_energy, natoms_force, form="compact", nthreads=0):
-        """FP64 restatement of jax_backend.py:374-460 (returns E, F[A,3], stress6, Ei, g)."""
+        """FP64 restatement of jax_backend.py:374-460 (returns E, F[A,3], stress6, Ei, g).
+
+        The reference passes natoms_energy positionally into the natoms_force
+        parameter of calc_energy_forces_stress (jax_backend.py:414-418), which
+        is the mask of accumulate_forces (:279-301): forces are masked by
+        natoms_energy and natoms_force has no effect."""
         if form == "compact":
             Ei, g = self.compact(itypes, all_rijs, all_jtypes, True, nthreads)
         else:
             Ei, g = self.tree_eval(itypes, all_rijs, all_jtypes, True, nthreads)
-        F = self.forces(g, all_js, natoms_force)
+        F = self.forces(g, all_js, natoms_energy)
         s6, _ = self.stress(all_rijs, g, volume, cell_rank)
         E = self.energy(Ei, natoms_energy)
         return E, F, s6, Ei, g

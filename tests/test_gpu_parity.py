@@ -85,12 +85,62 @@ def test_bcc2889_level16(jm, O):
 @pytest.mark.parametrize("variant", [1, 2, 4, 5])
 @pytest.mark.parametrize("rows", [24, 25])
 def test_masks_and_padding_rows(jm, O, rows, variant):
-    """natoms_energy / natoms_force below the row count (jax_backend.py:279-301, 447-452); an odd row count
-    leaves the paired-row kernels' last workgroup with one row."""
+    """natoms_energy below the row count and natoms_force different from it; an odd row count leaves the
+    paired-row kernels' last workgroup with one row.
+
+    The reference's padded entry hands natoms_energy to calc_energy_forces_stress's natoms_force parameter
+    (jax_backend.py:414-418), so accumulate_forces (:279-301) masks rows and Newton-3 targets by
+    natoms_energy and natoms_force changes nothing.  The expected forces are built here from the oracle's
+    slot gradients with that mask, apart from the oracle's padded entry, and a second call with another
+    natoms_force must give the same bits."""
     w = jm.configs.small_cluster(12, 2, 20, seed=9, cell=7.5, cutoff=5.0, max_nbr=64)
     nb = jm.neighbors.padded_neighbors(w.pos, w.types, w.cell, w.cutoff, w.max_nbr, rows=rows)
     print(f"[case] masks rows={rows} natoms=20 n_e=17 n_f=15 variant={variant}", flush=True)
-    _check(_run_gpu(jm, w, nb, n_e=17, n_f=15, variant=variant), _run_oracle(O, w, nb, n_e=17, n_f=15))
+    gpu = _run_gpu(jm, w, nb, n_e=17, n_f=15, variant=variant)
+    orc = oracle_for(O, w.params)
+    Ei, g = orc.compact(nb["itypes"], nb["rij"], nb["jtypes"])
+    js = nb["js"]
+    A, K = js.shape
+    F = np.zeros((A, 3))
+    for i in range(17):  # F_a = [a < n_e] sum_k g_ak - sum_{i < n_e, k: j_ik = a < n_e} g_ik
+        F[i] += g[i].sum(axis=0)
+        for k in range(K):
+            if js[i, k] < 17:
+                F[js[i, k]] -= g[i, k]
+    s6, _ = O.Oracle.stress(nb["rij"], g, w.volume, 3)
+    ref = dict(E=float(Ei[:17].sum()), F=F, s6=s6, eatom=Ei, g=g)
+    _check(gpu, ref)
+    assert np.all(gpu["F"][17:] == 0.0)
+    other = _run_gpu(jm, w, nb, n_e=17, n_f=rows, variant=variant)  # natoms_force has no effect
+    np.testing.assert_array_equal(other["F"], gpu["F"])
+    assert other["E"] == gpu["E"]
+
+
+def test_nonfinite_slot_is_an_error(jm, O):
+    """A NaN or Inf component in one neighbor vector makes the reference's energy and forces NaN
+    (jax_backend.py:83-95, 232).  Here the host-rows entry returns MTP_ERR_NONFINITE and the device entry
+    sets status bit 2 with a NaN energy; the next clean call is clean again (the flag is cleared)."""
+    w = jm.configs.small_cluster(10, 2, 16, seed=3, cell=7.0, cutoff=5.0, max_nbr=48)
+    nb = w.neighbors()
+    pot = jm.MTPPotential(w.params, max_atoms=w.natoms, max_nbr=48)
+    for bad in (np.nan, np.inf):
+        rij = nb["rij"].copy()
+        rij[3, 0, 1] = bad  # slot 0 of row 3 holds a typed neighbor
+        assert nb["jtypes"][3, 0] >= 0
+        print(f"[case] non-finite slot {bad}", flush=True)
+        with pytest.raises(jm.capi.MTPError) as e:
+            pot.compute_rows(w.natoms, rij.reshape(w.natoms, -1), nb["itypes"], nb["js"], nb["counts"],
+                             nb["jtypes"], w.volume)
+        assert e.value.code == jm.capi.MTP_ERR_NONFINITE
+        E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], rij, nb["jtypes"], 3,
+                                                               w.volume, w.natoms, w.natoms, return_all=True)
+        assert int(extra["status"].item()) & 2 and np.isnan(float(E.item()))
+    E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           w.volume, w.natoms, w.natoms, return_all=True)
+    assert float(extra["status"].item()) == 0.0
+    ref = _run_oracle(O, w, nb)
+    assert abs(float(E.item()) - ref["E"]) <= TOL * np.max(np.abs(ref["eatom"]))
+    pot.close()
 
 
 def test_repeated_calls_graph_replay(jm, O):
@@ -153,9 +203,9 @@ def test_repeated_calls_graph_replay(jm, O):
 
 def test_reverse_neighbor_reuse(jm, O):
     """One potential over a sequence of calls: the reverse-neighbor CSR of the
-    force gather is reused when js, natoms_force and the shape repeat, and
-    rebuilt when any of them changes (a different neighbor list of the same
-    shape, a smaller natoms_force)."""
+    force gather is reused when js, the force mask (natoms_energy) and the
+    shape repeat, and rebuilt when any of them changes (a different neighbor
+    list of the same shape, a smaller natoms_energy)."""
     wa = jm.configs.small_cluster(12, 2, 24, seed=31, cell=8.0, cutoff=5.0, max_nbr=64)
     wb = jm.configs.small_cluster(12, 2, 24, seed=32, cell=8.0, cutoff=5.0, max_nbr=64)
     wb.params = wa.params
@@ -164,11 +214,11 @@ def test_reverse_neighbor_reuse(jm, O):
     assert na["js"].shape == nb["js"].shape and not np.array_equal(na["js"], nb["js"])
     pot = jm.MTPPotential(wa.params, max_atoms=24, max_nbr=64)
     seq = [(wa, na, 24), (wa, na, 24), (wb, nb, 24), (wa, na, 24), (wa, na, 19), (wa, na, 19), (wb, nb, 19)]
-    for q, (w, nbr, n_f) in enumerate(seq):
-        print(f"[case] reuse step {q} n_f={n_f}", flush=True)
+    for q, (w, nbr, n_e) in enumerate(seq):
+        print(f"[case] reuse step {q} n_e={n_e}", flush=True)
         E, F, s6, extra = pot.calc_energy_forces_stress_padded(nbr["itypes"], nbr["js"], nbr["rij"], nbr["jtypes"], 3,
-                                                               w.volume, 24, n_f, return_all=True)
-        ref = _run_oracle(O, w, nbr, n_e=24, n_f=n_f)
+                                                               w.volume, n_e, 24, return_all=True)
+        ref = _run_oracle(O, w, nbr, n_e=n_e, n_f=24)
         assert _rel(F.cpu().numpy(), ref["F"]) <= TOL, q
         assert abs(float(E.item()) - ref["E"]) <= TOL * np.max(np.abs(ref["eatom"])), q
     pot.close()

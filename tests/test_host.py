@@ -76,6 +76,9 @@ def test_malformed_files(jm, tmp_path):
         # moment 25 is read (entry {0, 25, 1, 34}) before the entry that builds it
         "out_of_order": good.replace("{0, 25, 1, 34}, ", "").replace("{0, 0, 1, 25}", "{0, 25, 1, 34}, {0, 0, 1, 25}"),
         "bad_number": good.replace("min_dist = ", "min_dist = x"),
+        # a count declared again after the radial block leaves the table short of the final sizes
+        "late_radial_count": good + "radial_basis_size = 9\n",
+        "late_species_count": good.replace("species_count = 1", "species_count = 1") + "species_count = 2\n",
     }
     for name, text in cases.items():
         path = tmp_path / f"{name}.mtp"
@@ -83,6 +86,15 @@ def test_malformed_files(jm, tmp_path):
             path.write_text(text)
         rc = L.mtp_check_file(str(path).encode(), 64, None, None, None, None, None, None, None)
         assert rc == jm.capi.MTP_ERR_FILE, name
+
+
+def test_header_variant_count_matches_library(jm):
+    """MTP_NUM_VARIANTS sizes mtp_autotune's ms_per_variant: it must equal the library's variant count."""
+    header = (ROOT / "include" / "mtp_hip.h").read_text()
+    n = int(re.search(r"#define\s+MTP_NUM_VARIANTS\s+(\d+)", header).group(1))
+    assert n == len(jm.MTPPotential.VARIANTS)
+    for v in jm.MTPPotential.VARIANTS:
+        assert re.search(rf"\b{v}\b", header.split("#define MTP_NUM_VARIANTS")[0].rsplit("/*", 1)[1]), v
 
 
 def test_per_atom_values_need_a_handle(jm):

@@ -209,6 +209,19 @@ int mtp_jacobian_tapply_device(int nrows, int nslots, int S, int n_f, const doub
 int mtp_gather_forces_device(mtp_ctx *ctx, int nrows, int nslots, const int *d_js, const double *d_g, int n_f,
                              double *d_forces, void *stream);
 
+/* The same gather without a context and without allocating: for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  No counterpart in the
+ * reference.  The reverse-neighbor CSR lives in a static per-device scratch
+ * array of the library (18.4 MB per GPU, held until the process ends), so
+ * nrows <= 131072 and nrows * nslots <= 4194304 (MTP_ERR_CAPACITY beyond).
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other streams, other threads) race.  The CSR is rebuilt on every
+ * call (no reuse across calls).  Errors: mtp_global_error().  Enqueued on
+ * stream (NULL = the null stream); results are ready after it completes. */
+int mtp_gather_forces_noalloc(int nrows, int nslots, const int *d_js, const double *d_g, int n_f, double *d_forces,
+                              void *stream);
+
 /* ---- distilled head ---------------------------------------------------
  * No counterpart in the reference.  An MLP over the basis values of a
  * low-level potential `desc` (level 12: 29 values) stands in for a

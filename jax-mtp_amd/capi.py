@@ -52,6 +52,7 @@ SIGNATURES = {
     "mtp_jacobian_apply_device": (_I, [_I, _I, _I, _P, _P, _P, _P]),
     "mtp_jacobian_tapply_device": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "mtp_gather_forces_device": (_I, [_P, _I, _I, _P, _P, _I, _P, _P]),
+    "mtp_gather_forces_noalloc": (_I, [_I, _I, _P, _P, _I, _P, _P]),
     "mtp_head_theta_size": (_I, []),
     "mtp_head_open": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(_P)]),
     "mtp_head_close": (None, [_P]),

@@ -1205,6 +1205,87 @@ int mtp_gather_forces_device(mtp_ctx* ctx, int nrows, int nslots, const int* d_j
     return MTP_OK;
 }
 
+}  // extern "C"
+
+// Scratch of mtp_gather_forces_noalloc: the reverse-neighbor CSR (counts and
+// cursors, scan tiles, offsets, slot lists) of at most kNoAllocMaxRows rows
+// and kNoAllocMaxSlots row slots, in one file-scope device array.  Every GPU
+// the library runs on holds its own copy (18.4 MB) until the process ends.
+// Host code never uses the array's name as an address: the device address
+// comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr int kNoAllocMaxRows = 131072;
+constexpr long long kNoAllocMaxSlots = 4194304;  // rows x slots per row
+constexpr int kNoAllocCnt = 2 * kNoAllocMaxRows + 2 * ((kNoAllocMaxRows + 4095) / 4096) + 16;
+constexpr long long kNoAllocInts = kNoAllocCnt + (kNoAllocMaxRows + 1) + kNoAllocMaxSlots;
+constexpr int kNoAllocMaxDevices = 64;
+
+__device__ __align__(256) int scratch[kNoAllocInts];
+
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this device's scratch (looked up on the first call per device).
+hipError_t noalloc_scratch(int** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int mtp_gather_forces_noalloc(int nrows, int nslots, const int* d_js, const double* d_g, int n_f, double* d_forces,
+                              void* stream)
+{
+    if (nrows < 0 || nslots <= 0) {
+        set_global_error("mtp_gather_forces_noalloc: rows must be >= 0 and slots > 0");
+        return MTP_ERR_ARG;
+    }
+    if (nrows > kNoAllocMaxRows || (long long)nrows * nslots > kNoAllocMaxSlots) {
+        set_global_error("mtp_gather_forces_noalloc: " + std::to_string(nrows) + " rows x " + std::to_string(nslots) +
+                         " slots exceed the static scratch (at most " + std::to_string(kNoAllocMaxRows) +
+                         " rows and " + std::to_string(kNoAllocMaxSlots) + " slots)");
+        return MTP_ERR_CAPACITY;
+    }
+    if (nrows == 0) return MTP_OK;
+    if (!d_js || !d_g || !d_forces) {
+        set_global_error("mtp_gather_forces_noalloc: null device array");
+        return MTP_ERR_ARG;
+    }
+    int* base = nullptr;
+    hipError_t e = noalloc_scratch(&base);
+    if (e != hipSuccess) {
+        set_global_error(std::string("mtp_gather_forces_noalloc: scratch address: ") + hipGetErrorString(e));
+        return MTP_ERR_HIP;
+    }
+    AtomArgs A{};
+    A.nrows = nrows; A.nslots = nslots; A.js = d_js; A.g = const_cast<double*>(d_g);
+    ForceArgs F{};
+    F.n_f = n_f; F.F = d_forces;
+    F.cnt = base;                           // counts and cursors, cleared by the gather's own fill kernel
+    F.off = base + kNoAllocCnt;
+    F.rev = base + kNoAllocCnt + (kNoAllocMaxRows + 1);
+    F.csr_state = nullptr;                  // no reuse across calls: every call builds its CSR
+    e = launch_gather(A, F, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_global_error(std::string("mtp_gather_forces_noalloc: ") + hipGetErrorString(e));
+        return MTP_ERR_HIP;
+    }
+    return MTP_OK;
+}
+
 const char* mtp_head_last_error(const mtp_head* h) { return h ? h->err.c_str() : mtp_global_error(); }
 
 int mtp_head_theta_size(void) { return kHeadTheta; }

@@ -55,6 +55,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profile_values(name, kernel):
+    """key=value lines of a committed rocprofv3 summary under profiles/ (None when absent or for another kernel)."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    kv = dict(line.split("=", 1) for line in f.read_text().splitlines() if "=" in line and not line.startswith("#"))
+    if kv.get("kernel", "").strip() != kernel:
+        return None
+    return {k: v.strip() for k, v in kv.items()}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+PEAK_LDS_TBS = 78.6    # 128 B/clk/CU (64-bit reads) x 256 CUs x 2.4 GHz
+PEAK_HBM_TBS = 8.0
+
+
 def traffic_bytes(workload, kernel):
     """HBM bytes per atom-kernel launch from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
@@ -270,6 +295,29 @@ def main():
                               info["basic_count"], info["times_count"], info["scalar_count"])
     achieved = flops / (r["atom_kernel_ms"] * 1e-3) / 1e12
     value = w.natoms / (r["ms_per_step"] * 1e-3)
+    kname = {0: "atom_kernel<double>", 1: "atom_kernel_v2<double, 256>", 2: "atom_kernel_v2<double, 512>",
+             3: "atom_kernel_v2<double, 64>", 4: "atom_kernel_pair<double, 1024>",
+             5: "atom_kernel_pair<double, 512>"}[best]
+    traffic = traffic_bytes(w.name, pot.VARIANTS[best])
+    cnt = profile_values(f"atom_kernel_{w.name}_counters.txt", kname)
+    kt = r["atom_kernel_ms"] * 1e-3
+    roofs = {"fp64_valu": {"achieved_algorithmic": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                           "frac": achieved / PEAK_FP64_TFLOPS}}
+    if traffic is not None:
+        roofs["hbm"] = {"achieved": traffic / kt / 1e12, "peak": PEAK_HBM_TBS, "unit": "TB/s",
+                        "frac": traffic / kt / 1e12 / PEAK_HBM_TBS}
+    if cnt is not None:
+        # counter-derived rates: the counts of the committed rocprofv3 passes over this time
+        fc = float(cnt["fp64_flop_counter"])
+        roofs["fp64_valu"].update(achieved_counter=fc / kt / 1e12, frac_counter=fc / kt / 1e12 / PEAK_FP64_TFLOPS,
+                                  flop_counter_per_launch=fc)
+        lb = float(cnt["lds_bytes"])
+        roofs["lds"] = {"achieved": lb / kt / 1e12, "peak": PEAK_LDS_TBS, "unit": "TB/s",
+                        "frac": lb / kt / 1e12 / PEAK_LDS_TBS, "busy_frac": float(cnt["lds_busy_frac"]),
+                        "bytes_per_launch": lb}
+        roofs["l2"] = {"achieved": float(cnt["l2_read_bytes"]) / kt / 1e12, "unit": "TB/s",
+                       "bytes_per_launch": float(cnt["l2_read_bytes"])}
+    nearest = max((k for k in roofs if "frac" in roofs[k]), key=lambda k: roofs[k]["frac"])
     line = {
         "metric": METRIC,
         "value": value,
@@ -288,8 +336,13 @@ def main():
                    "radial_basis_size": info["radial_basis_size"], "radial_funcs_count": info["radial_funcs_count"],
                    "cutoff": w.cutoff, "max_nbr": K, "mean_neighbors": float(nb["counts"].mean()),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_bytes(w.name, pot.VARIANTS[best]),
+        "roofline": {"bound": {"fp64_valu": "valu"}.get(nearest, nearest),
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "achieved_counter": roofs["fp64_valu"].get("achieved_counter"),
+                     "lds": roofs.get("lds"), "roofs": roofs,
+                     "note": "achieved/peak/frac: algorithmic FP64 FLOP/s of the atom kernel (HIP events, this run) "
+                             "against the FP64 vector peak; bound: the roof with the largest fraction",
                      "kernel": f"atom_kernel ({pot.VARIANTS[best]})", "kernel_ms": r["atom_kernel_ms"],
                      "algorithmic_flop_per_launch": flops,
                      "algorithmic_bytes_per_launch": algorithmic_bytes(nb["counts"][: w.natoms])},
@@ -302,9 +355,13 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, len(os.sched_getaffinity(0)))
+        # every host CPU this process may run on (the OpenMP thread count when it is set)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         log(f"[bench] cpu baseline on {threads} threads")
         cb, ferr, Fref = cpu_baseline(w, nb, r["F"], threads)
+        cb["cpu_model"] = cpu_model()
+        cb["host_cpus_visible"] = os.cpu_count()
+        cb["cpus_allowed"] = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cb
         f32 = results["radial_fp32"]["F"]
         line["force_err_vs_fp64_oracle"] = {

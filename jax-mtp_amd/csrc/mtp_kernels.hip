@@ -295,16 +295,6 @@ size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
 
-// One contraction layer.  Each wavefront walks its own entry stream (two
-// entries per lane and step, see DevPlan), with the table loads of the next
-// kStreamUnroll steps in flight while the current ones are summed (two
-// register sets used in turn).  A group ends on the same step on all lanes
-// (flagged in the step's fourth word), so the loop has no per-lane control
-// flow.  Operand offsets are byte offsets: a from the LDS base, b from it
-// (forward) or from the adjoints (reverse).  A layer's stores never alias
-// its operands (targets have a higher depth than their operands in the
-// forward pass, a lower depth than the targets they read in the reverse
-// pass).
 template <bool REV>
 __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, const char* bb, char* lb, double& acc)
 {
@@ -501,6 +491,90 @@ __device__ __forceinline__ void store_basic(const DevPlan& P, int t, const doubl
         if (dst[mu] >= 0) m[ms * dst[mu]] = acc[mu];
 }
 
+// The same over slot pairs with 128-bit LDS reads: px, py, pz hold the
+// powers of slots (k, k + 1) in one 16-byte pair, f[k * Rp + mu] the radial
+// values with mu pairs in 16 bytes (the same address on every lane of the
+// wavefront), and nvp (even) slots, the last one a zero slot when the row's
+// count is odd.
+template <int NMU>
+__device__ __forceinline__ void basic_moment_sum2(const double* px, const double* py, const double* pz,
+                                                  const double* f, int Rp, int nvp, double* acc)
+{
+    constexpr int NP = (NMU + 1) / 2;                       // mu pairs
+    constexpr int UB = NMU <= 2 ? 4 : (NMU <= 4 ? 2 : 1);   // slot pairs per batch
+    int k = 0;
+    for (; k + 2 * UB <= nvp; k += 2 * UB) {
+        double2 a[UB], b[UB], c[UB], f0[UB][NP], f1[UB][NP];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int kk = k + 2 * u;
+            a[u] = *reinterpret_cast<const double2*>(px + kk);
+            b[u] = *reinterpret_cast<const double2*>(py + kk);
+            c[u] = *reinterpret_cast<const double2*>(pz + kk);
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                f0[u][j] = *reinterpret_cast<const double2*>(f + (size_t)kk * Rp + 2 * j);
+                f1[u][j] = *reinterpret_cast<const double2*>(f + (size_t)(kk + 1) * Rp + 2 * j);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const double P0 = a[u].x * b[u].x * c[u].x;
+            const double P1 = a[u].y * b[u].y * c[u].y;
+#pragma unroll
+            for (int mu = 0; mu < NMU; ++mu) {
+                const double g0 = (mu & 1) ? f0[u][mu >> 1].y : f0[u][mu >> 1].x;
+                const double g1 = (mu & 1) ? f1[u][mu >> 1].y : f1[u][mu >> 1].x;
+                acc[mu] = fma(g1, P1, fma(g0, P0, acc[mu]));
+            }
+        }
+    }
+    for (; k < nvp; k += 2) {
+        const double2 a = *reinterpret_cast<const double2*>(px + k);
+        const double2 b = *reinterpret_cast<const double2*>(py + k);
+        const double2 c = *reinterpret_cast<const double2*>(pz + k);
+        const double P0 = a.x * b.x * c.x, P1 = a.y * b.y * c.y;
+#pragma unroll
+        for (int mu = 0; mu < NMU; ++mu) acc[mu] = fma(f[(size_t)(k + 1) * Rp + mu], P1, fma(f[(size_t)k * Rp + mu], P0, acc[mu]));
+    }
+}
+
+// Basic moments of one row of the single-row kernels (paired-slot layout of
+// basic_moment_sum2; power table row stride Kp2 = pw_stride(K)).
+template <int NW>
+__device__ __forceinline__ void basic_moments2(const DevPlan& P, int w, int lane, int nvp, const double* pw, int Kp2,
+                                               int np1, const double* f, int Rp, double* m)
+{
+    const int ngrp = (P.n_mono + kWave - 1) / kWave;
+    double acc[kMaxR];
+    for (int g = w; g < ngrp; g += NW) {
+        const int t = g * kWave + lane;
+        if (t < P.n_mono) {
+            const I4 mo = P.mono[t];
+            const int gmax = __builtin_amdgcn_readfirstlane(mo.w >> 24);
+            const double* px = pw + (size_t)mo.x * Kp2;
+            const double* py = pw + (size_t)(np1 + mo.y) * Kp2;
+            const double* pz = pw + (size_t)(2 * np1 + mo.z) * Kp2;
+#pragma unroll
+            for (int mu = 0; mu < kMaxR; ++mu) acc[mu] = 0.0;
+            switch (gmax) {
+                case 0: basic_moment_sum2<1>(px, py, pz, f, Rp, nvp, acc); break;
+                case 1: basic_moment_sum2<2>(px, py, pz, f, Rp, nvp, acc); break;
+                case 2: basic_moment_sum2<3>(px, py, pz, f, Rp, nvp, acc); break;
+                case 3: basic_moment_sum2<4>(px, py, pz, f, Rp, nvp, acc); break;
+                case 4: basic_moment_sum2<5>(px, py, pz, f, Rp, nvp, acc); break;
+                case 5: basic_moment_sum2<6>(px, py, pz, f, Rp, nvp, acc); break;
+                case 6: basic_moment_sum2<7>(px, py, pz, f, Rp, nvp, acc); break;
+                default: basic_moment_sum2<8>(px, py, pz, f, Rp, nvp, acc); break;
+            }
+            store_basic(P, t, acc, m, 1);
+        }
+    }
+    __syncthreads();
+}
+
+
 template <int NW>
 __device__ __forceinline__ void basic_moments(const DevPlan& P, int w, int lane, int nv, const double* pw, int Kp,
                                               int np1, const double* f, double* m, int ms)
@@ -649,35 +723,39 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         // same.)
     } else {
         const int np1 = P.nu_max + 1;
-        const int Kp = P.k_layout | 1;
+        const int Kp2 = pw_stride(P.k_layout);
+        const int Rp = f_stride(P.R);
+        const int nvp = (nv + 1) & ~1;  // slots in pairs; an odd count gets a zero slot
         double* pw = dm;
-        // radial values: one thread per (slot, mu)
+        // radial values f[k * Rp + mu]: one thread per (slot, mu); zero for
+        // the padding (mu >= R, the pair's zero slot)
         if (P.coef_lds) {
-            for (int i = tid; i < nv * P.R; i += NT) {
-                const int c = i / P.R, mu = i - c * P.R;
-                double fv, dfv;
-                radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N, fv, dfv);
+            for (int i = tid; i < nvp * Rp; i += NT) {
+                const int c = i / Rp, mu = i - c * Rp;
+                double fv = 0.0, dfv;
+                if (c < nv && mu < P.R)
+                    radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N,
+                                             fv, dfv);
                 f[i] = fv;
             }
         } else {
-            for (int i = tid; i < nv * P.R; i += NT) {
-                const int c = i / P.R, mu = i - c * P.R;
-                double fv, dfv;
-                radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
+            for (int i = tid; i < nvp * Rp; i += NT) {
+                const int c = i / Rp, mu = i - c * Rp;
+                double fv = 0.0, dfv;
+                if (c < nv && mu < P.R) radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
                 f[i] = fv;
             }
         }
-        for (int t = tid; t < nv * 3; t += NT) {
+        for (int t = tid; t < nvp * 3; t += NT) {
             const int k = t / 3, q = t - 3 * k;
-            const double inv = 1.0 / rr[k];
-            const double u = (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * inv;
-            double* col = pw + (size_t)q * np1 * Kp + k;
+            const double u = k < nv ? (q == 0 ? vx[k] : (q == 1 ? vy[k] : vz[k])) * (1.0 / rr[k]) : 0.0;
+            double* col = pw + (size_t)q * np1 * Kp2 + k;
             double v = 1.0;
             col[0] = 1.0;
-            for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp] = v; }
+            for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp2] = v; }
         }
         __syncthreads();
-        basic_moments<NW>(P, tid / kWave, tid & (kWave - 1), nv, pw, Kp, np1, f, m, 1);
+        basic_moments2<NW>(P, tid / kWave, tid & (kWave - 1), nvp, pw, Kp2, np1, f, Rp, m);
         // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
         // region, free once the power table and f are consumed)
         for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;

@@ -67,9 +67,10 @@ struct AtomLds2 {
 //   zero [kLdsM] lds[0] = 0.0: operand of the padding entries of the
 //               contraction streams and the sink of lanes without an item
 //   m   [n_live] moments [0, n_live) (basic or read as operands)
-//   dm  [dlen]  A/B: power table (row stride K | 1, odd so that a wave's
-//               reads of different powers fall in different LDS banks),
-//               then f[k * R + mu]; C/D: the values of the never-operand
+//   dm  [dlen]  A/B: power table (row stride pw_stride(K): slot pairs,
+//               an odd number of them, so that a wave's reads of different
+//               powers fall in different LDS banks), then f[k * Rp + mu]
+//               (Rp = f_stride(R)); C/D: the values of the never-operand
 //               moments (m[x] for x >= n_live aliases dm[x - n_live], dead
 //               after D) and, at the end, the energy reduction; E/F:
 //               adjoints, of which F reads only the basic ones [0, B): from
@@ -80,16 +81,25 @@ struct AtomLds2 {
 //               coefficients of the row's species, when staged)
 // Without tables (a per-row pass over cached moments, which skips phases
 // A's radial values and B) the power table and f are left out.
+// Row stride (doubles) of the single-row kernels' power table: slots in
+// 16-byte pairs, and an odd number of pairs per row, so that a wave's
+// 128-bit reads of different powers fall in different LDS banks.
+__host__ __device__ constexpr int pw_stride(int K) { return 2 * (((K + 1) / 2) | 1); }
+// Row stride of their radial values f[k][mu]: R rounded up to even (mu pairs in 16 bytes).
+__host__ __device__ constexpr int f_stride(int R) { return R + (R & 1); }
+
 __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B, int K, int R, int nu_max,
                                                     int n_parts, int nt, int max_partials, int xi_len,
                                                     bool tables = true, int coef_len = 0)
 {
     AtomLds2 L;
-    const int pw_len = tables ? (K | 1) * 3 * (nu_max + 1) : 0;
-    const int mlen = n_live > 0 ? n_live : 1;
+    const int pw_len = tables ? pw_stride(K) * 3 * (nu_max + 1) : 0;
+    const int f_len = f_stride(R) * (K + 1);  // slots padded to an even count
+    // even, so that the adjoint region (power table, f) starts 16-byte aligned
+    const int mlen = n_live > 0 ? (n_live + 1) & ~1 : 2;
     const int g0 = B > 3 * K ? B : 3 * K;
     int dlen = amc;
-    if (tables && dlen < pw_len + R * K) dlen = pw_len + R * K;
+    if (tables && dlen < pw_len + f_len) dlen = pw_len + f_len;
     if (dlen < (amc - n_live) + nt) dlen = (amc - n_live) + nt;
     if (dlen < g0 + n_parts * K * 3 + 2 * R * K) dlen = g0 + n_parts * K * 3 + 2 * R * K;
     L.m = kLdsM;

@@ -59,6 +59,33 @@ def test_write_read_roundtrip_is_stable(jm, tmp_path):
     assert p.level == 12 and p.species_count == 2
 
 
+def test_parameter_vector_groups(jm):
+    """The flat parameter vector is (scaling,) moment, species, radial in that order (data.py:60-102);
+    the setter inverts the getter, and initialize draws only the missing groups within their ranges."""
+    p = jm.mtp_io.untrained(10)
+    p.species_count = 2
+    p.initialize(np.random.default_rng(1))
+    assert np.all(np.abs(p.moment_coeffs) <= 5) and np.all(np.abs(p.species_coeffs) <= 5)
+    assert p.radial_coeffs.shape == (2, 2, p.radial_funcs_count, p.radial_basis_size)
+    assert np.all(np.abs(p.radial_coeffs) <= 0.1)
+    flat = p.parameters
+    S, R = len(p.moment_coeffs), p.radial_coeffs.size
+    np.testing.assert_array_equal(flat[:S], p.moment_coeffs)
+    np.testing.assert_array_equal(flat[S:S + 2], p.species_coeffs)
+    np.testing.assert_array_equal(flat[S + 2:], p.radial_coeffs.ravel())
+    q = p.copy()
+    q.optimized = ["scaling"] + q.optimized
+    q.parameters = np.concatenate([[1.25], flat * 2])
+    assert q.scaling == 1.25
+    np.testing.assert_array_equal(q.radial_coeffs, 2 * p.radial_coeffs)
+    keep = p.moment_coeffs.copy()
+    p.initialize(np.random.default_rng(2))  # nothing missing: nothing redrawn
+    np.testing.assert_array_equal(p.moment_coeffs, keep)
+    # a count declared again after the radial block is rejected by the Python reader as well
+    with pytest.raises(ValueError):
+        jm.parse_mtp_text(jm.format_mtp(p) + "radial_basis_size = 3\n")
+
+
 def test_untrained_file_rejected_by_library(jm):
     path = jm.mtp_io.POTENTIALS / "untrained_10.mtp"
     rc = jm.capi.lib().mtp_check_file(str(path).encode(), 64, None, None, None, None, None, None, None)

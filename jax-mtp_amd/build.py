@@ -20,6 +20,8 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libmtp_hip.so"
 MD = PKG / "md" / "md_nve"
+# the pair style's own source compiled against the LAMMPS stand-ins of tests/lammps_stub (its test driver)
+PAIR_CHECK = ROOT / "tests" / "lammps_stub" / "run_pair"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MTP_OFFLOAD_ARCH", "gfx950")
@@ -42,9 +44,14 @@ def _compile(src: str) -> pathlib.Path:
 
 def build(force: bool = True) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
-    deps = [CSRC / s for s in SOURCES + HEADERS] + [ROOT / "include" / "mtp_hip.h", PKG / "md" / "md_nve.cpp"]
-    if (not force and LIB.exists() and MD.exists()
-            and min(LIB.stat().st_mtime, MD.stat().st_mtime) >= max(d.stat().st_mtime for d in deps)):
+    stub = ROOT / "tests" / "lammps_stub"
+    pair_src = [PKG / "lammps" / "pair_jax_mtp.cpp", PKG / "lammps" / "pair_jax_mtp.h", stub / "run_pair.cpp",
+                stub / "lammps_stub.h"]
+    deps = ([CSRC / s for s in SOURCES + HEADERS] + [ROOT / "include" / "mtp_hip.h", PKG / "md" / "md_nve.cpp"]
+            + [p for p in pair_src if p.exists()])
+    outs = [LIB, MD] + ([PAIR_CHECK] if stub.exists() else [])
+    if (not force and all(o.exists() for o in outs)
+            and min(o.stat().st_mtime for o in outs) >= max(d.stat().st_mtime for d in deps)):
         return LIB
     try:
         with cf.ThreadPoolExecutor(max_workers=4) as ex:
@@ -57,6 +64,12 @@ def build(force: bool = True) -> pathlib.Path:
         subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{CSRC}",
                         str(PKG / "md" / "md_nve.cpp"), f"-L{PKG}", "-lmtp_hip", "-Wl,-rpath,$ORIGIN/..",
                         "-o", str(MD)], check=True, capture_output=True, text=True)
+        if stub.exists():
+            subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", f"-I{stub}", f"-I{PKG / 'lammps'}",
+                            f"-I{ROOT / 'include'}", f"-I{CSRC}", str(stub / "run_pair.cpp"),
+                            str(PKG / "lammps" / "pair_jax_mtp.cpp"), f"-L{PKG}", "-lmtp_hip",
+                            "-Wl,-rpath,$ORIGIN/../../jax-mtp_amd", "-o", str(PAIR_CHECK)],
+                           check=True, capture_output=True, text=True)
     except subprocess.CalledProcessError as e:
         sys.stderr.write(e.stdout or "")
         sys.stderr.write(e.stderr or "")

@@ -16,7 +16,8 @@
        xx yy zz xy xz yz, energy units); the reference added its
        volume-normalised stress in another order (pair_jax_mtp.cpp:203-205,
        README:40,47).  Set JAXMTP_VIRIAL_COMPAT=1 to get the old behaviour;
-     * natoms > MAX_ATOMS and neighbor truncation are reported, not silent.
+     * natoms > MAX_ATOMS and neighbor truncation are reported, not silent;
+     * the cell volume of a triclinic box is xprd * yprd * zprd.
 ------------------------------------------------------------------------- */
 #include "pair_jax_mtp.h"
 
@@ -117,11 +118,11 @@ void PairJaxMTP::compute(int eflag, int vflag)
     error->warning(FLERR, "jax/mtp: {} in-cutoff neighbors dropped (MAX_ATOMS_PER_NEIGHBOR = {})", truncated,
                    max_neighbors);
 
-  const double volume = domain->triclinic
-      ? (domain->h[0] * (domain->h[4] * domain->h[8] - domain->h[5] * domain->h[7]) -
-         domain->h[1] * (domain->h[3] * domain->h[8] - domain->h[5] * domain->h[6]) +
-         domain->h[2] * (domain->h[3] * domain->h[7] - domain->h[4] * domain->h[6]))
-      : (domain->xprd * domain->yprd * domain->zprd);
+  // LAMMPS's h = (xprd, yprd, zprd, yz, xz, xy) is upper triangular, so the
+  // cell volume is xprd * yprd * zprd for triclinic boxes too.  (The
+  // reference, pair_jax_mtp.cpp:179-181, took a 3x3 determinant of h[0..8],
+  // reading past the six elements of Domain::h for triclinic boxes.)
+  const double volume = domain->xprd * domain->yprd * domain->zprd;
 
   double energy = 0.0, stress6[6], virial6[6];
   const int rc = mtp_compute(ctx, nlocal, (int) nneigh_total, pos_rows.data(), atom_types.data(), nbr_rows.data(),

@@ -541,6 +541,30 @@ def J_along(orc: Oracle, itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS,
     return np.array(out)
 
 
+def J_direction(orc: Oracle, itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS, v, ts, nthreads=0):
+    """J at theta + t v for each t, v = (moment, species, radial) direction in the order of param_grad_exact."""
+    alpha, Delta = _alpha_delta(itypes, js, rij, jtypes, conf_off, volumes, gE, gF, gS)
+    k = orc._keep
+    arrs = (k["moment"], k["species"], k["radial"].reshape(-1))
+    v = np.asarray(v, dtype=np.float64)
+    parts, o = [], 0
+    for a in arrs:
+        parts.append(v[o:o + a.size])
+        o += a.size
+    if o != v.size:
+        raise ValueError("direction has the wrong length")
+    x0 = [a.copy() for a in arrs]
+    out = []
+    for t in ts:
+        for a, a0, d in zip(arrs, x0, parts):
+            a[:] = a0 + t * d
+        Ei, g = orc.compact(itypes, rij, jtypes, True, nthreads)
+        out.append(float(alpha @ Ei + np.sum(g * Delta)))
+    for a, a0 in zip(arrs, x0):
+        a[:] = a0
+    return np.array(out)
+
+
 # ---------------------------------------------------------------------------
 # Neighbor packing restatement (pair_jax_mtp.cpp, zero_overhead.cpp)
 # ---------------------------------------------------------------------------

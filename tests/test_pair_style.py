@@ -1,0 +1,155 @@
+"""The LAMMPS pair style's own source, compiled against the LAMMPS stand-ins
+of tests/lammps_stub and driven the way the reference's example input
+(test_lammps.in:7-14) uses it: read_data bulk_ni3al_small.data (107 atoms),
+replicate 3 3 3 (2,889 atoms), pair_style jax/mtp Ni3Al-12g.mtp 2889 60 5.0,
+pair_coeff * *.  The example's trained potential is the reference's
+training/training_data/Ni3Al-12g.mtp.
+
+Checked on the GPU, against the FP64 oracle on the same positions (1e-10
+relative): the energy, the forces added into atom->f, the global virial
+(LAMMPS convention, and the reference's stress6 under
+JAXMTP_VIRIAL_COMPAT=1), the per-atom energies and virials (eflag_atom,
+vflag_atom); then an NVE run through the pair style (energy drift, forces
+along the trajectory).  On the CPU: the harness builds, and the pair
+style's argument checks raise the reference's errors.
+"""
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+RUN = ROOT / "tests" / "lammps_stub" / "run_pair"
+DATA = ROOT / "tests" / "golden" / "bulk_ni3al_small.data"
+MASSES = ["26.98", "58.69"]  # test_lammps.in:8-9 (type 1 Al, type 2 Ni)
+
+
+def _pot(jm):
+    return jm.mtp_io.POTENTIALS / "Ni3Al-12g.mtp"
+
+
+def _run(jm, out, steps=0, dt=0.001, T=0.0, dump_every=0, max_atoms=2889, max_nbr=60, cutoff=5.0, env=None,
+         timeout=300):
+    cmd = [str(RUN), str(_pot(jm)), str(DATA), "3", "3", "3", str(max_atoms), str(max_nbr), str(cutoff), str(steps),
+           str(dt), str(T), "0", str(out), str(dump_every), *MASSES]
+    e = dict(os.environ)
+    e.pop("JAXMTP_VIRIAL_COMPAT", None)
+    e.update(env or {})
+    return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=e)
+
+
+def _read_dump(path):
+    raw = path.read_bytes()
+    n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    o = 4
+    L = np.frombuffer(raw[o:o + 24], dtype=np.float64); o += 24
+    t = np.frombuffer(raw[o:o + 4 * n], dtype=np.int32); o += 4 * n
+    x = np.frombuffer(raw[o:o + 24 * n], dtype=np.float64).reshape(n, 3); o += 24 * n
+    F = np.frombuffer(raw[o:o + 24 * n], dtype=np.float64).reshape(n, 3)
+    return L, t, x, F
+
+
+def _read_step0(path):
+    raw = path.read_bytes()
+    n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    v = np.frombuffer(raw[4:], dtype=np.float64)
+    return float(v[0]), v[1:7].copy(), v[7:7 + n].copy(), v[7 + n:7 + 7 * n].reshape(n, 6).copy()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def _oracle_case(jm, O, x, t, L):
+    """Oracle energies, slot gradients and forces of the dumped configuration (full periodic list)."""
+    nb = jm.neighbors.padded_neighbors(x, t - 1, L, 5.0, 60)
+    orc = O.Oracle(O.read_mtp(str(_pot(jm))))
+    n = len(x)
+    E, F, s6, Ei, g = orc.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           float(np.prod(L)), n, n)
+    W = np.einsum("ika,ikb->iab", nb["rij"], g)  # per-row sum_k r_ik (x) g_ik
+    return E, F, s6, Ei, W, nb
+
+
+def _lammps_virial(W):
+    """-sym(W) in LAMMPS order xx yy zz xy xz yz (rows or total)."""
+    return -np.stack([W[..., 0, 0], W[..., 1, 1], W[..., 2, 2], 0.5 * (W[..., 0, 1] + W[..., 1, 0]),
+                      0.5 * (W[..., 0, 2] + W[..., 2, 0]), 0.5 * (W[..., 1, 2] + W[..., 2, 1])], axis=-1)
+
+
+def test_harness_is_built():
+    assert RUN.exists(), "build with python jax-mtp_amd/build.py"
+
+
+def test_example_data_fixture():
+    """bulk_ni3al_small.data: 107 atoms, 2 types, cubic 11.01 A box (the reference example's input)."""
+    text = DATA.read_text()
+    assert "107 atoms" in text and "2 atom types" in text and "11.010000000 xlo xhi" in text
+
+
+@pytest.mark.gpu
+def test_pair_style_reference_example(jm, O, tmp_path):
+    print("[case] pair style, Ni3Al example: 2889 atoms, K = 60, cutoff 5.0, one force call", flush=True)
+    r = _run(jm, tmp_path / "a", dump_every=1)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0
+    assert "JAX-MTP (MI355X HIP evaluator) initialized" in r.stdout
+    L, t, x, F = _read_dump(tmp_path / "a.dump000000")
+    E, virial, eatom, vatom = _read_step0(tmp_path / "a.step0")
+    n = len(x)
+    assert n == 2889 and np.allclose(L, 33.03)
+    E_r, F_r, s6_r, Ei_r, W_r, nb = _oracle_case(jm, O, x, t, L)
+    assert nb["counts"].max() <= 60  # no truncation in the example
+    print(f"  E {E:.15e} oracle {E_r:.15e}; force rel err {_rel(F, F_r):.2e}", flush=True)
+    assert abs(E - E_r) <= 1e-10 * abs(E_r)
+    assert _rel(F, F_r) <= 1e-10
+    assert np.abs(F.sum(axis=0)).max() <= 1e-10 * np.abs(F).max()
+    assert _rel(virial, _lammps_virial(W_r.sum(axis=0))) <= 1e-10
+    assert _rel(eatom, Ei_r) <= 1e-10
+    assert _rel(vatom, _lammps_virial(W_r)) <= 1e-10
+    np.testing.assert_allclose(eatom.sum(), E, rtol=1e-12)
+    np.testing.assert_allclose(vatom.sum(axis=0), virial, rtol=1e-10, atol=1e-10 * np.abs(virial).max())
+    # the reference's convention: its stress6 added raw into virial[] (pair_jax_mtp.cpp:203-205)
+    r2 = _run(jm, tmp_path / "b", env={"JAXMTP_VIRIAL_COMPAT": "1"})
+    assert r2.returncode == 0, r2.stderr
+    _, virial_compat, _, _ = _read_step0(tmp_path / "b.step0")
+    assert _rel(virial_compat, s6_r) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_pair_style_nve_on_example(jm, O, tmp_path):
+    """500 steps of 1 fs NVE from the example's structure (velocities at 100 K, the example's
+    temperature) through the pair style: total-energy drift and forces along the trajectory."""
+    steps, every = 500, 250
+    print(f"[case] pair style NVE on the Ni3Al example, {steps} steps", flush=True)
+    r = _run(jm, tmp_path / "md", steps=steps, T=100.0, dump_every=every, timeout=600)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    th = np.loadtxt(tmp_path / "md.thermo")
+    assert th.shape[0] == steps + 1
+    n = 2889
+    drift = float(np.max(np.abs(th[:, 3] - th[0, 3])) / n)
+    print(f"  E_tot drift max {drift:.3e} eV/atom, <KE>/atom {th[:, 2].mean() / n:.4f} eV", flush=True)
+    assert drift < 1e-4
+    for step in range(0, steps + 1, every):
+        L, t, x, F = _read_dump(tmp_path / f"md.dump{step:06d}")
+        _, F_r, _, _, _, _ = _oracle_case(jm, O, x, t, L)
+        err = _rel(F, F_r)
+        print(f"  step {step}: force rel err {err:.2e}", flush=True)
+        assert err <= 1e-10
+
+
+def test_pair_style_settings_check(jm, tmp_path):
+    """settings() rejects a non-positive cutoff with the reference's message (pair_jax_mtp.cpp:241-243),
+    before any GPU work."""
+    r = _run(jm, tmp_path / "d", cutoff=0.0)
+    assert r.returncode == 3 and "must be positive" in r.stderr
+
+
+@pytest.mark.gpu
+def test_pair_style_capacity_error(jm, tmp_path):
+    """MAX_ATOMS below the atom count is an error here (undefined behaviour in the reference)."""
+    r = _run(jm, tmp_path / "c", max_atoms=1000)
+    assert r.returncode == 3 and "exceed MAX_ATOMS" in r.stderr

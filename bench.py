@@ -153,6 +153,98 @@ def training_leg(seed, lbfgs_steps=5, reps=10):
             "loss_after_lbfgs": losses[-1], "lbfgs_steps": lbfgs_steps, "host_batch_build_s": t_pack}
 
 
+def lammps_full_list(pos, types, cell, cutoff, skin):
+    """Locals + periodic ghost copies (owner tags, 1-based types) and a full neighbor list with a skin, as
+    LAMMPS hands them to the pair style (rows unsorted, self excluded)."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(cell, dtype=float)
+    n = len(pos)
+    rg = cutoff + skin
+    xs, owners = [pos], [np.arange(n)]
+    for sh in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij")).reshape(3, -1).T:
+        if not sh.any():
+            continue
+        p = pos + sh * L
+        keep = np.all((p >= -rg) & (p < L + rg), axis=1)
+        xs.append(p[keep]); owners.append(np.nonzero(keep)[0])
+    x = np.concatenate(xs)
+    own = np.concatenate(owners)
+    tree = cKDTree(x)
+    rows = tree.query_ball_point(pos, rg)
+    neigh = [np.array([j for j in r if j != i], dtype=np.int32) for i, r in enumerate(rows)]
+    return (np.ascontiguousarray(x), (np.asarray(types)[own] + 1).astype(np.int32), (own + 1).astype(np.int32),
+            neigh)
+
+
+def pair_style_leg(jm, pot, w, reps=10):
+    """The pair style's per-step cost at this workload: host packing + rows through mtp_compute (the
+    reference's flow), and mtp_compute_full_list (packing and staging on the GPU), both from the same
+    LAMMPS-like arrays; wall-clock per call, host work and PCIe transfers included."""
+    import ctypes
+
+    t0 = time.perf_counter()
+    x, typ, tag, neigh = lammps_full_list(w.pos, w.types, w.cell, w.cutoff, 1.5)
+    t_list = time.perf_counter() - t0
+    n, K = w.natoms, pot.max_nbr
+    L = jm.capi.lib()
+    vp = ctypes.c_void_p
+    numneigh = np.array([len(r) for r in neigh], np.int32)
+    first = (vp * n)(*[r.ctypes.data for r in neigh])
+    ilist = np.arange(n, dtype=np.int32)
+    prd = np.asarray(w.cell, dtype=np.float64)
+    per = np.ones(3, np.int32)
+    pos = np.zeros((n, 3 * K)); nbr = np.zeros((n, K), np.int32); nt = np.zeros((n, K), np.int32)
+    cnt = np.zeros(n, np.int32); at = np.zeros(n, np.int32)
+    prow = (vp * n)(*[pos[i].ctypes.data for i in range(n)])
+    nrow = (vp * n)(*[nbr[i].ctypes.data for i in range(n)])
+    trow = (vp * n)(*[nt[i].ctypes.data for i in range(n)])
+    E, F = ctypes.c_double(), np.zeros((n, 3))
+    s6, v6 = np.zeros(6), np.zeros(6)
+    tr = ctypes.c_longlong()
+    vol = float(np.prod(prd))
+
+    def rows_step():
+        tot = L.mtp_pack_full_list(n, x.ctypes.data_as(vp), typ.ctypes.data_as(vp), tag.ctypes.data_as(vp), n,
+                                   ilist.ctypes.data_as(vp), numneigh.ctypes.data_as(vp), ctypes.cast(first, vp),
+                                   w.cutoff, prd.ctypes.data_as(vp), per.ctypes.data_as(vp), n, K,
+                                   pos.ctypes.data_as(vp), nbr.ctypes.data_as(vp), nt.ctypes.data_as(vp),
+                                   cnt.ctypes.data_as(vp), at.ctypes.data_as(vp), ctypes.byref(tr))
+        jm.capi.check(L.mtp_compute(pot.ctx, n, int(tot), ctypes.cast(prow, vp), at.ctypes.data_as(vp),
+                                    ctypes.cast(nrow, vp), cnt.ctypes.data_as(vp), ctypes.cast(trow, vp), vol,
+                                    ctypes.byref(E), F.ctypes.data_as(vp), s6.ctypes.data_as(vp),
+                                    v6.ctypes.data_as(vp)), pot.ctx)
+        return E.value, F.copy()
+
+    def full_list_step():
+        jm.capi.check(L.mtp_compute_full_list(pot.ctx, n, len(x), x.ctypes.data_as(vp), typ.ctypes.data_as(vp),
+                                              tag.ctypes.data_as(vp), n, ilist.ctypes.data_as(vp),
+                                              numneigh.ctypes.data_as(vp), ctypes.cast(first, vp), new_list[0],
+                                              prd.ctypes.data_as(vp), per.ctypes.data_as(vp), vol, ctypes.byref(E),
+                                              F.ctypes.data_as(vp), s6.ctypes.data_as(vp), v6.ctypes.data_as(vp),
+                                              ctypes.byref(tr)), pot.ctx)
+        return E.value, F.copy()
+
+    def timed(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return 1e3 * (time.perf_counter() - t) / reps
+
+    new_list = [1]
+    ms_rows = timed(rows_step)
+    ms_full_rebuild = timed(full_list_step)   # every call a neighbor-list rebuild step
+    new_list[0] = 0
+    ms_full = timed(full_list_step)           # steps between rebuilds: positions only
+    Er, Fr = rows_step()
+    Ef, Ff = full_list_step()
+    return {"ms_per_step_host_rows": ms_rows, "ms_per_step_full_list": ms_full,
+            "ms_per_step_full_list_rebuild": ms_full_rebuild,
+            "neighbor_entries": int(numneigh.sum()), "ghosts": int(len(x) - n), "skin": 1.5,
+            "bitwise_equal": bool(Er == Ef and np.array_equal(Fr, Ff)), "list_build_s": t_list}
+
+
 def cpu_baseline(w, nb, gpu_F, threads):
     """Reference-algorithm (full-tensor tree) CPU timing on a bounded sample,
     plus the compact CPU oracle on the whole workload (and the GPU force
@@ -206,6 +298,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-head", action="store_true", help="skip the distilled-head leg")
     ap.add_argument("--no-train", action="store_true", help="skip the packed training-step leg")
+    ap.add_argument("--no-pair-style", action="store_true", help="skip the pair-style (host arrays) leg")
     ap.add_argument("--head-steps", type=int, default=3000)
     ap.add_argument("--head-save", default=None, help="directory to keep the trained head and its teacher")
     ap.add_argument("--head-infer", default=None,
@@ -354,6 +447,11 @@ def main():
                         "atom_kernel_ms": results["radial_fp32"]["atom_kernel_ms"]},
         "cpu_baseline": None,
     }
+    if not args.no_pair_style:
+        pot.set_precision(False)
+        line["pair_style_step"] = pair_style_leg(jm, pot, w)
+        line["pair_style_step"]["ms_per_step_device"] = r["ms_per_step"]
+        log(f"[bench] pair style step: {line['pair_style_step']}")
     if not args.no_cpu_baseline:
         # every host CPU this process may run on (the OpenMP thread count when it is set)
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))

@@ -86,6 +86,30 @@ int mtp_compute(mtp_ctx *ctx, int natoms, int nneigh_total, const double *const 
                 const int *const *nbr_rows, const int *nbr_counts, const int *const *nbr_type_rows, double volume,
                 double *energy, double *forces, double *stress6, double *virial6);
 
+/* The pair style's whole force call with the packing and staging on the
+ * GPU: the LAMMPS arrays as they are (x[nall][3] contiguous, 1-based
+ * type[nall], tag[nall], the full neighbor list ilist[inum],
+ * numneigh[nlocal], firstneigh[nlocal], indices may carry LAMMPS's special
+ * bits) and the orthogonal box (prd[3], periodic[3]).  The host flattens
+ * the list into pinned memory and uploads it with the atoms; a kernel then
+ * does the walk of PairJaxMTP::compute (pair_jax_mtp.cpp:96-173: minimum
+ * image, r^2 < MTP_CUTOFF^2 while fewer than MAX_ATOMS_PER_NEIGHBOR are
+ * kept, ghosts folded onto owners by tag, unmapped -> nlocal / type -1) and
+ * the staging of zero_overhead.cpp:61-117, bit for bit what
+ * mtp_pack_full_list + mtp_compute produce, and the evaluation follows.
+ * Replaces that pair (pair_jax_mtp.cpp:96-189) for the pair style.
+ * list_is_new = 0 promises that the list, ilist, type and tag are those of
+ * the previous call on ctx (LAMMPS: neighbor->ago > 0, no rebuild since):
+ * only x is uploaded then.  A change of nlocal, nall, inum or the list
+ * length uploads everything regardless.
+ * Outputs as mtp_compute; *n_truncated (may be NULL) receives the in-cutoff
+ * neighbors dropped by the row limit.  mtp_last_atom_values works after it.
+ * Synchronous. */
+int mtp_compute_full_list(mtp_ctx *ctx, int nlocal, int nall, const double *x, const int *type, const int *tag,
+                          int inum, const int *ilist, const int *numneigh, const int *const *firstneigh,
+                          int list_is_new, const double *prd, const int *periodic, double volume, double *energy,
+                          double *forces, double *stress6, double *virial6, long long *n_truncated);
+
 /* Device-resident evaluation with the 8 dynamic arguments of
  * calc_energy_forces_stress_padded (jax_backend.py:374-392; dtypes as
  * compile_cuda.py:225-232 but FP64): d_itypes[nrows], d_js[nrows*nslots],

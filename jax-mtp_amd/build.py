@@ -30,8 +30,9 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
-SOURCES = ["mtp_kernels.hip", "mtp_train.hip", "mtp_head.hip", "mtp_capi.cpp", "mtp_plan.cpp", "mtp_format.cpp"]
-HEADERS = ["mtp_kernels.hpp", "mtp_train.hpp", "mtp_head.hpp", "mtp_device.hpp", "mtp_plan.hpp", "mtp_format.hpp", "nbr_pack.hpp"]
+SOURCES = ["mtp_kernels.hip", "mtp_train.hip", "mtp_head.hip", "mtp_pack.hip", "mtp_capi.cpp", "mtp_plan.cpp",
+           "mtp_format.cpp"]
+HEADERS = ["mtp_pack.hpp", "mtp_kernels.hpp", "mtp_train.hpp", "mtp_head.hpp", "mtp_device.hpp", "mtp_plan.hpp", "mtp_format.hpp", "nbr_pack.hpp"]
 
 
 def _compile(src: str) -> pathlib.Path:

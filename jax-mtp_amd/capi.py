@@ -35,6 +35,7 @@ SIGNATURES = {
     "mtp_set_params": (_I, [_P, _P, _P, _P, _D]),
     "mtp_set_params_device": (_I, [_P, _P, _P, _P, _P]),
     "mtp_compute": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]),
+    "mtp_compute_full_list": (_I, [_P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _D, _P, _P, _P, _P, _P]),
     "mtp_compute_padded_device": (_I, [_P, _I, _I, _P, _P, _P, _P, _I, _D, _I, _I, _P, _P, _P, _P]),
     "mtp_last_slot_gradients": (_P, [_P]),
     "mtp_last_atom_values": (_I, [_P, _I, _P, _P]),

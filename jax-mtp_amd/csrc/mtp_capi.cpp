@@ -5,18 +5,21 @@
 // pool the same way, zero_overhead.cpp:123-150).  No Python anywhere.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mtp_hip.h"
 #include "mtp_format.hpp"
 #include "mtp_head.hpp"
 #include "mtp_kernels.hpp"
+#include "mtp_pack.hpp"
 #include "mtp_plan.hpp"
 #include "mtp_train.hpp"
 #include "nbr_pack.hpp"
@@ -128,6 +131,17 @@ struct mtp_ctx {
     int *h_it = nullptr, *h_js = nullptr, *h_jt = nullptr;
     double* h_F = nullptr;
     double* h_out = nullptr;
+    // full-list packing on the device (mtp_compute_full_list): grown on demand
+    struct Grow {
+        void* d = nullptr;
+        void* h = nullptr;  // pinned staging of the upload
+        size_t bytes = 0;
+    };
+    Grow g_x, g_type, g_tag, g_ilist, g_off, g_nbr, g_map;
+    unsigned long long* d_pack_cnt = nullptr;
+    unsigned long long* h_pack_cnt = nullptr;
+    int list_nlocal = -1, list_nall = -1, list_inum = -1, list_max_tag = -1;  // the list now on the device
+    long long list_nnz = -1;
     int last_host_natoms = 0;             // rows of the last mtp_compute whose E_i, W_i are in d_Ei, d_Wi
     std::vector<double> h_wi;             // staging of mtp_last_atom_values
     // training buffers (allocated at the first training call)
@@ -169,6 +183,12 @@ struct mtp_ctx {
         if (h_jt) hipHostFree(h_jt);
         if (h_F) hipHostFree(h_F);
         if (h_out) hipHostFree(h_out);
+        for (Grow* g : {&g_x, &g_type, &g_tag, &g_ilist, &g_off, &g_nbr, &g_map}) {
+            if (g->d) hipFree(g->d);
+            if (g->h) hipHostFree(g->h);
+        }
+        if (d_pack_cnt) hipFree(d_pack_cnt);
+        if (h_pack_cnt) hipHostFree(h_pack_cnt);
         for (auto& pr : ev_pool) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
         if (stream) hipStreamDestroy(stream);
     }
@@ -855,6 +875,160 @@ int mtp_compute(mtp_ctx* ctx, int natoms, int nneigh_total, const double* const*
                                                      : "non-finite energy or virial");
     }
     ctx->last_host_natoms = natoms;
+    return MTP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Device buffer (and, with pinned, a pinned host buffer) of at least bytes, kept across calls.
+hipError_t grow(mtp_ctx::Grow& g, size_t bytes, bool pinned)
+{
+    if (bytes <= g.bytes && g.d) return hipSuccess;
+    if (g.d) hipFree(g.d);
+    if (g.h) hipHostFree(g.h);
+    g.d = g.h = nullptr;
+    g.bytes = 0;
+    const size_t b = bytes + bytes / 4 + 256;
+    hipError_t e = hipMalloc(&g.d, b);
+    if (e == hipSuccess && pinned) e = hipHostMalloc(&g.h, b, hipHostMallocDefault);
+    if (e == hipSuccess) g.bytes = b;
+    return e;
+}
+}  // namespace
+
+extern "C" {
+
+int mtp_compute_full_list(mtp_ctx* ctx, int nlocal, int nall, const double* x, const int* type, const int* tag,
+                          int inum, const int* ilist, const int* numneigh, const int* const* firstneigh,
+                          int list_is_new, const double* prd, const int* periodic, double volume, double* energy,
+                          double* forces, double* stress6, double* virial6, long long* n_truncated)
+{
+    if (!ctx) return MTP_ERR_ARG;
+    if (nlocal < 0 || nall < nlocal || inum < 0 || inum > nlocal)
+        return ctx->fail(MTP_ERR_ARG, "need 0 <= inum <= nlocal <= nall");
+    if (nlocal > ctx->max_atoms)
+        return ctx->fail(MTP_ERR_CAPACITY, "natoms " + std::to_string(nlocal) + " exceeds max_atoms " +
+                                               std::to_string(ctx->max_atoms));
+    if (!energy || (nlocal > 0 && !forces) || !prd || !periodic)
+        return ctx->fail(MTP_ERR_ARG, "null output or box pointer");
+    if (nall > 0 && (!x || !type || !tag)) return ctx->fail(MTP_ERR_ARG, "null atom array");
+    if (inum > 0 && (!ilist || !numneigh || !firstneigh)) return ctx->fail(MTP_ERR_ARG, "null neighbor list");
+    const int K = ctx->max_nbr;
+    hipStream_t s = ctx->stream;
+    // the list, types and tags already on the device are reused when the caller says they are unchanged
+    // (LAMMPS: no neighbor-list rebuild since the last call) and the sizes agree
+    // host: check ilist, size the flattened list, find the largest local tag
+    long long nnz = 0;
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = ilist[ii];
+        if (i < 0 || i >= nlocal) return ctx->fail(MTP_ERR_ARG, "ilist entry outside the local atoms");
+        if (numneigh[i] < 0 || (numneigh[i] > 0 && !firstneigh[i])) return ctx->fail(MTP_ERR_ARG, "bad neighbor row");
+        nnz += numneigh[i];
+    }
+    const bool upload_list = list_is_new || ctx->list_nlocal != nlocal || ctx->list_nall != nall ||
+                             ctx->list_inum != inum || ctx->list_nnz != nnz;
+    ctx->list_nlocal = ctx->list_nall = ctx->list_inum = -1;  // until this call's upload has been issued
+    int max_tag = 0;
+    if (upload_list)
+        for (int i = 0; i < nlocal; ++i) max_tag = tag[i] > max_tag ? tag[i] : max_tag;
+    else
+        max_tag = ctx->list_max_tag;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = grow(ctx->g_x, sizeof(double) * 3 * (size_t)(nall > 0 ? nall : 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_type, sizeof(int) * (size_t)(nall > 0 ? nall : 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_tag, sizeof(int) * (size_t)(nall > 0 ? nall : 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_ilist, sizeof(int) * (size_t)(inum + 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_off, sizeof(long long) * (size_t)(inum + 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_nbr, sizeof(int) * (size_t)(nnz > 0 ? nnz : 1), true);
+    if (e == hipSuccess) e = grow(ctx->g_map, sizeof(int) * ((size_t)max_tag + 1), false);
+    if (e == hipSuccess && !ctx->d_pack_cnt) {
+        e = hipMalloc((void**)&ctx->d_pack_cnt, 4 * sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_pack_cnt, 4 * sizeof(unsigned long long), hipHostMallocDefault);
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "full-list buffers");
+    auto up = [&](mtp_ctx::Grow& g, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(g.d, g.h, bytes, hipMemcpyHostToDevice, s);
+    };
+    if (nall > 0) std::memcpy(ctx->g_x.h, x, sizeof(double) * 3 * (size_t)nall);
+    up(ctx->g_x, sizeof(double) * 3 * (size_t)nall);
+    if (upload_list) {
+        // host: the flattened list in ilist order, staged in pinned memory (rows copied by several threads)
+        long long* off = static_cast<long long*>(ctx->g_off.h);
+        int* nbr = static_cast<int*>(ctx->g_nbr.h);
+        off[0] = 0;
+        for (int ii = 0; ii < inum; ++ii) off[ii + 1] = off[ii] + numneigh[ilist[ii]];
+        const int nth = (int)std::max(1L, std::min<long>(8, nnz / 200000));
+        std::vector<unsigned> jmax(nth, 0u);  // neighbor indices past nall would read outside x on the device
+        auto copy_rows = [&](int t) {
+            unsigned m = 0;
+            for (int ii = (int)((long long)inum * t / nth); ii < (int)((long long)inum * (t + 1) / nth); ++ii) {
+                const int i = ilist[ii];
+                const int n = numneigh[i];
+                int* dst = nbr + off[ii];
+                if (n > 0) std::memcpy(dst, firstneigh[i], sizeof(int) * (size_t)n);
+                for (int q = 0; q < n; ++q) m = std::max(m, (unsigned)(dst[q] & 0x1FFFFFFF));
+            }
+            jmax[t] = m;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t) pool.emplace_back(copy_rows, t);
+        copy_rows(0);
+        for (auto& th : pool) th.join();
+        if (nnz > 0 && *std::max_element(jmax.begin(), jmax.end()) >= (unsigned)nall)
+            return ctx->fail(MTP_ERR_ARG, "a neighbor index is not below nall");
+        if (nall > 0) {
+            std::memcpy(ctx->g_type.h, type, sizeof(int) * (size_t)nall);
+            std::memcpy(ctx->g_tag.h, tag, sizeof(int) * (size_t)nall);
+        }
+        if (inum > 0) std::memcpy(ctx->g_ilist.h, ilist, sizeof(int) * (size_t)inum);
+        up(ctx->g_type, sizeof(int) * (size_t)nall);
+        up(ctx->g_tag, sizeof(int) * (size_t)nall);
+        up(ctx->g_ilist, sizeof(int) * (size_t)inum);
+        up(ctx->g_off, sizeof(long long) * (size_t)(inum + 1));
+        up(ctx->g_nbr, sizeof(int) * (size_t)nnz);
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "full-list upload");
+    ctx->list_nlocal = nlocal; ctx->list_nall = nall; ctx->list_inum = inum; ctx->list_max_tag = max_tag;
+    ctx->list_nnz = nnz;
+    // device: packing + staging into the context's row buffers, then the evaluation
+    PackArgs pa{};
+    pa.nlocal = nlocal; pa.inum = inum; pa.max_nbr = K; pa.max_tag = max_tag; pa.cutoff = ctx->cutoff;
+    for (int q = 0; q < 3; ++q) { pa.prd[q] = prd[q]; pa.periodic[q] = periodic[q]; }
+    pa.x = static_cast<const double*>(ctx->g_x.d);
+    pa.type = static_cast<const int*>(ctx->g_type.d);
+    pa.tag = static_cast<const int*>(ctx->g_tag.d);
+    pa.ilist = static_cast<const int*>(ctx->g_ilist.d);
+    pa.off = static_cast<const long long*>(ctx->g_off.d);
+    pa.nbr = static_cast<const int*>(ctx->g_nbr.d);
+    pa.map = static_cast<int*>(ctx->g_map.d);
+    pa.itypes = ctx->d_it; pa.js = ctx->d_js; pa.jtypes = ctx->d_jt; pa.rij = ctx->d_rij;
+    pa.counters = ctx->d_pack_cnt;
+    if (nlocal > 0) {
+        e = launch_pack(pa, s);
+        if (e != hipSuccess) return ctx->hip_fail(e, "packing launch");
+    }
+    int rc = mtp_compute_padded_device(ctx, nlocal, K, ctx->d_it, ctx->d_js, ctx->d_rij, ctx->d_jt, 3, volume, nlocal,
+                                       nlocal, ctx->d_out, ctx->d_F, nullptr, s);
+    if (rc) return rc;
+    const size_t A = (size_t)nlocal;
+    if (nlocal > 0) e = hipMemcpyAsync(ctx->h_F, ctx->d_F, A * 3 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->h_out, ctx->d_out, 14 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && nlocal > 0)
+        e = hipMemcpyAsync(ctx->h_pack_cnt, ctx->d_pack_cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "evaluation");
+    *energy = ctx->h_out[0];
+    if (nlocal > 0) std::memcpy(forces, ctx->h_F, A * 3 * sizeof(double));
+    if (stress6) std::memcpy(stress6, ctx->h_out + 1, 6 * sizeof(double));
+    if (virial6) std::memcpy(virial6, ctx->h_out + 7, 6 * sizeof(double));
+    if (n_truncated) *n_truncated = nlocal > 0 ? (long long)ctx->h_pack_cnt[1] : 0;
+    if (ctx->h_out[13] != 0.0) {
+        const int st = (int)ctx->h_out[13];
+        return ctx->fail(MTP_ERR_NONFINITE, (st & 2) ? "non-finite neighbor vector r_ij in the input rows"
+                                                     : "non-finite energy or virial");
+    }
+    ctx->last_host_natoms = nlocal;
     return MTP_OK;
 }
 

@@ -1,14 +1,19 @@
 /* ----------------------------------------------------------------------
    LAMMPS pair style jax/mtp on the MI355X HIP evaluator (libmtp_hip.so).
 
-   Host flow per force call, as in the reference PairJaxMTP::compute
+   Flow per force call, as in the reference PairJaxMTP::compute
    (pair_jax_mtp.cpp:69-207):
-     1. pack the full neighbor list into padded per-atom rows
-        (mtp::pack_full_list, the reference's loop at :96-173),
-     2. mtp_compute(): stage + evaluate on the GPU (replaces
-        ZeroOverheadManager::execute_potential, zero_overhead.cpp:171-272),
+     1. pack the full neighbor list into padded per-atom rows (the
+        reference's loop at :96-173) and stage them (zero_overhead.cpp:61-117),
+     2. evaluate on the GPU (replaces ZeroOverheadManager::execute_potential,
+        zero_overhead.cpp:171-272),
      3. add forces for ii < inum, energy if eflag_global, virial if
         vflag_global.
+   Steps 1-2 are one call, mtp_compute_full_list(): positions, types, tags
+   and the full list go to the GPU as they are and a kernel packs and stages
+   the rows there.  JAXMTP_HOST_PACK=1 (and triclinic boxes) keep the
+   reference's split: mtp::pack_full_list on the host, then mtp_compute().
+   Both give the same rows bit for bit.
 
    Differences, each deliberate:
      * the evaluation is FP64 (the reference cast r to float);
@@ -55,6 +60,8 @@ PairJaxMTP::PairJaxMTP(LAMMPS *lmp) : Pair(lmp)
   total_truncated = 0;
   const char *compat = std::getenv("JAXMTP_VIRIAL_COMPAT");
   virial_compat = (compat && std::strcmp(compat, "1") == 0) ? 1 : 0;
+  const char *hp = std::getenv("JAXMTP_HOST_PACK");
+  host_pack = (hp && std::strcmp(hp, "1") == 0) ? 1 : 0;
   if (comm->me == 0) utils::logmesg(lmp, "JAX-MTP (MI355X HIP evaluator) initialized\n");
 }
 
@@ -105,19 +112,6 @@ void PairJaxMTP::compute(int eflag, int vflag)
   if (nlocal > max_atoms)
     error->all(FLERR, "jax/mtp: {} local atoms exceed MAX_ATOMS = {}", nlocal, max_atoms);
 
-  auto min_image = [this](double &dx, double &dy, double &dz) {
-    domain->minimum_image(FLERR, dx, dy, dz);
-  };
-  mtp::PackOut out{pos.data(), nbr.data(), ntype.data(), counts.data(), atom_types.data()};
-  long long truncated = 0;
-  const long long nneigh_total =
-      mtp::pack_full_list(nlocal, x, atom->type, atom->tag, inum, list->ilist, list->numneigh, list->firstneigh,
-                          cutoff, max_atoms, max_neighbors, min_image, out, tag_to_dense, &truncated);
-  total_truncated += truncated;
-  if (truncated && debug_level > 0)
-    error->warning(FLERR, "jax/mtp: {} in-cutoff neighbors dropped (MAX_ATOMS_PER_NEIGHBOR = {})", truncated,
-                   max_neighbors);
-
   // LAMMPS's h = (xprd, yprd, zprd, yz, xz, xy) is upper triangular, so the
   // cell volume is xprd * yprd * zprd for triclinic boxes too.  (The
   // reference, pair_jax_mtp.cpp:179-181, took a 3x3 determinant of h[0..8],
@@ -125,9 +119,32 @@ void PairJaxMTP::compute(int eflag, int vflag)
   const double volume = domain->xprd * domain->yprd * domain->zprd;
 
   double energy = 0.0, stress6[6], virial6[6];
-  const int rc = mtp_compute(ctx, nlocal, (int) nneigh_total, pos_rows.data(), atom_types.data(), nbr_rows.data(),
-                             counts.data(), ntype_rows.data(), volume, &energy, forces.data(), stress6, virial6);
+  long long truncated = 0;
+  int rc;
+  if (host_pack || domain->triclinic) {
+    // the reference's flow: rows packed on the host, then staged and evaluated
+    auto min_image = [this](double &dx, double &dy, double &dz) {
+      domain->minimum_image(FLERR, dx, dy, dz);
+    };
+    mtp::PackOut out{pos.data(), nbr.data(), ntype.data(), counts.data(), atom_types.data()};
+    const long long nneigh_total =
+        mtp::pack_full_list(nlocal, x, atom->type, atom->tag, inum, list->ilist, list->numneigh, list->firstneigh,
+                            cutoff, max_atoms, max_neighbors, min_image, out, tag_to_dense, &truncated);
+    rc = mtp_compute(ctx, nlocal, (int) nneigh_total, pos_rows.data(), atom_types.data(), nbr_rows.data(),
+                     counts.data(), ntype_rows.data(), volume, &energy, forces.data(), stress6, virial6);
+  } else {
+    // the same rows, packed and staged on the GPU from the uploaded full list
+    const double prd[3] = {domain->xprd, domain->yprd, domain->zprd};
+    // the list, types and tags are re-sent only after a neighbor-list rebuild (neighbor->ago == 0)
+    rc = mtp_compute_full_list(ctx, nlocal, nlocal + atom->nghost, x[0], atom->type, atom->tag, inum, list->ilist,
+                               list->numneigh, list->firstneigh, neighbor->ago == 0 ? 1 : 0, prd,
+                               domain->periodicity, volume, &energy, forces.data(), stress6, virial6, &truncated);
+  }
   if (rc != MTP_OK) error->all(FLERR, "jax/mtp evaluation failed: {}", mtp_last_error(ctx));
+  total_truncated += truncated;
+  if (truncated && debug_level > 0)
+    error->warning(FLERR, "jax/mtp: {} in-cutoff neighbors dropped (MAX_ATOMS_PER_NEIGHBOR = {})", truncated,
+                   max_neighbors);
 
   for (int ii = 0; ii < inum; ii++) {
     const int i = list->ilist[ii];

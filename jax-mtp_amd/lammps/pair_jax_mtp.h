@@ -44,6 +44,7 @@ class PairJaxMTP : public Pair {
   int max_atoms, max_neighbors;
   int debug_level;
   int virial_compat;  // 1: add the reference's stress6 into virial[] (JAXMTP_VIRIAL_COMPAT=1)
+  int host_pack;      // 1: pack the rows on the host (JAXMTP_HOST_PACK=1); 0: on the GPU
   mtp_ctx *ctx;
   long long total_calls, total_truncated;
 

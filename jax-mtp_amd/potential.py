@@ -279,6 +279,35 @@ class MTPPotential:
             F.ctypes.data_as(vp), s6.ctypes.data_as(vp), v6.ctypes.data_as(vp)), self.ctx)
         return E.value, F, s6, v6
 
+    def compute_full_list(self, nlocal, x, types, tags, ilist, neighbors, prd, periodic=(1, 1, 1), volume=None,
+                          list_is_new=True):
+        """The pair style's force call with packing and staging on the GPU (mtp_compute_full_list).
+
+        x [nall, 3], types [nall] (1-based), tags [nall]; ilist [inum] and neighbors (a list of
+        index arrays, one per local atom) form the full neighbor list.  Returns (energy,
+        forces[nlocal, 3], stress6, virial6, n_truncated)."""
+        vp = ctypes.c_void_p
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        tags = np.ascontiguousarray(tags, dtype=np.int32)
+        ilist = np.ascontiguousarray(ilist, dtype=np.int32)
+        rows = [np.ascontiguousarray(r, dtype=np.int32) for r in neighbors]
+        numneigh = np.array([len(r) for r in rows], dtype=np.int32)
+        first = (vp * max(len(rows), 1))(*[r.ctypes.data if len(r) else None for r in rows])
+        prd = np.ascontiguousarray(prd, dtype=np.float64)
+        per = np.ascontiguousarray(periodic, dtype=np.int32)
+        vol = float(np.prod(prd)) if volume is None else float(volume)
+        E = ctypes.c_double()
+        F = np.zeros((nlocal, 3))
+        s6, v6 = np.zeros(6), np.zeros(6)
+        trunc = ctypes.c_longlong()
+        capi.check(capi.lib().mtp_compute_full_list(
+            self.ctx, int(nlocal), len(x), x.ctypes.data_as(vp), types.ctypes.data_as(vp), tags.ctypes.data_as(vp),
+            len(ilist), ilist.ctypes.data_as(vp), numneigh.ctypes.data_as(vp), ctypes.cast(first, vp),
+            int(bool(list_is_new)), prd.ctypes.data_as(vp), per.ctypes.data_as(vp), vol, ctypes.byref(E), F.ctypes.data_as(vp),
+            s6.ctypes.data_as(vp), v6.ctypes.data_as(vp), ctypes.byref(trunc)), self.ctx)
+        return E.value, F, s6, v6, trunc.value
+
     def last_atom_values(self, natoms):
         """Per-atom energies [natoms] and LAMMPS virials [natoms, 6] (xx yy zz xy xz yz) of the last
         compute_rows call; the virial rows sum to its virial6."""

@@ -131,6 +131,7 @@ enum { REQ_FULL = 1 << 0 };
 class Neighbor {
  public:
     NeighList* lists[1] = {nullptr};
+    int ago = 0;  // steps since the last neighbor-list build (0: built for this step)
     int requests = 0, request_flags = 0;
     void add_request(void*, int flags)
     {

@@ -7,7 +7,11 @@
 //
 // Per force call: periodic ghost copies carrying their owners' tags, a full
 // neighbor list over locals + ghosts with a skin, forces cleared, then
-// PairJaxMTP::compute(eflag, vflag).  Step 0 asks for the global and
+// PairJaxMTP::compute(eflag, vflag).  As with "neighbor 1.5 bin" and
+// "neigh_modify every 1 delay 5 check yes": the ghosts and the list are
+// rebuilt when an atom has moved more than half the skin since the last
+// build (at most every 5 steps); in between, ghost positions follow their
+// owners (forward communication) and neighbor->ago counts the steps.  Step 0 asks for the global and
 // per-atom energy and virial (eflag 3, vflag 5); later steps for the
 // global energy.  Velocity Verlet in metal units.
 //
@@ -93,6 +97,8 @@ struct Sys {
     std::vector<double> x;  // [nall][3], locals first
     std::vector<int> type;
     std::vector<int> tag;
+    std::vector<int> owner;     // [nghost] local atom of each ghost
+    std::vector<double> shift;  // [nghost][3] its periodic image shift
     std::vector<double> v, f, mass;
     std::vector<double*> xrow, frow;
     int nall() const { return (int)type.size(); }
@@ -112,6 +118,8 @@ void make_ghosts(Sys& s, double rg)
     s.x.resize(3 * (size_t)s.nlocal);
     s.type.resize(s.nlocal);
     s.tag.resize(s.nlocal);
+    s.owner.clear();
+    s.shift.clear();
     for (int i = 0; i < s.nlocal; ++i)
         for (int a = -1; a <= 1; ++a)
             for (int b = -1; b <= 1; ++b)
@@ -125,7 +133,18 @@ void make_ghosts(Sys& s, double rg)
                     s.x.insert(s.x.end(), p, p + 3);
                     s.type.push_back(s.type[i]);
                     s.tag.push_back(s.tag[i]);
+                    s.owner.push_back(i);
+                    const double sh[3] = {a * s.L[0], b * s.L[1], c * s.L[2]};
+                    s.shift.insert(s.shift.end(), sh, sh + 3);
                 }
+}
+
+// ghost positions from their owners (forward communication between rebuilds)
+void update_ghosts(Sys& s)
+{
+    for (size_t g = 0; g < s.owner.size(); ++g)
+        for (int q = 0; q < 3; ++q)
+            s.x[3 * (s.nlocal + g) + q] = s.x[3 * (size_t)s.owner[g] + q] + s.shift[3 * g + q];
 }
 
 void neighbor_list(const Sys& s, double rn, std::vector<std::vector<int>>& nl)
@@ -242,10 +261,30 @@ int main(int argc, char** argv)
     std::vector<int> ilist(n), numneigh(n);
     std::vector<int*> firstneigh(n);
     for (int i = 0; i < n; ++i) ilist[i] = i;
-    auto setup = [&]() {
-        wrap(s);
-        make_ghosts(s, cutoff + skin);
-        neighbor_list(s, cutoff + skin, nl);
+    std::vector<double> x_built;
+    int rebuilds = 0;
+    auto setup = [&](bool first) {
+        bool rebuild = first;
+        if (!first && neighbor.ago + 1 >= 5) {  // delay 5, check yes
+            double dmax2 = 0.0;
+            for (int i = 0; i < n; ++i) {
+                double d2 = 0.0;
+                for (int q = 0; q < 3; ++q) { const double t = s.x[3 * i + q] - x_built[3 * i + q]; d2 += t * t; }
+                dmax2 = std::max(dmax2, d2);
+            }
+            rebuild = dmax2 > 0.25 * skin * skin;
+        }
+        if (rebuild) {
+            wrap(s);
+            make_ghosts(s, cutoff + skin);
+            neighbor_list(s, cutoff + skin, nl);
+            x_built.assign(s.x.begin(), s.x.begin() + 3 * (size_t)n);
+            neighbor.ago = 0;
+            ++rebuilds;
+        } else {
+            update_ghosts(s);
+            ++neighbor.ago;
+        }
         const int nall = s.nall();
         s.f.assign(3 * (size_t)nall, 0.0);  // force_clear
         s.xrow.resize(nall);
@@ -297,7 +336,7 @@ int main(int argc, char** argv)
             std::fwrite(s.f.data(), sizeof(double), 3 * (size_t)n, fp);
             std::fclose(fp);
         };
-        setup();
+        setup(true);
         pair.compute(3, 5);  // global + per-atom energy and virial
         {
             FILE* fp = std::fopen((out + ".step0").c_str(), "wb");
@@ -317,7 +356,7 @@ int main(int argc, char** argv)
                     s.v[3 * i + q] += 0.5 * dt * s.f[3 * i + q] / s.mass[i] * kFtm2v;
                     s.x[3 * i + q] += dt * s.v[3 * i + q];
                 }
-            setup();
+            setup(false);
             update.ntimestep = step;
             pair.compute(1, 0);
             for (int i = 0; i < n; ++i)
@@ -329,6 +368,7 @@ int main(int argc, char** argv)
             if (step % 100 == 0) { std::printf("step %d etot %.10g\n", step, pe + ke); std::fflush(stdout); }
         }
         std::fclose(th);
+        std::printf("neighbor list builds: %d\n", rebuilds);
         std::printf("%s", lmp.log.c_str());
         for (auto& w : error.warnings) std::printf("WARNING: %s\n", w.c_str());
     } catch (const std::exception& e) {

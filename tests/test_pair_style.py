@@ -111,6 +111,14 @@ def test_pair_style_reference_example(jm, O, tmp_path):
     assert _rel(vatom, _lammps_virial(W_r)) <= 1e-10
     np.testing.assert_allclose(eatom.sum(), E, rtol=1e-12)
     np.testing.assert_allclose(vatom.sum(axis=0), virial, rtol=1e-10, atol=1e-10 * np.abs(virial).max())
+    # the reference's split (rows packed on the host, JAXMTP_HOST_PACK=1) gives the same bits
+    r3 = _run(jm, tmp_path / "h", dump_every=1, env={"JAXMTP_HOST_PACK": "1"})
+    assert r3.returncode == 0, r3.stderr
+    _, _, _, Fh = _read_dump(tmp_path / "h.dump000000")
+    Eh, virial_h, eatom_h, vatom_h = _read_step0(tmp_path / "h.step0")
+    assert Eh == E
+    for a, b in ((Fh, F), (virial_h, virial), (eatom_h, eatom), (vatom_h, vatom)):
+        np.testing.assert_array_equal(a, b)
     # the reference's convention: its stress6 added raw into virial[] (pair_jax_mtp.cpp:203-205)
     r2 = _run(jm, tmp_path / "b", env={"JAXMTP_VIRIAL_COMPAT": "1"})
     assert r2.returncode == 0, r2.stderr
@@ -139,6 +147,21 @@ def test_pair_style_nve_on_example(jm, O, tmp_path):
         err = _rel(F, F_r)
         print(f"  step {step}: force rel err {err:.2e}", flush=True)
         assert err <= 1e-10
+
+
+@pytest.mark.gpu
+def test_pair_style_gpu_packing_trajectory_bitexact(jm, tmp_path):
+    """40 NVE steps with the rows packed on the GPU (the list re-sent only after rebuilds) and with the
+    reference's host packing: the same trajectory bit for bit."""
+    print("[case] pair style: GPU vs host packing over 40 steps", flush=True)
+    a = _run(jm, tmp_path / "g", steps=40, T=300.0)
+    b = _run(jm, tmp_path / "h", steps=40, T=300.0, env={"JAXMTP_HOST_PACK": "1"})
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    builds = [int(line.split()[-1]) for line in a.stdout.splitlines() if line.startswith("neighbor list builds")]
+    print(f"  neighbor list builds: {builds}", flush=True)
+    assert builds and builds[0] < 40  # steps between rebuilds reuse the uploaded list
+    ta, tb = np.loadtxt(tmp_path / "g.thermo"), np.loadtxt(tmp_path / "h.thermo")
+    np.testing.assert_array_equal(ta, tb)
 
 
 def test_pair_style_settings_check(jm, tmp_path):

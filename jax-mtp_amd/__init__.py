@@ -10,7 +10,7 @@ head (head).
 The directory name contains a hyphen, so import it with
 ``importlib.import_module("jax-mtp_amd")``.
 """
-from . import capi, configs, distill, head, mtp_io, neighbors, training  # noqa: F401
+from . import capi, cfg_io, configs, distill, head, mtp_io, neighbors, training  # noqa: F401
 from .mtp_io import MTPData, format_mtp, parse_mtp_text, read_mtp, untrained, write_mtp  # noqa: F401
 from .potential import MTPPotential  # noqa: F401
 

@@ -404,3 +404,75 @@ class Trainer:
         with torch.no_grad():
             self.theta.copy_(new)
         return f
+
+
+# ---------------------------------------------------------------------------
+# Training from MLIP .cfg data (motep_jax_train_function.py:17-229)
+def batch_from_cfg(configs, cutoff: float, max_nbr: int):
+    """A packed batch and its labels from .cfg configurations (orthogonal periodic cells): energies [C],
+    forces [N, 3] and stresses [C, 6] = -PlusStress / V in the reference's Voigt order."""
+    cells = [(c.positions, c.types, c.orthogonal_box()) for c in configs]
+    batch = pack_configs(cells, cutoff, max_nbr)
+    E = np.array([c.energy for c in configs], dtype=np.float64)
+    F = np.concatenate([c.forces for c in configs]).astype(np.float64)
+    S = np.stack([c.stress for c in configs]).astype(np.float64)
+    return batch, E, F, S
+
+
+def train_cfg(cfg_path, level: int = 12, steps: int = 50, species_count: int = 1, min_dist: float = 0.5,
+              max_dist: float = 5.0, scaling: float = 1.0, radial_basis_size: int | None = None, seed: int = 10,
+              max_nbr: int = 64, split: float = 0.75, log=None):
+    """The reference's training recipe on a .cfg file (motep_jax_train_function.py:17-50, 154-213): the untrained
+    level's alpha tables, species_count / min_dist / max_dist / scaling set, parameters drawn uniformly from a
+    seeded generator (data.py:41-58), the first `split` of the configurations as the training set, linear least
+    squares for the moment coefficients, then L-BFGS on the weighted E/F/S loss with the hand-written gradient.
+
+    Returns (trained MTPData, dict of losses and per-set RMSEs).  The reference takes its validation set as
+    images[1 - n_train:] (motep_jax_train_function.py:25), which overlaps the training set; here the held-out
+    configurations are images[n_train:].
+    """
+    import torch
+
+    from .cfg_io import read_cfg
+    from .mtp_io import untrained
+    from .potential import MTPPotential
+
+    configs = read_cfg(cfg_path)
+    n_train = int(split * len(configs))
+    p = untrained(level)
+    p.species_count = species_count
+    p.min_dist, p.max_dist, p.scaling = min_dist, max_dist, scaling
+    if radial_basis_size:
+        p.radial_basis_size = radial_basis_size
+    p.initialize(np.random.default_rng(seed))
+    batch, E, F, S = batch_from_cfg(configs[:n_train], max_dist, max_nbr)
+    pot = MTPPotential(p, max_atoms=max(batch.natoms, 1), max_nbr=max_nbr)
+    tr = Trainer(pot, batch, E, F, S, lls=True, graph=False, autotune=False)
+    history = [float(tr.loss().detach())]
+    for _ in range(steps):
+        history.append(tr.step())
+    theta = tr.theta.detach().cpu().numpy()
+    S_n, sp = len(p.moment_coeffs), p.species_count
+    pot.set_params(moment_coeffs=theta[:S_n], species_coeffs=theta[S_n:S_n + sp],
+                   radial_coeffs=theta[S_n + sp:].reshape(np.asarray(pot.params.radial_coeffs).shape))
+    trained = pot.params.copy()
+    pot.close()
+
+    def errors(cfgs):
+        if not cfgs:
+            return {}
+        b, Eg, Fg, Sg = batch_from_cfg(cfgs, max_dist, max_nbr)
+        q = MTPPotential(trained, max_atoms=b.natoms, max_nbr=max_nbr)
+        Ep, Fp, Sp = (t.cpu().numpy() for t in batch_forward(q, b.to_device()))
+        torch.cuda.synchronize()
+        q.close()
+        n = np.diff(b.conf_off)
+        return {"configs": len(cfgs), "energy_rmse_meV_per_atom": float(np.sqrt(np.mean(((Ep - Eg) / n) ** 2))) * 1e3,
+                "force_rmse_meV_per_A": float(np.sqrt(np.mean((Fp - Fg) ** 2))) * 1e3,
+                "stress_rmse": float(np.sqrt(np.mean((Sp - Sg) ** 2)))}
+
+    res = {"level": level, "loss_initial_after_lls": history[0], "loss_final": history[-1], "lbfgs_steps": steps,
+           "train": errors(configs[:n_train]), "held_out": errors(configs[n_train:])}
+    if log:
+        log(f"[train_cfg] {res}")
+    return trained, res

@@ -367,3 +367,39 @@ def test_capacity_errors(jm):
         pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3, w.volume, 8, 8)
     assert e.value.code == jm.capi.MTP_ERR_CAPACITY
     pot.close()
+
+
+def test_level26_and_level28(jm, O):
+    """Levels 26 and 28 (alpha tables of the reference's untrained_mtps/26.mtp, 28.mtp; no JSON trees exist
+    for them, so the oracle is the compact form).  Level 26's contraction tables overflow the streams'
+    16-bit operand offsets, so it runs on the per-target kernel (its moments and adjoints, 9,396 of each,
+    just fit one workgroup's LDS at K = 80); level 28 needs 290 KB of LDS per atom and is refused with
+    MTP_ERR_UNSUPPORTED at open."""
+    import ctypes
+
+    rng = np.random.default_rng(26)
+    p = jm.mtp_io.untrained(26)
+    p.species_count, p.radial_basis_size, p.min_dist, p.max_dist = 2, 8, 1.5, 5.0
+    p.species_coeffs = rng.normal(0, 0.1, 2)
+    p.moment_coeffs = rng.normal(0, 0.1, len(p.alpha_moment_mapping))
+    p.radial_coeffs = rng.normal(0, 0.1, (2, 2, p.radial_funcs_count, 8))
+    w = jm.configs.small_cluster(10, 2, 12, seed=26, cell=7.0, cutoff=5.0, max_nbr=80)
+    w.params = p
+    nb = w.neighbors()
+    print(f"[case] level 26, {w.natoms} atoms", flush=True)
+    pot = jm.MTPPotential(p, max_atoms=w.natoms, max_nbr=80)
+    assert pot.variant == 0 and pot.info()["level"] == 26
+    E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           w.volume, w.natoms, w.natoms, return_all=True)
+    pot.close()
+    gpu = dict(E=float(E.item()), F=F.cpu().numpy(), s6=s6.cpu().numpy(), eatom=extra["eatom"].cpu().numpy(),
+               g=extra["g"].cpu().numpy(), status=float(extra["status"].item()))
+    _check(gpu, _run_oracle(O, w, nb))
+    q = jm.mtp_io.untrained(28)
+    q.species_count, q.radial_basis_size, q.min_dist, q.max_dist = 1, 8, 1.5, 5.0
+    q.initialize(rng)
+    L = jm.capi.lib()
+    ctx = ctypes.c_void_p()
+    rc = L.mtp_open_text(jm.format_mtp(q).encode(), 16, 80, 5.0, 0, ctypes.byref(ctx))
+    assert rc == jm.capi.MTP_ERR_UNSUPPORTED, rc
+    assert b"LDS" in L.mtp_global_error()

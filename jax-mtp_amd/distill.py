@@ -244,9 +244,9 @@ def evaluate(model, teacher: MTPData, infer_ncell=(25, 25, 40), seed=2, reps=5):
     head = DistilledHead(model, exact, max_atoms=A, max_nbr=MAX_NBR)
     head.desc.autotune(it, js, rij, jt)
     lin = head.desc  # the fitted descriptor MTP itself
-    Ex, Fx, _ = exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
+    Ex, Fx, _, ex_x = exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
     Eh, Fh, _, extra = head.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
-    El, Fl, _ = lin.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A)
+    El, Fl, _, ex_l = lin.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, return_all=True)
     rm = lambda a, b: float(torch.sqrt(((a - b) ** 2).mean()))
     ms_exact = _time(lambda: exact.calc_energy_forces_stress_padded(it, js, rij, jt, 3, w.volume, A, A, sync=False),
                      reps=reps)
@@ -260,8 +260,11 @@ def evaluate(model, teacher: MTPData, infer_ncell=(25, 25, 40), seed=2, reps=5):
         "force_rmse_head_meV_per_A": rm(Fh, Fx) * 1e3,
         "force_rmse_desc_meV_per_A": rm(Fl, Fx) * 1e3,
         "force_rms_exact_eV_per_A": float(torch.sqrt((Fx ** 2).mean())),
-        "energy_err_head_meV_per_atom": abs(float(Eh - Ex)) / A * 1e3,
-        "energy_err_desc_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
+        # site energies E_i against the exact ones, atom by atom (RMSE), and the error of the cell's total
+        "energy_rmse_head_meV_per_atom": rm(extra["eatom"], ex_x["eatom"]) * 1e3,
+        "energy_rmse_desc_meV_per_atom": rm(ex_l["eatom"], ex_x["eatom"]) * 1e3,
+        "total_energy_err_head_meV_per_atom": abs(float(Eh - Ex)) / A * 1e3,
+        "total_energy_err_desc_meV_per_atom": abs(float(El - Ex)) / A * 1e3,
         "variant_exact": exact.variant, "variant_desc": head.desc.variant,
     }
     head.close()

@@ -299,21 +299,32 @@ template <bool REV>
 __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, const char* bb, char* lb, double& acc)
 {
     constexpr int U = kStreamUnroll;
-    double a0[U], b0[U], a1[U], b1[U];
+    double a0[U], b0[U], a1[U], b1[U], a2[U], b2[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         a0[u] = *reinterpret_cast<const double*>(ab + (q[u].x & 0xffffu));
         b0[u] = *reinterpret_cast<const double*>(bb + (q[u].x >> 16));
-        a1[u] = *reinterpret_cast<const double*>(ab + (q[u].z & 0xffffu));
-        b1[u] = *reinterpret_cast<const double*>(bb + (q[u].z >> 16));
+        a1[u] = *reinterpret_cast<const double*>(ab + (q[u].y & 0xffffu));
+        b1[u] = *reinterpret_cast<const double*>(bb + (q[u].y >> 16));
+        a2[u] = *reinterpret_cast<const double*>(ab + (q[u].z & 0xffffu));
+        b2[u] = *reinterpret_cast<const double*>(bb + (q[u].z >> 16));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        acc = fma((double)__uint_as_float(q[u].y) * a0[u], b0[u], acc);
-        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
-            acc = fma((double)__uint_as_float(q[u].w) * a1[u], b1[u], acc);
-        } else {  // group end (wave-uniform): q.z is this lane's destination
-            *reinterpret_cast<double*>(lb + q[u].z) = acc;
+        // three entries with one multiplier; an item's last step has two,
+        // its third word is the destination.  The empty asm keeps the third
+        // entry's loads ahead of the branch (the compiler would otherwise
+        // sink them into it and wait there for every read in flight).
+        double t = a0[u] * b0[u];
+        t = fma(a1[u], b1[u], t);
+        asm volatile("" : : "v"(a2[u]), "v"(b2[u]));
+        const uint32_t w = q[u].w;
+        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
+            acc = fma((double)__uint_as_float(w), fma(a2[u], b2[u], t), acc);
+        } else {  // item end (wave-uniform)
+            acc = fma((double)__uint_as_float(w & kStreamMult), t, acc);
+            const uint32_t d = stream_dest(q[u].z, w);
+            if (d < kStreamNoDest) *reinterpret_cast<double*>(const_cast<char*>(bb) + d) = acc;
             acc = 0.0;
         }
     }
@@ -906,25 +917,37 @@ __global__ void __launch_bounds__(NT) atom_kernel_v2_list(const DevPlan P, AtomA
 __device__ __forceinline__ void stream_steps_pair(const uint4* q, char* lb, double2& acc)
 {
     constexpr int U = kStreamUnroll;
-    double2 a0[U], b0[U], a1[U], b1[U];
+    double2 a0[U], b0[U], a1[U], b1[U], a2[U], b2[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         a0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x & 0xffffu) << 4));
         b0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x >> 16) << 4));
-        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
-        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
+        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].y & 0xffffu) << 4));
+        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].y >> 16) << 4));
+        a2[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
+        b2[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const double w0 = (double)__uint_as_float(q[u].y);
-        acc.x = fma(w0 * a0[u].x, b0[u].x, acc.x);
-        acc.y = fma(w0 * a0[u].y, b0[u].y, acc.y);
-        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
-            const double w1 = (double)__uint_as_float(q[u].w);
-            acc.x = fma(w1 * a1[u].x, b1[u].x, acc.x);
-            acc.y = fma(w1 * a1[u].y, b1[u].y, acc.y);
-        } else {  // group end (wave-uniform): q.z is this lane's destination
-            *reinterpret_cast<double2*>(lb + q[u].z) = acc;
+        // three entries with one multiplier; an item's last step has two,
+        // its third word is the destination slot (the empty asm keeps the
+        // third entry's loads ahead of the branch)
+        double2 t;
+        t.x = a0[u].x * b0[u].x;
+        t.y = a0[u].y * b0[u].y;
+        t.x = fma(a1[u].x, b1[u].x, t.x);
+        t.y = fma(a1[u].y, b1[u].y, t.y);
+        asm volatile("" : : "v"(a2[u].x), "v"(a2[u].y), "v"(b2[u].x), "v"(b2[u].y));
+        const uint32_t w = q[u].w;
+        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
+            const double m = (double)__uint_as_float(w);
+            acc.x = fma(m, fma(a2[u].x, b2[u].x, t.x), acc.x);
+            acc.y = fma(m, fma(a2[u].y, b2[u].y, t.y), acc.y);
+        } else {  // item end (wave-uniform)
+            const double m = (double)__uint_as_float(w & kStreamMult);
+            acc.x = fma(m, t.x, acc.x);
+            acc.y = fma(m, t.y, acc.y);
+            *reinterpret_cast<double2*>(lb + ((q[u].z >> 16) << 4)) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }

@@ -215,70 +215,139 @@ bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
 // with the entries that read it) are cut into items: a row of at most C
 // entries is one item that stores its sum in the row's own moment; a longer
 // row becomes k = ceil(n / C) near-equal chunks that store partial sums,
-// added afterwards in chunk order.  Items are sorted by length (longest
-// first, ties in row order) and taken 64 at a time as groups: one item per
-// lane, every lane padded with zero entries to the group's longest item, so
-// a group ends at the same step on all lanes of a wavefront and the kernel's
-// loop has no per-lane control flow.  The group's last step carries each
-// lane's destination in place of a second entry.  Groups go to the least-loaded wave
-// (longest group first).  C is chosen per layer from a small set by the
-// modelled time of the slowest wave plus the partial-sum combination.
+// added afterwards in chunk order.  A row's entries are ordered by
+// multiplier first, so a chunk holds few distinct multipliers.
+//
+// A step carries up to three entries of one lane that share a multiplier
+// (the kernel forms w (a0 b0 + a1 b1 + a2 b2)), so an item takes, per
+// multiplier of its entries, ceil(count / 3) steps; its last step holds at
+// most two entries and, in place of the third, the item's destination, with
+// the sign bit of the multiplier word set (multipliers are positive).  Items
+// are sorted by steps (most first, ties in row order) and taken 64 at a time
+// as groups: one item per lane, shorter items padded with zero steps before
+// their last step, so a group ends at the same step on all lanes of a
+// wavefront and the kernel's loop has no per-lane control flow.  A wave's
+// steps are padded to whole prefetch batches of kStreamUnroll steps.  Groups
+// go to the least-loaded of the first nwu waves (most steps first).  C and
+// nwu are chosen per layer by a model of the time: the mean of the slowest
+// wave's steps and of the steps of all waves (the LDS is shared by the
+// workgroups of a CU), plus the partial-sum combination.
 // ---------------------------------------------------------------------------
 namespace {
 
-struct Item { int len, e0, row, dest; };
+struct Item { int len, e0, row, dest, steps; };  // entries ord[e0, e0 + len)
 
-constexpr double kFlushSteps = 1.5;    // modelled cost of a group end, in steps
 constexpr double kCombSteps = 0.5;     // per partial sum added in the combination
 
 struct LayerChoice {
     std::vector<Item> items;           // sorted
     std::vector<std::vector<int>> wave_groups;
-    std::vector<int> gsteps;
+    std::vector<int> gsteps;           // steps of each group
     int npart = 0;
     double cost = 0;
 };
 
-void choose_layer(const std::vector<int>& beg, int l0, int l1, int C, int nw, int nt, LayerChoice& lc,
-                  const std::vector<int>& dest_of_row, const StreamLayout& lay)
+int batch_steps(int steps)
+{
+    return (steps + kStreamUnroll - 1) / kStreamUnroll * kStreamUnroll;
+}
+
+// Runs of equal multipliers of entries ord[e0, e0 + n) (ordered by
+// multiplier), in the order the item's steps take them: the first run whose
+// length is not a multiple of 3 goes last, so its remainder fills the last
+// step.  Returns the steps.
+int item_runs(const std::vector<int>& ord, const std::vector<double>& mult, int e0, int n,
+              std::vector<std::pair<int, int>>* runs_out)
+{
+    std::vector<std::pair<int, int>> runs;  // (first position, length)
+    for (int j = 0; j < n;) {
+        int k = j + 1;
+        while (k < n && mult[ord[e0 + k]] == mult[ord[e0 + j]]) ++k;
+        runs.push_back({e0 + j, k - j});
+        j = k;
+    }
+    int steps = 0;
+    for (const auto& r : runs) steps += (r.second + 2) / 3;
+    const auto rem = std::find_if(runs.begin(), runs.end(), [](const std::pair<int, int>& r) { return r.second % 3 != 0; });
+    if (rem != runs.end()) std::rotate(rem, rem + 1, runs.end());
+    else ++steps;  // every run fills whole steps: the last step holds two entries of the last run
+    if (runs_out) *runs_out = std::move(runs);
+    return steps;
+}
+
+void cut_items(const std::vector<int>& beg, const std::vector<int>& ord, const std::vector<double>& mult, int l0,
+               int l1, int C, LayerChoice& lc, const std::vector<int>& dest_of_row, const StreamLayout& lay,
+               std::vector<int>& comb_k)
 {
     lc = LayerChoice();
-    std::vector<int> comb_k;
+    comb_k.clear();
     for (int pos = l0; pos < l1; ++pos) {
         const int n = beg[pos + 1] - beg[pos];
         if (n == 0) continue;
         if (n <= C) {
-            lc.items.push_back(Item{n, beg[pos], pos, dest_of_row[pos - l0]});
+            lc.items.push_back(Item{n, beg[pos], pos, dest_of_row[pos - l0], item_runs(ord, mult, beg[pos], n, nullptr)});
             continue;
         }
         const int k = (n + C - 1) / C, c = (n + k - 1) / k;
         for (int q = 0; q < k; ++q) {
             const int e0 = beg[pos] + q * c, e1 = std::min(beg[pos + 1], e0 + c);
-            lc.items.push_back(Item{e1 - e0, e0, pos, (lay.part_off + lc.npart) * lay.unit});
+            lc.items.push_back(Item{e1 - e0, e0, pos, (lay.part_off + lc.npart) * lay.unit,
+                                    item_runs(ord, mult, e0, e1 - e0, nullptr)});
             ++lc.npart;
         }
         comb_k.push_back(k);
     }
-    std::stable_sort(lc.items.begin(), lc.items.end(), [](const Item& a, const Item& b) { return a.len > b.len; });
+    std::stable_sort(lc.items.begin(), lc.items.end(), [](const Item& a, const Item& b) { return a.steps > b.steps; });
     const int ng = ((int)lc.items.size() + 63) / 64;
     lc.gsteps.resize(ng);
-    for (int g = 0; g < ng; ++g) lc.gsteps[g] = lc.items[(size_t)g * 64].len / 2 + 1;  // longest first, + dest
+    for (int g = 0; g < ng; ++g) lc.gsteps[g] = lc.items[(size_t)g * 64].steps;
+}
+
+// Groups to the first nwu waves, most steps first to the least-loaded;
+// returns the modelled cost.
+double assign_waves(LayerChoice& lc, int nw, int nwu, int nt, const std::vector<int>& comb_k)
+{
+    const int ng = (int)lc.gsteps.size();
     std::vector<int> order(ng);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lc.gsteps[a] > lc.gsteps[b]; });
-    std::vector<double> load(nw, 0.0);
+    std::vector<int> load(nw, 0);
     lc.wave_groups.assign(nw, {});
     for (int g : order) {
-        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[w] += lc.gsteps[g] + kFlushSteps;
+        const int w = (int)(std::min_element(load.begin(), load.begin() + nwu) - load.begin());
+        load[w] += lc.gsteps[g];
         lc.wave_groups[w].push_back(g);
     }
-    // combination: rows round-robin over the threads
-    std::sort(comb_k.begin(), comb_k.end(), std::greater<int>());
+    int mx = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int st = batch_steps(load[w]);
+        mx = std::max(mx, st);
+        tot += st;
+    }
+    std::vector<int> ks = comb_k;
+    std::sort(ks.begin(), ks.end(), std::greater<int>());
     std::vector<int> tl(nt, 0);
-    for (size_t i = 0; i < comb_k.size(); ++i) tl[i % nt] += comb_k[i];
-    const int comb_max = comb_k.empty() ? 0 : *std::max_element(tl.begin(), tl.end());
-    lc.cost = *std::max_element(load.begin(), load.end()) + kCombSteps * comb_max + (comb_k.empty() ? 0.0 : 2.0);
+    for (size_t i = 0; i < ks.size(); ++i) tl[i % nt] += ks[i];
+    const int comb_max = ks.empty() ? 0 : *std::max_element(tl.begin(), tl.end());
+    lc.cost = 0.5 * mx + 0.5 * (double)tot / nw + kCombSteps * comb_max + (ks.empty() ? 0.0 : 2.0);
+    return lc.cost;
+}
+
+// Destination of an item's last step: the third entry word and the bits it
+// adds to the multiplier word (the inverse of the kernels' stream_dest).
+// false when the destination does not fit.
+bool dest_word(int dest_bytes, const StreamLayout& lay, bool rev, uint32_t& e2, uint32_t& wbits)
+{
+    if (lay.absolute) {
+        const uint32_t slot = (uint32_t)(dest_bytes / lay.unit);
+        e2 = slot << 16;
+        wbits = 0;
+        return slot <= 0xffffu;
+    }
+    const int rel = dest_bytes - (rev ? lay.dm_off * lay.unit : 0);
+    e2 = ((uint32_t)rel & 0xffffu) << 16;
+    wbits = (uint32_t)rel >> 16;
+    return rel >= 0 && rel % 8 == 0 && rel < (1 << 18);
 }
 
 // LDS byte addresses (from the LDS base) of the two operands of an entry word.
@@ -299,7 +368,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                   const StreamLayout& lay, int part_budget, StreamPlan& sp, int L, std::string& err)
 {
     const int nw = nt / 64;
-    // entry words (operand offsets, FP32 multiplier)
+    // entry words (operand offsets) and FP32 multipliers
     const uint32_t ms = (uint32_t)lay.m_off;
     auto entry = [&](int e, uint32_t& x, uint32_t& y) -> bool {
         const uint32_t i1 = pair[e] & 0xffffu, i2 = pair[e] >> 16;
@@ -313,22 +382,45 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
         }
         if (a > 0xffffu || b > 0xffffu) return false;
         const float mf = (float)mult[e];
-        if ((double)mf != mult[e]) return false;
+        if ((double)mf != mult[e] || !(mf > 0.0f)) return false;
         x = a | (b << 16);
         std::memcpy(&y, &mf, 4);
-        return true;
+        return (y & 3u) == 0;
     };
     std::vector<int> dest_of_row(l1 - l0);
     for (int pos = l0; pos < l1; ++pos)
         dest_of_row[pos - l0] = (rev ? lay.dm_off + mom[pos] : lay.m_off + mom[pos]) * lay.unit;
+    // entries of each row ordered by multiplier (ties: entry order)
+    std::vector<int> ord(beg.empty() ? 0 : beg[l1 > l0 ? l1 : l0]);
+    for (int pos = l0; pos < l1; ++pos) {
+        std::iota(ord.begin() + beg[pos], ord.begin() + beg[pos + 1], beg[pos]);
+        std::stable_sort(ord.begin() + beg[pos], ord.begin() + beg[pos + 1],
+                         [&](int x, int y) { return mult[x] < mult[y]; });
+    }
 
-    static const int kC[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1 << 30};
     LayerChoice best, lc;
     bool have = false;
-    for (int C : kC) {
-        choose_layer(beg, l0, l1, C, nw, nt, lc, dest_of_row, lay);
-        if (lc.npart > part_budget && C < (1 << 30)) continue;
-        if (!have || lc.cost < best.cost) { best = std::move(lc); have = true; }
+    std::vector<int> comb_k;
+    int max_len = 0;
+    for (int pos = l0; pos < l1; ++pos) max_len = std::max(max_len, beg[pos + 1] - beg[pos]);
+    for (int C = 1; C <= max_len; C += C < 64 ? 1 : (C < 256 ? 4 : 16)) {
+        if (C < max_len) {
+            long long np = 0;
+            for (int pos = l0; pos < l1; ++pos) {
+                const int n = beg[pos + 1] - beg[pos];
+                if (n > C) np += (n + C - 1) / C;
+            }
+            if (np > part_budget) continue;
+        }
+        cut_items(beg, ord, mult, l0, l1, C, lc, dest_of_row, lay, comb_k);
+        for (int nwu = 1; nwu <= nw; ++nwu) {
+            assign_waves(lc, nw, nwu, nt, comb_k);
+            if (!have || lc.cost < best.cost) { best = lc; have = true; }
+        }
+    }
+    if (!have) {  // no entries in this layer
+        cut_items(beg, ord, mult, l0, l1, 1 << 30, best, dest_of_row, lay, comb_k);
+        assign_waves(best, nw, nw, nt, comb_k);
     }
     sp.max_partials = std::max(sp.max_partials, best.npart);
     sp.cost += best.cost;
@@ -337,8 +429,8 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     sp.cb_beg[L] = (int)sp.comb.size();
     sp.bo_beg[L] = (int)sp.bo.size();
     sp.xa_beg[L] = (int)sp.xa.size();
-    // partial slots are numbered in row order (choose_layer); a row's
-    // range is read back from its items' destinations
+    // partial slots are numbered in row order (cut_items); a row's range is
+    // read back from its items' destinations
     {
         std::vector<int> first(l1 - l0, -1), cnt(l1 - l0, 0);
         for (const Item& it : best.items) {
@@ -360,82 +452,117 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     }
 
     // streams
+    const int pass = 16, nbank = 16;
     for (int w = 0; w < nw; ++w) {
         const int first_step = (int)(sp.tbl.size() / 64);
-        int step = 0;
+        int wsteps = 0;
+        for (int g : best.wave_groups[w]) wsteps += best.gsteps[g];
+        const int steps = wsteps > 0 ? batch_steps(wsteps) : 0;
+        const size_t t0 = sp.tbl.size();
+        sp.tbl.resize(t0 + (size_t)steps * 64, U4{0u, 0u, 0u, 0u});
+        auto at = [&](int step, int l) -> U4& { return sp.tbl[t0 + (size_t)step * 64 + l]; };
+        int s0 = 0;
         for (int g : best.wave_groups[w]) {
-            const int steps = best.gsteps[g];
-            const size_t t0 = sp.tbl.size();
-            sp.tbl.resize(t0 + (size_t)steps * 64, U4{0u, 0u, 0u, 0u});
-            // entry words of every lane's item
-            std::vector<std::vector<U2>> ent(64);
+            const int G = best.gsteps[g];
+            // Per lane: the runs of its item and, per step of the group, the
+            // run it draws from and how many entries (-1: zero step).
+            std::vector<std::vector<std::pair<int, int>>> runs(64);
+            std::vector<std::vector<int>> srun(64, std::vector<int>(G, -1)), scnt(64, std::vector<int>(G, 0));
             for (int l = 0; l < 64; ++l) {
                 const size_t ii = (size_t)g * 64 + l;
-                U4& last = sp.tbl[t0 + (size_t)(steps - 1) * 64 + l];
-                last.z = 0u;  // lane without an item: zero entries, sum to lds[0]
-                last.w = kStreamFlush;
-                if (ii >= best.items.size()) continue;
+                U4& last = at(s0 + G - 1, l);
+                uint32_t e2 = 0, wbits = 0;
+                if (ii >= best.items.size()) {  // lane without an item: zero entries, no store (zero slot)
+                    last.z = 0u;
+                    last.w = kStreamEnd | (lay.absolute ? 0u : 3u);
+                    continue;
+                }
                 const Item& it = best.items[ii];
-                last.z = (uint32_t)it.dest;
-                for (int j = 0; j < it.len; ++j) {
-                    uint32_t x, y;
-                    if (!entry(it.e0 + j, x, y)) {
-                        err = "an operand offset or multiplier does not fit the packed contraction entries";
-                        return false;
+                item_runs(ord, mult, it.e0, it.len, &runs[l]);
+                const int nr = (int)runs[l].size();
+                // last run: the final step takes its remainder (or two entries)
+                const int cl = runs[l][nr - 1].second;
+                const int fin = cl % 3 ? cl % 3 : std::min(2, cl);
+                int st = 0;
+                for (int r = 0; r < nr; ++r) {
+                    int c = runs[l][r].second - (r == nr - 1 ? fin : 0);
+                    while (c > 0) {
+                        srun[l][st] = r;
+                        scnt[l][st] = std::min(3, c);
+                        c -= scnt[l][st];
+                        ++st;
                     }
-                    ent[l].push_back(U2{x, y});
                 }
+                if (st != it.steps - 1) {
+                    err = "internal error: contraction item steps";
+                    return false;
+                }
+                srun[l][G - 1] = nr - 1;
+                scnt[l][G - 1] = fin;
+                if (!dest_word(it.dest, lay, rev, e2, wbits)) {
+                    err = "a contraction destination does not fit the packed stream words";
+                    return false;
+                }
+                last.z = e2;
+                last.w = kStreamEnd | wbits;
             }
-            // Entry slots 0 .. 2 steps - 2 (the last is the destination).
-            // Each slot is read by two LDS instructions (operand a, operand
-            // b) over all lanes.  On gfx950 the LDS serves such a read 16
-            // lanes at a time, and two of those lanes whose slots differ
-            // but coincide modulo 16 slots (128 bytes for 8-byte slots, 256
-            // for 16-byte slots) serialize.  Slot by slot, each lane takes the remaining entry
-            // of its item whose operands collide least with those already
-            // placed in its pass (ties: item order).  This fixes the order
-            // in which a lane sums its item: deterministic, as every other
-            // order here.
-            const int pass = 16, nbank = 16;
-            std::vector<size_t> next(64, 0);
+            // Each entry slot of a step is read by two LDS instructions
+            // (operand a, operand b) over all lanes.  On gfx950 the LDS
+            // serves such a read 16 lanes at a time, and two of those lanes
+            // whose slots differ but coincide modulo 16 slots (128 bytes for
+            // 8-byte slots, 256 for 16-byte slots) serialize.  Slot by slot,
+            // each lane takes the remaining entry of its step's run whose
+            // operands collide least with those already placed in its pass
+            // (ties: entry order).  This fixes the order in which a lane sums
+            // its item: deterministic, as every other order here.
             std::vector<std::vector<char>> used(64);
-            for (int l = 0; l < 64; ++l) used[l].assign(ent[l].size(), 0);
-            for (int sl = 0; sl < 2 * steps - 1; ++sl) {
-                for (int l0 = 0; l0 < 64; l0 += pass) {
-                    std::vector<std::vector<uint32_t>> ba(nbank), bb(nbank);  // distinct addresses per bank
-                    auto cost = [](const std::vector<uint32_t>& v, uint32_t addr) {
-                        return std::find(v.begin(), v.end(), addr) != v.end() ? 0 : (int)v.size();
-                    };
-                    for (int l = l0; l < l0 + pass && l < 64; ++l) {
-                        int pick = -1, best_c = 0;
-                        uint32_t pa = 0, pb = 0;
-                        for (size_t j = next[l]; j < ent[l].size(); ++j) {
-                            if (used[l][j]) continue;
-                            uint32_t a, b;
-                            stream_operand_bytes(ent[l][j].x, lay, rev, a, b);
-                            const int c = cost(ba[(a / lay.unit) % nbank], a) + cost(bb[(b / lay.unit) % nbank], b);
-                            if (pick < 0 || c < best_c) { pick = (int)j; best_c = c; pa = a; pb = b; }
-                            if (c == 0) break;
-                        }
-                        if (pick < 0) continue;  // item done: zero entries (one broadcast address)
-                        used[l][pick] = 1;
-                        while (next[l] < ent[l].size() && used[l][next[l]]) ++next[l];
-                        auto add = [](std::vector<uint32_t>& v, uint32_t addr) {
-                            if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
+            for (int l = 0; l < 64; ++l) {
+                size_t n = 0;
+                for (auto& r : runs[l]) n += (size_t)r.second;
+                used[l].assign(n, 0);
+            }
+            for (int st = 0; st < G; ++st) {
+                for (int j = 0; j < 3; ++j) {
+                    for (int p0 = 0; p0 < 64; p0 += pass) {
+                        std::vector<std::vector<uint32_t>> ba(nbank), bb(nbank);  // distinct addresses per bank
+                        auto cost = [](const std::vector<uint32_t>& v, uint32_t addr) {
+                            return std::find(v.begin(), v.end(), addr) != v.end() ? 0 : (int)v.size();
                         };
-                        add(ba[(pa / lay.unit) % nbank], pa);
-                        add(bb[(pb / lay.unit) % nbank], pb);
-                        U4& q = sp.tbl[t0 + (size_t)(sl / 2) * 64 + l];
-                        if (sl % 2 == 0) { q.x = ent[l][pick].x; q.y = ent[l][pick].y; }
-                        else { q.z = ent[l][pick].x; q.w = ent[l][pick].y; }
+                        for (int l = p0; l < p0 + pass; ++l) {
+                            if (srun[l][st] < 0 || j >= scnt[l][st]) continue;
+                            const auto rr = runs[l][srun[l][st]];
+                            const size_t base = (size_t)(rr.first - best.items[(size_t)g * 64 + l].e0);
+                            int pick = -1, best_c = 0;
+                            uint32_t pa = 0, pb = 0, px = 0, py = 0;
+                            for (int k = 0; k < rr.second; ++k) {
+                                if (used[l][base + k]) continue;
+                                uint32_t x, y, a, b;
+                                if (!entry(ord[rr.first + k], x, y)) {
+                                    err = "an operand offset or multiplier does not fit the packed contraction "
+                                          "entries";
+                                    return false;
+                                }
+                                stream_operand_bytes(x, lay, rev, a, b);
+                                const int c = cost(ba[(a / lay.unit) % nbank], a) + cost(bb[(b / lay.unit) % nbank], b);
+                                if (pick < 0 || c < best_c) { pick = k; best_c = c; pa = a; pb = b; px = x; py = y; }
+                                if (c == 0) break;
+                            }
+                            used[l][base + pick] = 1;
+                            auto add = [](std::vector<uint32_t>& v, uint32_t addr) {
+                                if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
+                            };
+                            add(ba[(pa / lay.unit) % nbank], pa);
+                            add(bb[(pb / lay.unit) % nbank], pb);
+                            U4& q = at(s0 + st, l);
+                            (j == 0 ? q.x : (j == 1 ? q.y : q.z)) = px;
+                            q.w = st == G - 1 ? (q.w | py) : py;
+                        }
                     }
                 }
             }
-            step += steps;
+            s0 += G;
         }
-        const int padded = (step + kStreamUnroll - 1) / kStreamUnroll * kStreamUnroll;
-        sp.tbl.resize(sp.tbl.size() + (size_t)(padded - step) * 64, U4{0u, 0u, 0u, 0u});
-        sp.wave.push_back(I4{first_step, padded, 0, 0});
+        sp.wave.push_back(I4{first_step, steps, 0, 0});
     }
     return true;
 }

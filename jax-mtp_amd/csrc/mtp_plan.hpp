@@ -37,7 +37,21 @@ constexpr int kMaxNu = 12;    // tensor order supported
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
 constexpr int kStreamUnroll = 4;  // steps per prefetch batch of a contraction stream
-constexpr uint32_t kStreamFlush = 0xffffffffu;  // 4th word of a group's last step (never an FP32 multiplier)
+constexpr uint32_t kStreamEnd = 0x80000000u;  // sign bit of a step's multiplier word: the item ends with this step
+constexpr uint32_t kStreamMult = 0x7ffffffcu;  // multiplier bits of an item's last step (see stream_dest)
+
+// Destination of an item's last step, for the layouts with byte offsets: its
+// byte offset from operand b's read base (the LDS base forward, the adjoint
+// region in reverse), its low 16 bits in e2's operand-b field (operand a is
+// the zero slot, so the kernels' reads of that word stay aligned and
+// between the base and the destination; the word's product is not used),
+// and its bits 16-17 in the two low mantissa bits of the multiplier word w
+// (zero in every FP32 multiplier the planner accepts).  A lane without an
+// item has e2 = 0 and both bits set: kStreamNoDest, no store.  The layouts
+// with 16-byte slot indices store the slot in e2's operand-b field.
+constexpr uint32_t stream_dest(uint32_t e2, uint32_t w) { return (e2 >> 16) | ((w & 3u) << 16); }
+constexpr uint32_t kStreamNoDest = 3u << 16;
+
 constexpr int kLdsM = 2;      // m[] starts at this double of the atom kernels' LDS (lds[0] is a zero)
 
 // Device-side view of the plan (all pointers are device pointers).
@@ -76,15 +90,18 @@ struct DevPlan {
     // contraction layers as per-wavefront entry streams (see build_items).
     // Layer L (forward: depth L+1; reverse: rev_layer index L) gives wave w
     // of the workgroup the steps [wave[L * NW + w].x, + .y) of tbl (.y a
-    // multiple of kStreamUnroll).  Step s holds two entries per lane:
-    // tbl[s * 64 + lane] = (a | b << 16, mult, a | b << 16, mult), with a, b
-    // LDS byte offsets of the operands (a from the workgroup's LDS base; b
-    // from it in the forward pass, from the adjoint region in the reverse
-    // pass) and mult an FP32 value.  The last step of a group of 64 items
-    // holds one entry and (dest, kStreamFlush): each lane stores its sum at
-    // the LDS byte offset dest and restarts from zero.  Rows split into
-    // chunks leave partial sums, combined per layer by comb[cb_beg[L] ..
-    // cb_beg[L+1]] = (moment, first partial, end partial, -) in a fixed order.
+    // multiple of kStreamUnroll).  Step s holds up to three entries per lane
+    // that share a multiplier: tbl[s * 64 + lane] = (e0, e1, e2, w) with
+    // e = a | b << 16, a and b LDS byte offsets of the operands (a from the
+    // workgroup's LDS base; b from it in the forward pass, from the adjoint
+    // region in the reverse pass; 0 for an unused entry) and w the FP32
+    // multiplier, so that the lane's sum gains w (a0 b0 + a1 b1 + a2 b2).
+    // The last step of a group of 64 items has the sign bit of w set on
+    // every lane: e2 is then the lane's destination (stream_dest), e0 and e1
+    // its last entries (the sum gains w (a0 b0 + a1 b1)), and the lane stores
+    // its sum there and restarts from zero.  Multipliers are positive FP32 values with two zero low bits.  Rows split into chunks leave partial sums, combined per layer by
+    // comb[cb_beg[L] .. cb_beg[L+1]] = (moment, first partial, end partial,
+    // -) in a fixed order.
     int fwd_cb_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2];
     int rev_bo_beg[kMaxDepth + 2], rev_xa_beg[kMaxDepth + 2];
     const I4* fwd_wave;

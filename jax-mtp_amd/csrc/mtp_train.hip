@@ -28,51 +28,61 @@
 namespace mtp {
 
 // ---------------------------------------------------------------------------
-// Dual contraction streams.  Each entry of the atom kernels' streams
-// (operand slots a, b and an FP32 multiplier w, two entries per lane and
-// step, group ends flagged in the step's fourth word) reads two 16-byte
-// pairs.  Forward: a = (m_a, mt_a), b = (m_b, mt_b) and the sum gains
-// w (m_a m_b, mt_a m_b + m_a mt_b).  Reverse: a = (m_o, mt_o) of the operand,
-// b = (dm_t, dmt_t) of the target it feeds, and the sum gains
+// Dual contraction streams.  Each step of the atom kernels' streams holds up
+// to three entries of a lane with one FP32 multiplier w (operand slots a, b
+// per entry; an item's last step flagged in the sign of w, its third word
+// then the destination slot), and every entry reads two
+// 16-byte pairs.  Forward: a = (m_a, mt_a), b = (m_b, mt_b) and the sum
+// gains w (m_a m_b, mt_a m_b + m_a mt_b).  Reverse: a = (m_o, mt_o) of the
+// operand, b = (dm_t, dmt_t) of the target it feeds, and the sum gains
 // w (m_o dm_t + mt_o dmt_t, m_o dmt_t).
 // ---------------------------------------------------------------------------
+template <bool REV>
+__device__ __forceinline__ double2 dual_term(const double2 a, const double2 b, double2 t)
+{
+    if (REV) {
+        t.x = fma(a.x, b.x, fma(a.y, b.y, t.x));
+        t.y = fma(a.x, b.y, t.y);
+    } else {
+        t.x = fma(a.x, b.x, t.x);
+        t.y = fma(a.x, b.y, fma(a.y, b.x, t.y));
+    }
+    return t;
+}
+
 template <bool REV>
 __device__ __forceinline__ void dual_steps(const uint4* q, char* lb, double2& acc)
 {
     constexpr int U = kStreamUnroll;
-    double2 a0[U], b0[U], a1[U], b1[U];
+    double2 a0[U], b0[U], a1[U], b1[U], a2[U], b2[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         a0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x & 0xffffu) << 4));
         b0[u] = *reinterpret_cast<const double2*>(lb + ((q[u].x >> 16) << 4));
-        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
-        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
+        a1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].y & 0xffffu) << 4));
+        b1[u] = *reinterpret_cast<const double2*>(lb + ((q[u].y >> 16) << 4));
+        a2[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z & 0xffffu) << 4));
+        b2[u] = *reinterpret_cast<const double2*>(lb + ((q[u].z >> 16) << 4));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        {
-            const double w = (double)__uint_as_float(q[u].y);
-            const double wa = w * a0[u].x;
-            if (REV) {
-                acc.x = fma(wa, b0[u].x, fma(w * a0[u].y, b0[u].y, acc.x));
-                acc.y = fma(wa, b0[u].y, acc.y);
-            } else {
-                acc.x = fma(wa, b0[u].x, acc.x);
-                acc.y = fma(wa, b0[u].y, fma(w * a0[u].y, b0[u].x, acc.y));
-            }
-        }
-        if (__builtin_amdgcn_readfirstlane(q[u].w) != kStreamFlush) {
-            const double w = (double)__uint_as_float(q[u].w);
-            const double wa = w * a1[u].x;
-            if (REV) {
-                acc.x = fma(wa, b1[u].x, fma(w * a1[u].y, b1[u].y, acc.x));
-                acc.y = fma(wa, b1[u].y, acc.y);
-            } else {
-                acc.x = fma(wa, b1[u].x, acc.x);
-                acc.y = fma(wa, b1[u].y, fma(w * a1[u].y, b1[u].x, acc.y));
-            }
-        } else {  // group end (wave-uniform): q.z is this lane's destination
-            *reinterpret_cast<double2*>(lb + q[u].z) = acc;
+        // an item's last step has two entries, its third word is the
+        // destination slot (the empty asm keeps the third entry's loads
+        // ahead of the branch)
+        double2 t = dual_term<REV>(a0[u], b0[u], make_double2(0.0, 0.0));
+        t = dual_term<REV>(a1[u], b1[u], t);
+        asm volatile("" : : "v"(a2[u].x), "v"(a2[u].y), "v"(b2[u].x), "v"(b2[u].y));
+        const uint32_t w = q[u].w;
+        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
+            const double m = (double)__uint_as_float(w);
+            t = dual_term<REV>(a2[u], b2[u], t);
+            acc.x = fma(m, t.x, acc.x);
+            acc.y = fma(m, t.y, acc.y);
+        } else {  // item end (wave-uniform)
+            const double m = (double)__uint_as_float(w & kStreamMult);
+            acc.x = fma(m, t.x, acc.x);
+            acc.y = fma(m, t.y, acc.y);
+            *reinterpret_cast<double2*>(lb + ((q[u].z >> 16) << 4)) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }

@@ -4,7 +4,8 @@
 // streams of every layer on the CPU exactly as the atom kernel walks them
 // (LDS as a byte-addressed double array, one accumulator per lane, the
 // group end read from lane 0 and required to agree on all lanes, partial
-// sums combined in their fixed order), and compares the moments and their
+// sums combined in their fixed order; up to three entries per step sharing a
+// multiplier, item ends flagged in its sign), and compares the moments and their
 // adjoints with a direct evaluation of alpha_index_times in file order:
 //   forward  m[i3] += mult * m[i1] * m[i2]
 //   reverse  dE/dm from E = sum_s xi_s m[mapping[s]], entries in reverse order.
@@ -121,28 +122,44 @@ int main(int argc, char** argv)
             std::vector<double> acc(64 * W, 0.0);
             for (int s = wv.x; s < wv.x + wv.y; ++s) {
                 const U4* q = &sp.tbl[(size_t)s * 64];
-                const bool flush = q[0].w == kStreamFlush;
+                const bool end = (q[0].w & kStreamEnd) != 0;  // the kernel reads lane 0's word
                 for (int l = 0; l < 64; ++l) {
-                    if ((q[l].w == kStreamFlush) != flush) return fail("group end differs between lanes");
+                    if (((q[l].w & kStreamEnd) != 0) != end) return fail("item end differs between lanes");
+                    const uint32_t e[3] = {q[l].x, q[l].y, q[l].z};
+                    // the kernel issues both operand reads of all three entries
+                    size_t sa[3], sb[3];
+                    for (int j = 0; j < 3; ++j) sa[j] = slot(e[j] & 0xffffu, false), sb[j] = slot(e[j] >> 16, rev);
+                    if (end && sa[2] != 0) return fail("the third entry of an item's last step is not the zero slot");
+                    const uint32_t mw = end ? (q[l].w & kStreamMult) : q[l].w;
                     float mf;
-                    std::memcpy(&mf, &q[l].y, 4);
-                    const size_t a0 = slot(q[l].x & 0xffffu, false), b0 = slot(q[l].x >> 16, rev);
-                    for (int r = 0; r < W; ++r)
-                        acc[l * W + r] = std::fma((double)mf * lds[a0 * W + r], lds[b0 * W + r], acc[l * W + r]);
-                    if (!flush) {
-                        std::memcpy(&mf, &q[l].w, 4);
-                        const size_t a1 = slot(q[l].z & 0xffffu, false), b1 = slot(q[l].z >> 16, rev);
-                        for (int r = 0; r < W; ++r)
-                            acc[l * W + r] = std::fma((double)mf * lds[a1 * W + r], lds[b1 * W + r], acc[l * W + r]);
+                    std::memcpy(&mf, &mw, 4);
+                    for (int r = 0; r < W; ++r) {
+                        double t = lds[sa[0] * W + r] * lds[sb[0] * W + r];
+                        t = std::fma(lds[sa[1] * W + r], lds[sb[1] * W + r], t);
+                        if (!end) t = std::fma(lds[sa[2] * W + r], lds[sb[2] * W + r], t);
+                        acc[l * W + r] = std::fma((double)mf, t, acc[l * W + r]);
                     }
-                }
-                if (flush)
-                    for (int l = 0; l < 64; ++l)
+                    if (end) {
+                        const uint32_t rel = stream_dest(e[2], q[l].w);
+                        if (!lay.absolute && rel >= kStreamNoDest) {
+                            if (e[2] != 0 || rel != kStreamNoDest) return fail("malformed item end without a store");
+                            for (int r = 0; r < W; ++r) {
+                                if (acc[l * W + r] != 0.0) return fail("a lane without an item summed non-zero");
+                                acc[l * W + r] = 0.0;
+                            }
+                            continue;
+                        }
+                        const size_t d = lay.absolute ? dest_slot((e[2] >> 16) * (uint32_t)lay.unit)
+                                                      : dest_slot(rel + (rev ? (uint32_t)lay.dm_off * 8u : 0u));
                         for (int r = 0; r < W; ++r) {
-                            lds[dest_slot(q[l].z) * W + r] = acc[l * W + r];
+                            lds[d * W + r] = acc[l * W + r];
                             acc[l * W + r] = 0.0;
                         }
+                    }
+                }
             }
+            for (int l = 0; l < 64 * W; ++l)
+                if (acc[l] != 0.0) return fail("a sum left unstored at the end of a stream");
         }
         for (int r = 0; r < W; ++r)
             if (lds[r] != 0.0) return fail("the zero slot was written with a non-zero value");
@@ -197,13 +214,12 @@ int main(int argc, char** argv)
             const StreamPlan& sp = dir ? ip.rev : ip.fwd;
             for (size_t st = 0; st < sp.tbl.size() / 64; ++st) {
                 const U4* q = &sp.tbl[st * 64];
-                for (int half = 0; half < 2; ++half) {
-                    if (half == 1 && q[0].w == kStreamFlush) continue;
+                for (int half = 0; half < 3; ++half) {
                     for (int op = 0; op < 2; ++op)
                         for (int l0 = 0; l0 < 64; l0 += pass) {
                             std::vector<std::vector<uint32_t>> banks(nbank);
                             for (int l = l0; l < l0 + pass; ++l) {
-                                const uint32_t x = half ? q[l].z : q[l].x;
+                                const uint32_t x = half == 0 ? q[l].x : (half == 1 ? q[l].y : q[l].z);
                                 const uint32_t fld = op ? x >> 16 : x & 0xffffu;
                                 uint32_t addr = lay.absolute ? fld * lay.unit
                                                              : fld + ((op && dir) ? (uint32_t)lay.dm_off * 8u : 0u);

@@ -3,9 +3,10 @@
 Workload (the distilled-head configuration): a level-24, 3-species MTP is
 the teacher; the head reads level-12 basis values (29 per atom; 52 at the
 optional level 14) through a 2x64 SiLU network; it is trained on 1e5 site
-environments of thermally displaced 3-species FCC snapshots at 300-1500 K
-and evaluated on a 100,000-atom 3-species FCC cell, with exact level-24
-re-evaluation of every atom outside the training box.  Reported:
+environments of 3-species FCC cells taken from Langevin MD at 300-1500 K
+(one 1 ps trajectory per 256-atom cell, see md_snapshots) and labelled by
+the teacher, and evaluated on a 100,000-atom 3-species FCC cell, with exact
+level-24 re-evaluation of every atom outside the training box.  Reported:
 atom-steps/s of the head (with fallback) and of the plain level-24
 evaluation, the head's force RMSE against the level-24 forces, and the same
 RMSE for the descriptor MTP alone (the method without the network).
@@ -18,10 +19,10 @@ in meV/A.  The teacher therefore normalises each scalar basis function by
 its RMS over 300 K snapshots and damps it with its MTP level (level of
 M_{mu,nu} = 2 + 4 mu + nu, levels add under contraction), a converging
 body-order expansion, then scales everything so that the RMS force at
-300 K is 0.3 eV/A.  The random-parameter teacher has no physical
-equilibrium, so snapshots are Gaussian thermal displacements of the
-lattice, sigma(T) = 0.05 A sqrt(T / 300 K) per component, rather than MD
-frames.
+300 K is 0.3 eV/A.  The random-parameter teacher has no stable lattice, so
+it labels MD frames but does not drive the MD (see md_snapshots).  The
+100,000-atom evaluation cell has Gaussian thermal displacements,
+sigma = 0.05 A sqrt(900 K / 300 K) per component.
 """
 from __future__ import annotations
 
@@ -50,19 +51,138 @@ def basis_levels(p: MTPData) -> np.ndarray:
     return lev
 
 
-def thermal_snapshots(n_env: int, ncell=(4, 4, 4), T_range=(300.0, 1500.0), sigma300=0.05, seed=0):
-    """FCC snapshots (pos, types, cell, T) with at least n_env atoms in total."""
+# Langevin molecular dynamics of independent periodic cells, for training
+# snapshots.  The synthetic teacher cannot drive it: it has no repulsive core
+# and no stable lattice (under it a 300 K FCC cell collapses within about
+# 60 fs, the closest pair going from 2.55 A to 1.8 A, where its radial basis
+# is extrapolated).  The frames therefore come from a Morse pair potential
+# with copper's parameters (Girifalco and Weizer, Phys. Rev. 114, 687
+# (1959): D = 0.3429 eV, alpha = 1.3588 / A, r0 = 2.866 A), whose FCC lattice
+# constant is close to the 3.6 A of the cells, and the teacher labels them.
+AMU = 103.6427     # 1 amu A^2/fs^2 in eV
+KB = 8.617333e-5   # eV/K
+MASSES = (58.6934, 26.9815, 51.9961)  # amu of species 0, 1, 2 (Ni, Al, Cr)
+MORSE = (0.3429, 1.3588, 2.866)       # D (eV), alpha (1/A), r0 (A)
+
+
+def _min_image(x, box):
+    """Pair vectors d[c, i, j] = x_j - x_i of cells x [C, n, 3] under the minimum image."""
+    import torch
+
+    d = x[:, None, :, :] - x[:, :, None, :]
+    return d - box * torch.round(d / box)
+
+
+def _cell_rows(x, types, box, cutoff, K):
+    """Padded rows of C independent periodic cells (box >= 2 cutoff): x [C, n, 3],
+    types [C, n] (device).  Slots hold the K nearest neighbours (minimum
+    image), those beyond the cutoff marked empty (j = C n, type -1, r = 2000)."""
+    import torch
+
+    C, n, _ = x.shape
+    d = _min_image(x, box)
+    r2 = (d * d).sum(-1)
+    r2.diagonal(dim1=1, dim2=2).fill_(float("inf"))
+    k = min(K, n - 1)
+    vals, idx = torch.topk(r2, k, dim=2, largest=False)
+    if k == K and bool((vals[:, :, -1] < cutoff * cutoff).any()):
+        raise RuntimeError("more than K neighbours within the cutoff")
+    valid = vals < cutoff * cutoff
+    rij = torch.gather(d, 2, idx[..., None].expand(C, n, k, 3))
+    rij = torch.where(valid[..., None], rij, torch.full_like(rij, 2000.0))
+    off = (torch.arange(C, device=x.device) * n)[:, None, None]
+    js = torch.where(valid, idx + off, torch.full_like(idx, C * n)).to(torch.int32)
+    jt = torch.gather(types[:, None, :].expand(C, n, n), 2, idx)
+    jt = torch.where(valid, jt, torch.full_like(jt, -1)).to(torch.int32)
+    pad = K - k
+    if pad:
+        rij = torch.cat([rij, torch.full((C, n, pad, 3), 2000.0, dtype=rij.dtype, device=x.device)], 2)
+        js = torch.cat([js, torch.full((C, n, pad), C * n, dtype=js.dtype, device=x.device)], 2)
+        jt = torch.cat([jt, torch.full((C, n, pad), -1, dtype=jt.dtype, device=x.device)], 2)
+    return (types.reshape(-1).to(torch.int32), js.reshape(C * n, K), rij.reshape(C * n, K, 3),
+            jt.reshape(C * n, K))
+
+
+def _morse_forces(x, box, cutoff):
+    """Forces [C, n, 3] of the Morse pair potential MORSE, force shifted to zero at the cutoff."""
+    import torch
+
+    D, al, r0 = MORSE
+    d = _min_image(x, box)
+    r = torch.sqrt((d * d).sum(-1))
+    r.diagonal(dim1=1, dim2=2).fill_(float("inf"))
+
+    def dv(rr):  # dV/dr
+        e = torch.exp(-al * (rr - r0))
+        return 2.0 * al * D * (e - e * e)
+
+    dvc = float(dv(torch.tensor(cutoff, dtype=torch.float64)))
+    w = torch.where(r < cutoff, (dv(r) - dvc) / r, torch.zeros_like(r))
+    return (w[..., None] * d).sum(2)   # dV/dr > 0 pulls i toward j
+
+
+def md_snapshots(n_cells: int, ncell=(4, 4, 4), T_range=(300.0, 1500.0), steps=1000, dt=1.0, gamma=0.01,
+                 seed=0, driver=None, stats_every=0, log=None):
+    """Langevin MD (BAOAB, time step dt fs, friction gamma 1/fs) of n_cells
+    independent 3-species FCC cells (a = 3.6 A, species 40/30/30, masses
+    MASSES), each at its own temperature drawn uniformly from T_range,
+    started from the lattice with Maxwell velocities.  The forces are those
+    of the Morse potential MORSE, or of an MTP when driver is one.  Returns
+    the last frame of every cell as (pos, types, cell, T) and, with
+    stats_every, a list of per-interval statistics (step, mean kinetic
+    temperature, smallest pair distance, largest |F|)."""
+    import torch
+
     rng = np.random.default_rng(seed)
+    dev = torch.device("cuda")
     base, cell = configs.fcc(A_FCC, ncell)
-    out = []
-    total = 0
-    while total < n_env:
-        T = float(rng.uniform(*T_range))
-        pos = base + rng.normal(0.0, sigma300 * np.sqrt(T / 300.0), base.shape)
-        types = configs.species_by_fraction(len(base), FRACTIONS, rng)
-        out.append((pos, types, cell, T))
-        total += len(base)
-    return out
+    n = len(base)
+    types_np = np.stack([configs.species_by_fraction(n, FRACTIONS, rng) for _ in range(n_cells)])
+    T = rng.uniform(*T_range, n_cells)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    x = torch.as_tensor(np.broadcast_to(base, (n_cells, n, 3)).copy(), dtype=torch.float64, device=dev)
+    types = torch.as_tensor(types_np, dtype=torch.int64, device=dev)
+    box = torch.as_tensor(np.asarray(cell, dtype=float), dtype=torch.float64, device=dev)
+    m = torch.as_tensor(np.asarray(MASSES)[types_np], dtype=torch.float64, device=dev)[..., None] * AMU
+    kT = torch.as_tensor(KB * T, dtype=torch.float64, device=dev)[:, None, None]
+    sv = torch.sqrt(kT / m)
+    v = sv * torch.randn(x.shape, dtype=torch.float64, device=dev, generator=gen)
+    N = n_cells * n
+    pot = MTPPotential(driver, max_atoms=N, max_nbr=MAX_NBR) if driver is not None else None
+
+    def forces(xc):
+        if pot is None:
+            return _morse_forces(xc, box, CUTOFF)
+        it, js, rij, jt = _cell_rows(xc, types, box, CUTOFF, MAX_NBR)
+        _, F, _ = pot.calc_energy_forces_stress_padded(it, js, rij, jt, 3, 1.0, N, N)
+        return F.reshape(n_cells, n, 3)
+
+    c1 = float(np.exp(-gamma * dt))
+    c2 = float(np.sqrt(1.0 - c1 * c1))
+    stats = []
+    F = forces(x)
+    for step in range(1, steps + 1):
+        v = v + 0.5 * dt * F / m
+        x = x + 0.5 * dt * v
+        v = c1 * v + c2 * sv * torch.randn(x.shape, dtype=torch.float64, device=dev, generator=gen)
+        x = x + 0.5 * dt * v
+        F = forces(x)
+        v = v + 0.5 * dt * F / m
+        if stats_every and step % stats_every == 0:
+            tk = float((m * v * v).sum() / (3 * N * KB))
+            d = _min_image(x, box)
+            r2 = (d * d).sum(-1)
+            r2.diagonal(dim1=1, dim2=2).fill_(float("inf"))
+            row = (step, tk, float(torch.sqrt(r2.min())), float(F.norm(dim=-1).max()))
+            stats.append(row)
+            if log:
+                log(f"[md] step {step}: T_kin {row[1]:.0f} K, closest pair {row[2]:.3f} A, max |F| {row[3]:.2f} eV/A")
+    if pot is not None:
+        pot.close()
+    xs = x.cpu().numpy()
+    snaps = [(xs[c], types_np[c].astype(np.int32), cell, float(T[c])) for c in range(n_cells)]
+    return (snaps, stats) if stats_every else snaps
 
 
 def inference_cell(ncell=(25, 25, 40), T=900.0, sigma300=0.05, seed=1):
@@ -82,7 +202,7 @@ def teacher_params(seed=0, decay=0.25, f_rms=0.3, calib_cells=8) -> MTPData:
     p = configs.random_params(24, 3, 10, 2.0, CUTOFF, rng)
     lev = basis_levels(p)[np.asarray(p.alpha_moment_mapping)]
     raw = rng.normal(0.0, 1.0, len(lev)) * decay ** ((lev - 2) / 2.0)
-    snaps = thermal_snapshots(calib_cells * 256, T_range=(300.0, 300.0), seed=seed + 100)
+    snaps = md_snapshots(calib_cells, T_range=(300.0, 300.0), seed=seed + 100)
     batch = training.pack_configs([s[:3] for s in snaps], CUTOFF, MAX_NBR)
     pot = MTPPotential(p, max_atoms=batch.natoms, max_nbr=MAX_NBR)
     b = pot.descriptor(batch.itypes, batch.rij, batch.jtypes)
@@ -185,7 +305,7 @@ def _time(fn, reps=5, warmup=2):
 
 
 def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, seed=0, decay=0.25, log=print,
-        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=12, pair_mus=()):
+        save_dir=None, head_lr=3e-3, head_weights=(1.0, 1.0), desc_level=12, pair_mus=(), md_steps=1000):
     """Teacher, training set, descriptor fit, head training, then evaluate();
     with save_dir the trained head (head.npz) and the teacher (teacher.mtp)
     are written there for infer()."""
@@ -193,9 +313,10 @@ def run(n_env=100_000, infer_ncell=(25, 25, 40), head_steps=1500, desc_steps=0, 
 
     t_start = time.time()
     teacher = teacher_params(seed, decay=decay)
-    snaps = thermal_snapshots(n_env, seed=seed + 1)
+    snaps = md_snapshots(-(-n_env // 256), steps=md_steps, seed=seed + 1)
     batch = training.pack_configs([s[:3] for s in snaps], CUTOFF, MAX_NBR)
-    log(f"[distill] {len(snaps)} snapshots, {batch.natoms} environments, K={batch.nslots}")
+    log(f"[distill] {len(snaps)} MD snapshots ({md_steps} fs each, 300-1500 K), {batch.natoms} environments, "
+        f"K={batch.nslots}")
     tpot = MTPPotential(teacher, max_atoms=batch.natoms, max_nbr=MAX_NBR)
     E_site, F, E_c, S_c = label(tpot, batch)
     tpot.close()

@@ -288,11 +288,20 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->lds_max = lds_max;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || c->n_cu <= 0)
         c->n_cu = 256;
+    // Moments and adjoints beyond one workgroup's LDS (level 28): the
+    // per-target kernel keeps them in a global slice per resident workgroup.
+    bool global_moments = false;
     if ((long long)c->lds_bytes > (long long)lds_max) {
-        set_global_error("potential needs " + std::to_string(c->lds_bytes) + " bytes of LDS per atom, device has " +
-                         std::to_string(lds_max));
-        delete c;
-        return MTP_ERR_UNSUPPORTED;
+        const size_t lg = atom_lds_bytes(f.alpha_moments_count, max_nbr, f.radial_funcs_count, p.nu_max, p.n_parts,
+                                         true);
+        if ((long long)lg > (long long)lds_max) {
+            set_global_error("potential needs " + std::to_string(lg) + " bytes of LDS per atom, device has " +
+                             std::to_string(lds_max));
+            delete c;
+            return MTP_ERR_UNSUPPORTED;
+        }
+        c->lds_bytes = lg;
+        global_moments = true;
     }
     e = set_atom_kernel_lds(kVariantV1, c->lds_bytes);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);  // default priority
@@ -334,6 +343,11 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->d_csr_state = c->alloc<int>(8, e);
     c->d_js_copy = c->alloc<int>(A * K, e);
     c->d_status = c->alloc<int>(8, e);
+    const int m_grid = global_moments ? (int)std::min<size_t>(A, (size_t)4 * (size_t)c->n_cu) : 0;
+    const size_t m_stride = global_moments ? atom_global_moment_doubles(f.alpha_moments_count, max_nbr, p.nu_max,
+                                                                       p.n_parts)
+                                           : 0;
+    double* d_mscratch = global_moments ? c->alloc<double>((size_t)m_grid * m_stride, e) : nullptr;
     if (e == hipSuccess) {  // no previous build yet (valid flag 0); status clear
         static const int zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         e = hipMemcpyAsync(c->d_csr_state, zeros, sizeof(zeros), hipMemcpyHostToDevice, c->stream);
@@ -387,6 +401,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.lds_tables = 1;
     d.coef_lds = 0;
     d.n_mapped = (int)p.map_x.size(); d.map_x = map_x; d.map_beg = map_beg; d.map_s = map_s;
+    d.mscratch = d_mscratch; d.mscratch_grid = m_grid; d.mscratch_stride = (long long)m_stride;
 
     // --- kernel variants ------------------------------------------------
     c->vdev[kVariantV1] = d;

@@ -44,17 +44,21 @@ __device__ __forceinline__ void flag_nonfinite(int* status, bool bad)
     if (status && m != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(status, 1);
 }
 
-// LDS layout (in doubles) shared by host sizing and the kernel.
-__host__ __device__ inline AtomLds atom_lds_layout(int amc, int nslots, int R, int nu_max, int n_parts)
+// LDS layout (in doubles) shared by host sizing and the kernel.  gm: the
+// moment and adjoint regions (mlen + dlen doubles) live in a global slice.
+__host__ __device__ inline AtomLds atom_lds_layout(int amc, int nslots, int R, int nu_max, int n_parts,
+                                                   bool gm = false)
 {
     AtomLds L;
     int mlen = amc;
     if (mlen < n_parts * nslots * 3) mlen = n_parts * nslots * 3;
     int dlen = amc;
     if (dlen < nslots * 3 * (nu_max + 1)) dlen = nslots * 3 * (nu_max + 1);
+    L.mlen = mlen;
+    L.dlen = dlen;
     L.m = 0;
-    L.dm = L.m + mlen;
-    L.f = L.dm + dlen;
+    L.dm = gm ? 0 : L.m + mlen;
+    L.f = gm ? 0 : L.dm + dlen;
     L.fp = L.f + R * nslots;
     L.vx = L.fp + R * nslots;
     L.vy = L.vx + nslots;
@@ -66,21 +70,27 @@ __host__ __device__ inline AtomLds atom_lds_layout(int amc, int nslots, int R, i
     return L;
 }
 
-size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts)
+size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts, bool global_moments)
 {
-    return sizeof(double) * (size_t)atom_lds_layout(amc, nslots, R, nu_max, n_parts).total_doubles;
+    return sizeof(double) * (size_t)atom_lds_layout(amc, nslots, R, nu_max, n_parts, global_moments).total_doubles;
 }
 
-template <typename RadialReal>
-__global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs A)
+size_t atom_global_moment_doubles(int amc, int nslots, int nu_max, int n_parts)
 {
-    extern __shared__ double lds[];
-    const int row = blockIdx.x;
+    const AtomLds L = atom_lds_layout(amc, nslots, 1, nu_max, n_parts, true);
+    return (size_t)L.mlen + (size_t)L.dlen;
+}
+
+// One row of the per-target kernel.  gm: the moments and adjoints in the
+// global slice gm[0, mlen + dlen) of this workgroup, else in LDS.
+template <typename RadialReal>
+__device__ __forceinline__ void atom_row_v1(const DevPlan& P, const AtomArgs& A, const int row, double* lds,
+                                            const AtomLds& L, double* gm)
+{
     const int tid = threadIdx.x;
     const int K = A.nslots;
-    const AtomLds L = atom_lds_layout(P.amc, K, P.R, P.nu_max, P.n_parts);
-    double* m = lds + L.m;
-    double* dm = lds + L.dm;
+    double* m = gm ? gm : lds + L.m;
+    double* dm = gm ? gm + L.mlen : lds + L.dm;
     double* f = lds + L.f;
     double* fp = lds + L.fp;
     double* vx = lds + L.vx;
@@ -272,6 +282,25 @@ __global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs 
         double s = 0.0;
         for (int k = 0; k < nv; ++k) s += va[k] * gk[3 * k + b];
         A.Wi[(size_t)row * 9 + tid] = s;
+    }
+}
+
+// GM false: one workgroup per row (grid = rows).  GM true: a grid of
+// P.mscratch_grid workgroups walks the rows, each with its global slice
+// (barrier between rows: the next row reuses the LDS and the slice).
+template <typename RadialReal, bool GM>
+__global__ void __launch_bounds__(kBlock) atom_kernel(const DevPlan P, AtomArgs A)
+{
+    extern __shared__ double lds[];
+    const AtomLds L = atom_lds_layout(P.amc, A.nslots, P.R, P.nu_max, P.n_parts, GM);
+    if (!GM) {
+        atom_row_v1<RadialReal>(P, A, blockIdx.x, lds, L, nullptr);
+        return;
+    }
+    double* gm = P.mscratch + (size_t)blockIdx.x * (size_t)P.mscratch_stride;
+    for (int row = blockIdx.x; row < A.nrows; row += gridDim.x) {
+        atom_row_v1<RadialReal>(P, A, row, lds, L, gm);
+        __syncthreads();
     }
 }
 
@@ -1549,10 +1578,17 @@ hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes)
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
         }
         default: {
-            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double>,
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel<double, false>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel<float, false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel<double, true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
             if (e != hipSuccess) return e;
-            return hipFuncSetAttribute((const void*)atom_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            return hipFuncSetAttribute((const void*)atom_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       b);
         }
     }
 }
@@ -1601,8 +1637,15 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
             break;
         default:
             if (modes) return hipErrorInvalidValue;  // the per-target kernel has no optional modes
-            if (radial_fp32) hipLaunchKernelGGL(atom_kernel<float>, dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
-            else hipLaunchKernelGGL(atom_kernel<double>, dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
+            if (P.mscratch) {  // moments in global memory: a fixed grid walks the rows
+                const int g = grid < P.mscratch_grid ? grid : P.mscratch_grid;
+                if (radial_fp32) hipLaunchKernelGGL((atom_kernel<float, true>), dim3(g), dim3(kBlock), lds_bytes, stream, P, A);
+                else hipLaunchKernelGGL((atom_kernel<double, true>), dim3(g), dim3(kBlock), lds_bytes, stream, P, A);
+            } else if (radial_fp32) {
+                hipLaunchKernelGGL((atom_kernel<float, false>), dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
+            } else {
+                hipLaunchKernelGGL((atom_kernel<double, false>), dim3(grid), dim3(kBlock), lds_bytes, stream, P, A);
+            }
             break;
     }
     return hipGetLastError();

@@ -56,6 +56,7 @@ constexpr int kTotalsBlocks = 128;
 
 struct AtomLds {
     int m, dm, f, fp, vx, vy, vz, r, red, ints, total_doubles;
+    int mlen, dlen;  // doubles of the moment and adjoint regions (in LDS, or a global slice)
 };
 
 struct AtomLds2 {
@@ -182,7 +183,10 @@ constexpr int variant_parts(int v)
     return (v == kVariantV2x512 || v == kVariantPair1024) ? 8 : (v == kVariantV2x64 ? 1 : 4);
 }
 
-size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts);
+// global_moments: the moment and adjoint regions in a global slice of
+// atom_global_moment_doubles per workgroup instead of LDS (DevPlan::mscratch).
+size_t atom_lds_bytes(int amc, int nslots, int R, int nu_max, int n_parts, bool global_moments = false);
+size_t atom_global_moment_doubles(int amc, int nslots, int nu_max, int n_parts);
 size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max, int n_parts, int nt,
                        int max_partials, int xi_len = 0, bool tables = true, int coef_len = 0);
 hipError_t set_atom_kernel_lds(int variant, size_t bytes);

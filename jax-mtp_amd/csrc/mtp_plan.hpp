@@ -118,6 +118,13 @@ struct DevPlan {
     int coef_lds;                    // > 0: the radial coefficients of the row's species (Sp * R * N
                                      // doubles) are staged in LDS; 0: read from global memory
     int max_partials;
+    // Per-target kernel with the moments and adjoints in global memory (levels
+    // whose moments exceed the LDS): a grid of mscratch_grid workgroups walks
+    // the rows, each workgroup in its slice of mscratch_stride doubles.  Null
+    // when they live in LDS.
+    double* mscratch;
+    int mscratch_grid;
+    long long mscratch_stride;
     // inverse of alpha_moment_mapping: distinct mapped moments (increasing)
     // and, per moment, the scalar indices s mapped to it (increasing)
     int n_mapped;

@@ -45,6 +45,13 @@ def _run_oracle(O, w, nb, n_e=None, n_f=None):
     return dict(E=E, F=F, s6=s6, eatom=Ei, g=g)
 
 
+def _run_potential(pot, w, nb):
+    E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
+                                                           w.volume, w.natoms, w.natoms, return_all=True)
+    return dict(E=float(E.item()), F=F.cpu().numpy(), s6=s6.cpu().numpy(), eatom=extra["eatom"].cpu().numpy(),
+                g=extra["g"].cpu().numpy(), status=float(extra["status"].item()))
+
+
 def _check(gpu, ref, tol=TOL):
     print(f"E {gpu['E']:.17g} ref {ref['E']:.17g} | rel F {_rel(gpu['F'], ref['F']):.3e} "
           f"| rel s6 {_rel(gpu['s6'], ref['s6']):.3e} | rel Ei {_rel(gpu['eatom'], ref['eatom']):.3e}", flush=True)
@@ -373,10 +380,10 @@ def test_level26_and_level28(jm, O):
     """Levels 26 and 28 (alpha tables of the reference's untrained_mtps/26.mtp, 28.mtp; no JSON trees exist
     for them, so the oracle is the compact form).  Level 26's contraction tables overflow the streams'
     16-bit operand offsets, so it runs on the per-target kernel (its moments and adjoints, 9,396 of each,
-    just fit one workgroup's LDS at K = 80); level 28 needs 290 KB of LDS per atom and is refused with
-    MTP_ERR_UNSUPPORTED at open."""
-    import ctypes
-
+    just fit one workgroup's LDS at K = 80).  Level 28's (17,366 of each, 290 KB) do not: the per-target
+    kernel keeps them in a global slice per resident workgroup, and a fixed grid of workgroups walks the
+    rows.  Checked on a small cluster and on a 1,372-atom FCC cell, more rows than the grid, so that
+    workgroups reuse their slices."""
     rng = np.random.default_rng(26)
     p = jm.mtp_io.untrained(26)
     p.species_count, p.radial_basis_size, p.min_dist, p.max_dist = 2, 8, 1.5, 5.0
@@ -389,17 +396,25 @@ def test_level26_and_level28(jm, O):
     print(f"[case] level 26, {w.natoms} atoms", flush=True)
     pot = jm.MTPPotential(p, max_atoms=w.natoms, max_nbr=80)
     assert pot.variant == 0 and pot.info()["level"] == 26
-    E, F, s6, extra = pot.calc_energy_forces_stress_padded(nb["itypes"], nb["js"], nb["rij"], nb["jtypes"], 3,
-                                                           w.volume, w.natoms, w.natoms, return_all=True)
+    gpu = _run_potential(pot, w, nb)
     pot.close()
-    gpu = dict(E=float(E.item()), F=F.cpu().numpy(), s6=s6.cpu().numpy(), eatom=extra["eatom"].cpu().numpy(),
-               g=extra["g"].cpu().numpy(), status=float(extra["status"].item()))
     _check(gpu, _run_oracle(O, w, nb))
+
     q = jm.mtp_io.untrained(28)
-    q.species_count, q.radial_basis_size, q.min_dist, q.max_dist = 1, 8, 1.5, 5.0
-    q.initialize(rng)
-    L = jm.capi.lib()
-    ctx = ctypes.c_void_p()
-    rc = L.mtp_open_text(jm.format_mtp(q).encode(), 16, 80, 5.0, 0, ctypes.byref(ctx))
-    assert rc == jm.capi.MTP_ERR_UNSUPPORTED, rc
-    assert b"LDS" in L.mtp_global_error()
+    q.species_count, q.radial_basis_size, q.min_dist, q.max_dist = 2, 8, 1.5, 5.0
+    q.species_coeffs = rng.normal(0, 0.1, 2)
+    q.moment_coeffs = rng.normal(0, 0.1, len(q.alpha_moment_mapping))
+    q.radial_coeffs = rng.normal(0, 0.1, (2, 2, q.radial_funcs_count, 8))
+    pos, cell = jm.configs.fcc(3.6, (7, 7, 7))
+    cases = [jm.configs.small_cluster(10, 2, 12, seed=28, cell=7.0, cutoff=5.0, max_nbr=80),
+             jm.configs.Workload("fcc1372_level28", None, pos + rng.normal(0.0, 0.05, pos.shape),
+                                 rng.integers(0, 2, len(pos)).astype(np.int32), cell, 5.0, 80)]
+    for w in cases:
+        w.params = q
+        nb = w.neighbors()
+        print(f"[case] level 28, {w.natoms} atoms", flush=True)
+        pot = jm.MTPPotential(q, max_atoms=w.natoms, max_nbr=80)
+        assert pot.variant == 0 and pot.info()["level"] == 28
+        gpu = _run_potential(pot, w, nb)
+        pot.close()
+        _check(gpu, _run_oracle(O, w, nb))

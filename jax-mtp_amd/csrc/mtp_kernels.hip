@@ -340,20 +340,16 @@ __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, con
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        // three entries with one multiplier; an item's last step has two,
-        // its third word is the destination.  The empty asm keeps the third
-        // entry's loads ahead of the branch (the compiler would otherwise
-        // sink them into it and wait there for every read in flight).
+        // three entries with one integer multiplier; on the last step of
+        // the lane's item (per lane) the sum goes to the item's destination
+        // and restarts from zero
         double t = a0[u] * b0[u];
         t = fma(a1[u], b1[u], t);
-        asm volatile("" : : "v"(a2[u]), "v"(b2[u]));
+        t = fma(a2[u], b2[u], t);
         const uint32_t w = q[u].w;
-        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
-            acc = fma((double)__uint_as_float(w), fma(a2[u], b2[u], t), acc);
-        } else {  // item end (wave-uniform)
-            acc = fma((double)__uint_as_float(w & kStreamMult), t, acc);
-            const uint32_t d = stream_dest(q[u].z, w);
-            if (d < kStreamNoDest) *reinterpret_cast<double*>(const_cast<char*>(bb) + d) = acc;
+        acc = fma((double)stream_mult(w), t, acc);
+        if (w & kStreamEnd) {
+            *reinterpret_cast<double*>(lb + ((size_t)stream_dest_slot(w) << 3)) = acc;
             acc = 0.0;
         }
     }
@@ -958,25 +954,22 @@ __device__ __forceinline__ void stream_steps_pair(const uint4* q, char* lb, doub
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        // three entries with one multiplier; an item's last step has two,
-        // its third word is the destination slot (the empty asm keeps the
-        // third entry's loads ahead of the branch)
+        // three entries with one integer multiplier; on the last step of
+        // the lane's item (per lane) the sums go to the item's destination
+        // slot and restart from zero
         double2 t;
         t.x = a0[u].x * b0[u].x;
         t.y = a0[u].y * b0[u].y;
         t.x = fma(a1[u].x, b1[u].x, t.x);
         t.y = fma(a1[u].y, b1[u].y, t.y);
-        asm volatile("" : : "v"(a2[u].x), "v"(a2[u].y), "v"(b2[u].x), "v"(b2[u].y));
+        t.x = fma(a2[u].x, b2[u].x, t.x);
+        t.y = fma(a2[u].y, b2[u].y, t.y);
         const uint32_t w = q[u].w;
-        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
-            const double m = (double)__uint_as_float(w);
-            acc.x = fma(m, fma(a2[u].x, b2[u].x, t.x), acc.x);
-            acc.y = fma(m, fma(a2[u].y, b2[u].y, t.y), acc.y);
-        } else {  // item end (wave-uniform)
-            const double m = (double)__uint_as_float(w & kStreamMult);
-            acc.x = fma(m, t.x, acc.x);
-            acc.y = fma(m, t.y, acc.y);
-            *reinterpret_cast<double2*>(lb + ((q[u].z >> 16) << 4)) = acc;
+        const double m = (double)stream_mult(w);
+        acc.x = fma(m, t.x, acc.x);
+        acc.y = fma(m, t.y, acc.y);
+        if (w & kStreamEnd) {
+            *reinterpret_cast<double2*>(lb + ((size_t)stream_dest_slot(w) << 4)) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }

@@ -2,8 +2,10 @@
 #include "mtp_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
+#include <queue>
 
 namespace mtp {
 
@@ -219,19 +221,18 @@ bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
 // multiplier first, so a chunk holds few distinct multipliers.
 //
 // A step carries up to three entries of one lane that share a multiplier
-// (the kernel forms w (a0 b0 + a1 b1 + a2 b2)), so an item takes, per
-// multiplier of its entries, ceil(count / 3) steps; its last step holds at
-// most two entries and, in place of the third, the item's destination, with
-// the sign bit of the multiplier word set (multipliers are positive).  Items
-// are sorted by steps (most first, ties in row order) and taken 64 at a time
-// as groups: one item per lane, shorter items padded with zero steps before
-// their last step, so a group ends at the same step on all lanes of a
-// wavefront and the kernel's loop has no per-lane control flow.  A wave's
-// steps are padded to whole prefetch batches of kStreamUnroll steps.  Groups
-// go to the least-loaded of the first nwu waves (most steps first).  C and
-// nwu are chosen per layer by a model of the time: the mean of the slowest
-// wave's steps and of the steps of all waves (the LDS is shared by the
-// workgroups of a CU), plus the partial-sum combination.
+// (the kernel forms mult (a0 b0 + a1 b1 + a2 b2)), so an item takes, per
+// multiplier of its entries, ceil(count / 3) steps; its last step carries
+// the end flag and the item's destination in the multiplier word.  Every
+// lane of a wavefront walks its own queue of items back to back (the end
+// flag is per lane), so a lane never waits with zero steps for a longer
+// item of another lane: items go, most steps first (ties in row order), to
+// the least-loaded lane of the first nwu waves (ties: lowest lane), and a
+// wave's stream is its longest lane queue, padded with zero steps to whole
+// prefetch batches of kStreamUnroll steps.  C and nwu are chosen per layer
+// by a model of the time: the mean of the slowest wave's steps and of the
+// steps of all waves (the LDS is shared by the workgroups of a CU), plus
+// the partial-sum combination.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -240,9 +241,9 @@ struct Item { int len, e0, row, dest, steps; };  // entries ord[e0, e0 + len)
 constexpr double kCombSteps = 0.5;     // per partial sum added in the combination
 
 struct LayerChoice {
-    std::vector<Item> items;           // sorted
-    std::vector<std::vector<int>> wave_groups;
-    std::vector<int> gsteps;           // steps of each group
+    std::vector<Item> items;                  // sorted, most steps first
+    std::vector<std::vector<int>> lane_items;  // [nw * 64]: the items of each lane, in walking order
+    std::vector<int> wave_steps;              // [nw]: the longest lane queue of each wave
     int npart = 0;
     double cost = 0;
 };
@@ -253,9 +254,7 @@ int batch_steps(int steps)
 }
 
 // Runs of equal multipliers of entries ord[e0, e0 + n) (ordered by
-// multiplier), in the order the item's steps take them: the first run whose
-// length is not a multiple of 3 goes last, so its remainder fills the last
-// step.  Returns the steps.
+// multiplier), in that order.  Returns the steps.
 int item_runs(const std::vector<int>& ord, const std::vector<double>& mult, int e0, int n,
               std::vector<std::pair<int, int>>* runs_out)
 {
@@ -268,9 +267,6 @@ int item_runs(const std::vector<int>& ord, const std::vector<double>& mult, int 
     }
     int steps = 0;
     for (const auto& r : runs) steps += (r.second + 2) / 3;
-    const auto rem = std::find_if(runs.begin(), runs.end(), [](const std::pair<int, int>& r) { return r.second % 3 != 0; });
-    if (rem != runs.end()) std::rotate(rem, rem + 1, runs.end());
-    else ++steps;  // every run fills whole steps: the last step holds two entries of the last run
     if (runs_out) *runs_out = std::move(runs);
     return steps;
 }
@@ -298,29 +294,29 @@ void cut_items(const std::vector<int>& beg, const std::vector<int>& ord, const s
         comb_k.push_back(k);
     }
     std::stable_sort(lc.items.begin(), lc.items.end(), [](const Item& a, const Item& b) { return a.steps > b.steps; });
-    const int ng = ((int)lc.items.size() + 63) / 64;
-    lc.gsteps.resize(ng);
-    for (int g = 0; g < ng; ++g) lc.gsteps[g] = lc.items[(size_t)g * 64].steps;
 }
 
-// Groups to the first nwu waves, most steps first to the least-loaded;
-// returns the modelled cost.
-double assign_waves(LayerChoice& lc, int nw, int nwu, int nt, const std::vector<int>& comb_k)
+// Items to the lanes of the first nwu waves, most steps first to the
+// least-loaded lane; returns the modelled cost.
+double assign_lanes(LayerChoice& lc, int nw, int nwu, int nt, const std::vector<int>& comb_k)
 {
-    const int ng = (int)lc.gsteps.size();
-    std::vector<int> order(ng);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lc.gsteps[a] > lc.gsteps[b]; });
-    std::vector<int> load(nw, 0);
-    lc.wave_groups.assign(nw, {});
-    for (int g : order) {
-        const int w = (int)(std::min_element(load.begin(), load.begin() + nwu) - load.begin());
-        load[w] += lc.gsteps[g];
-        lc.wave_groups[w].push_back(g);
+    using LoadLane = std::pair<int, int>;
+    std::priority_queue<LoadLane, std::vector<LoadLane>, std::greater<LoadLane>> pq;
+    for (int l = 0; l < nwu * 64; ++l) pq.push({0, l});
+    lc.lane_items.assign((size_t)nw * 64, {});
+    std::vector<int> load((size_t)nw * 64, 0);
+    for (int i = 0; i < (int)lc.items.size(); ++i) {
+        const LoadLane t = pq.top();
+        pq.pop();
+        lc.lane_items[t.second].push_back(i);
+        load[t.second] = t.first + lc.items[i].steps;
+        pq.push({load[t.second], t.second});
     }
+    lc.wave_steps.assign(nw, 0);
     int mx = 0, tot = 0;
     for (int w = 0; w < nw; ++w) {
-        const int st = batch_steps(load[w]);
+        lc.wave_steps[w] = *std::max_element(load.begin() + (size_t)w * 64, load.begin() + (size_t)(w + 1) * 64);
+        const int st = batch_steps(lc.wave_steps[w]);
         mx = std::max(mx, st);
         tot += st;
     }
@@ -331,23 +327,6 @@ double assign_waves(LayerChoice& lc, int nw, int nwu, int nt, const std::vector<
     const int comb_max = ks.empty() ? 0 : *std::max_element(tl.begin(), tl.end());
     lc.cost = 0.5 * mx + 0.5 * (double)tot / nw + kCombSteps * comb_max + (ks.empty() ? 0.0 : 2.0);
     return lc.cost;
-}
-
-// Destination of an item's last step: the third entry word and the bits it
-// adds to the multiplier word (the inverse of the kernels' stream_dest).
-// false when the destination does not fit.
-bool dest_word(int dest_bytes, const StreamLayout& lay, bool rev, uint32_t& e2, uint32_t& wbits)
-{
-    if (lay.absolute) {
-        const uint32_t slot = (uint32_t)(dest_bytes / lay.unit);
-        e2 = slot << 16;
-        wbits = 0;
-        return slot <= 0xffffu;
-    }
-    const int rel = dest_bytes - (rev ? lay.dm_off * lay.unit : 0);
-    e2 = ((uint32_t)rel & 0xffffu) << 16;
-    wbits = (uint32_t)rel >> 16;
-    return rel >= 0 && rel % 8 == 0 && rel < (1 << 18);
 }
 
 // LDS byte addresses (from the LDS base) of the two operands of an entry word.
@@ -368,9 +347,9 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                   const StreamLayout& lay, int part_budget, StreamPlan& sp, int L, std::string& err)
 {
     const int nw = nt / 64;
-    // entry words (operand offsets) and FP32 multipliers
+    // entry words (operand offsets) and integer multipliers
     const uint32_t ms = (uint32_t)lay.m_off;
-    auto entry = [&](int e, uint32_t& x, uint32_t& y) -> bool {
+    auto entry = [&](int e, uint32_t& x, uint32_t& mw) -> bool {
         const uint32_t i1 = pair[e] & 0xffffu, i2 = pair[e] >> 16;
         uint32_t a, b;
         if (lay.absolute) {
@@ -381,11 +360,11 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
             b = rev ? i2 * (uint32_t)lay.unit : (ms + i2) * (uint32_t)lay.unit;
         }
         if (a > 0xffffu || b > 0xffffu) return false;
-        const float mf = (float)mult[e];
-        if ((double)mf != mult[e] || !(mf > 0.0f)) return false;
+        const double mv = mult[e];
+        if (!(mv >= 1.0 && mv <= (double)kStreamMultMask && mv == std::floor(mv))) return false;
         x = a | (b << 16);
-        std::memcpy(&y, &mf, 4);
-        return (y & 3u) == 0;
+        mw = (uint32_t)mv;
+        return true;
     };
     std::vector<int> dest_of_row(l1 - l0);
     for (int pos = l0; pos < l1; ++pos)
@@ -414,13 +393,13 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
         }
         cut_items(beg, ord, mult, l0, l1, C, lc, dest_of_row, lay, comb_k);
         for (int nwu = 1; nwu <= nw; ++nwu) {
-            assign_waves(lc, nw, nwu, nt, comb_k);
+            assign_lanes(lc, nw, nwu, nt, comb_k);
             if (!have || lc.cost < best.cost) { best = lc; have = true; }
         }
     }
     if (!have) {  // no entries in this layer
         cut_items(beg, ord, mult, l0, l1, 1 << 30, best, dest_of_row, lay, comb_k);
-        assign_waves(best, nw, nw, nt, comb_k);
+        assign_lanes(best, nw, nw, nt, comb_k);
     }
     sp.max_partials = std::max(sp.max_partials, best.npart);
     sp.cost += best.cost;
@@ -452,115 +431,95 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     }
 
     // streams
+    struct StepDesc { int item = -1, run = -1, cnt = 0; bool last = false; };
     const int pass = 16, nbank = 16;
     for (int w = 0; w < nw; ++w) {
         const int first_step = (int)(sp.tbl.size() / 64);
-        int wsteps = 0;
-        for (int g : best.wave_groups[w]) wsteps += best.gsteps[g];
-        const int steps = wsteps > 0 ? batch_steps(wsteps) : 0;
+        const int steps = best.wave_steps[w] > 0 ? batch_steps(best.wave_steps[w]) : 0;
         const size_t t0 = sp.tbl.size();
         sp.tbl.resize(t0 + (size_t)steps * 64, U4{0u, 0u, 0u, 0u});
         auto at = [&](int step, int l) -> U4& { return sp.tbl[t0 + (size_t)step * 64 + l]; };
-        int s0 = 0;
-        for (int g : best.wave_groups[w]) {
-            const int G = best.gsteps[g];
-            // Per lane: the runs of its item and, per step of the group, the
-            // run it draws from and how many entries (-1: zero step).
-            std::vector<std::vector<std::pair<int, int>>> runs(64);
-            std::vector<std::vector<int>> srun(64, std::vector<int>(G, -1)), scnt(64, std::vector<int>(G, 0));
-            for (int l = 0; l < 64; ++l) {
-                const size_t ii = (size_t)g * 64 + l;
-                U4& last = at(s0 + G - 1, l);
-                uint32_t e2 = 0, wbits = 0;
-                if (ii >= best.items.size()) {  // lane without an item: zero entries, no store (zero slot)
-                    last.z = 0u;
-                    last.w = kStreamEnd | (lay.absolute ? 0u : 3u);
-                    continue;
-                }
-                const Item& it = best.items[ii];
-                item_runs(ord, mult, it.e0, it.len, &runs[l]);
-                const int nr = (int)runs[l].size();
-                // last run: the final step takes its remainder (or two entries)
-                const int cl = runs[l][nr - 1].second;
-                const int fin = cl % 3 ? cl % 3 : std::min(2, cl);
-                int st = 0;
-                for (int r = 0; r < nr; ++r) {
-                    int c = runs[l][r].second - (r == nr - 1 ? fin : 0);
-                    while (c > 0) {
-                        srun[l][st] = r;
-                        scnt[l][st] = std::min(3, c);
-                        c -= scnt[l][st];
-                        ++st;
+        // Per lane: the runs of each of its items and, per step, the item,
+        // the run it draws from and how many entries.
+        std::vector<std::vector<StepDesc>> sd(64, std::vector<StepDesc>(steps));
+        std::vector<std::vector<std::vector<std::pair<int, int>>>> runs(64);
+        std::vector<std::vector<std::vector<char>>> used(64);
+        for (int l = 0; l < 64; ++l) {
+            const std::vector<int>& li = best.lane_items[(size_t)w * 64 + l];
+            runs[l].resize(li.size());
+            used[l].resize(li.size());
+            int st = 0;
+            for (size_t q = 0; q < li.size(); ++q) {
+                const Item& it = best.items[li[q]];
+                item_runs(ord, mult, it.e0, it.len, &runs[l][q]);
+                used[l][q].assign(it.len, 0);
+                for (int r = 0; r < (int)runs[l][q].size(); ++r) {
+                    for (int c = runs[l][q][r].second; c > 0;) {
+                        StepDesc& d = sd[l][st++];
+                        d.item = (int)q;
+                        d.run = r;
+                        d.cnt = std::min(3, c);
+                        c -= d.cnt;
                     }
                 }
-                if (st != it.steps - 1) {
-                    err = "internal error: contraction item steps";
+                sd[l][st - 1].last = true;
+                const uint32_t dslot = (uint32_t)(it.dest / lay.unit);
+                if (it.dest % lay.unit != 0 || dslot > kStreamDestMask) {
+                    err = "a contraction destination does not fit the stream's multiplier word";
                     return false;
                 }
-                srun[l][G - 1] = nr - 1;
-                scnt[l][G - 1] = fin;
-                if (!dest_word(it.dest, lay, rev, e2, wbits)) {
-                    err = "a contraction destination does not fit the packed stream words";
-                    return false;
-                }
-                last.z = e2;
-                last.w = kStreamEnd | wbits;
+                at(st - 1, l).w = kStreamEnd | (dslot << kStreamDestShift);
             }
-            // Each entry slot of a step is read by two LDS instructions
-            // (operand a, operand b) over all lanes.  On gfx950 the LDS
-            // serves such a read 16 lanes at a time, and two of those lanes
-            // whose slots differ but coincide modulo 16 slots (128 bytes for
-            // 8-byte slots, 256 for 16-byte slots) serialize.  Slot by slot,
-            // each lane takes the remaining entry of its step's run whose
-            // operands collide least with those already placed in its pass
-            // (ties: entry order).  This fixes the order in which a lane sums
-            // its item: deterministic, as every other order here.
-            std::vector<std::vector<char>> used(64);
-            for (int l = 0; l < 64; ++l) {
-                size_t n = 0;
-                for (auto& r : runs[l]) n += (size_t)r.second;
-                used[l].assign(n, 0);
-            }
-            for (int st = 0; st < G; ++st) {
-                for (int j = 0; j < 3; ++j) {
-                    for (int p0 = 0; p0 < 64; p0 += pass) {
-                        std::vector<std::vector<uint32_t>> ba(nbank), bb(nbank);  // distinct addresses per bank
-                        auto cost = [](const std::vector<uint32_t>& v, uint32_t addr) {
-                            return std::find(v.begin(), v.end(), addr) != v.end() ? 0 : (int)v.size();
-                        };
-                        for (int l = p0; l < p0 + pass; ++l) {
-                            if (srun[l][st] < 0 || j >= scnt[l][st]) continue;
-                            const auto rr = runs[l][srun[l][st]];
-                            const size_t base = (size_t)(rr.first - best.items[(size_t)g * 64 + l].e0);
-                            int pick = -1, best_c = 0;
-                            uint32_t pa = 0, pb = 0, px = 0, py = 0;
-                            for (int k = 0; k < rr.second; ++k) {
-                                if (used[l][base + k]) continue;
-                                uint32_t x, y, a, b;
-                                if (!entry(ord[rr.first + k], x, y)) {
-                                    err = "an operand offset or multiplier does not fit the packed contraction "
-                                          "entries";
-                                    return false;
-                                }
-                                stream_operand_bytes(x, lay, rev, a, b);
-                                const int c = cost(ba[(a / lay.unit) % nbank], a) + cost(bb[(b / lay.unit) % nbank], b);
-                                if (pick < 0 || c < best_c) { pick = k; best_c = c; pa = a; pb = b; px = x; py = y; }
-                                if (c == 0) break;
+        }
+        // Each entry slot of a step is read by two LDS instructions
+        // (operand a, operand b) over all lanes.  On gfx950 the LDS serves
+        // such a read 16 lanes at a time, and two of those lanes whose
+        // slots differ but coincide modulo 16 slots (128 bytes for 8-byte
+        // slots, 256 for 16-byte slots) serialize.  Slot by slot, each lane
+        // takes the remaining entry of its step's run whose operands collide
+        // least with those already placed in its pass (ties: entry order).
+        // This fixes the order in which a lane sums its item: deterministic,
+        // as every other order here.
+        for (int st = 0; st < steps; ++st) {
+            for (int j = 0; j < 3; ++j) {
+                for (int p0 = 0; p0 < 64; p0 += pass) {
+                    std::vector<std::vector<uint32_t>> ba(nbank), bb(nbank);  // distinct addresses per bank
+                    auto cost = [](const std::vector<uint32_t>& v, uint32_t addr) {
+                        return std::find(v.begin(), v.end(), addr) != v.end() ? 0 : (int)v.size();
+                    };
+                    for (int l = p0; l < p0 + pass; ++l) {
+                        const StepDesc& d = sd[l][st];
+                        if (d.item < 0 || j >= d.cnt) continue;
+                        const auto rr = runs[l][d.item][d.run];
+                        const Item& it = best.items[best.lane_items[(size_t)w * 64 + l][d.item]];
+                        const size_t base = (size_t)(rr.first - it.e0);
+                        std::vector<char>& u = used[l][d.item];
+                        int pick = -1, best_c = 0;
+                        uint32_t pa = 0, pb = 0, px = 0, pm = 0;
+                        for (int k = 0; k < rr.second; ++k) {
+                            if (u[base + k]) continue;
+                            uint32_t x, mw, a, b;
+                            if (!entry(ord[rr.first + k], x, mw)) {
+                                err = "an operand offset or multiplier does not fit the packed contraction entries";
+                                return false;
                             }
-                            used[l][base + pick] = 1;
-                            auto add = [](std::vector<uint32_t>& v, uint32_t addr) {
-                                if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
-                            };
-                            add(ba[(pa / lay.unit) % nbank], pa);
-                            add(bb[(pb / lay.unit) % nbank], pb);
-                            U4& q = at(s0 + st, l);
-                            (j == 0 ? q.x : (j == 1 ? q.y : q.z)) = px;
-                            q.w = st == G - 1 ? (q.w | py) : py;
+                            stream_operand_bytes(x, lay, rev, a, b);
+                            const int c = cost(ba[(a / lay.unit) % nbank], a) + cost(bb[(b / lay.unit) % nbank], b);
+                            if (pick < 0 || c < best_c) { pick = k; best_c = c; pa = a; pb = b; px = x; pm = mw; }
+                            if (c == 0) break;
                         }
+                        u[base + pick] = 1;
+                        auto add = [](std::vector<uint32_t>& v, uint32_t addr) {
+                            if (std::find(v.begin(), v.end(), addr) == v.end()) v.push_back(addr);
+                        };
+                        add(ba[(pa / lay.unit) % nbank], pa);
+                        add(bb[(pb / lay.unit) % nbank], pb);
+                        U4& q = at(st, l);
+                        (j == 0 ? q.x : (j == 1 ? q.y : q.z)) = px;
+                        q.w = (q.w & ~kStreamMultMask) | pm;
                     }
                 }
             }
-            s0 += G;
         }
         sp.wave.push_back(I4{first_step, steps, 0, 0});
     }

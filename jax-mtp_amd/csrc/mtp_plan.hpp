@@ -36,21 +36,18 @@ constexpr int kMaxR = 8;      // radial functions (mu) supported
 constexpr int kMaxNu = 12;    // tensor order supported
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
-constexpr int kStreamUnroll = 4;  // steps per prefetch batch of a contraction stream
-constexpr uint32_t kStreamEnd = 0x80000000u;  // sign bit of a step's multiplier word: the item ends with this step
-constexpr uint32_t kStreamMult = 0x7ffffffcu;  // multiplier bits of an item's last step (see stream_dest)
-
-// Destination of an item's last step, for the layouts with byte offsets: its
-// byte offset from operand b's read base (the LDS base forward, the adjoint
-// region in reverse), its low 16 bits in e2's operand-b field (operand a is
-// the zero slot, so the kernels' reads of that word stay aligned and
-// between the base and the destination; the word's product is not used),
-// and its bits 16-17 in the two low mantissa bits of the multiplier word w
-// (zero in every FP32 multiplier the planner accepts).  A lane without an
-// item has e2 = 0 and both bits set: kStreamNoDest, no store.  The layouts
-// with 16-byte slot indices store the slot in e2's operand-b field.
-constexpr uint32_t stream_dest(uint32_t e2, uint32_t w) { return (e2 >> 16) | ((w & 3u) << 16); }
-constexpr uint32_t kStreamNoDest = 3u << 16;
+constexpr int kStreamUnroll = 2;  // steps per prefetch batch of a contraction stream
+// Multiplier word of a contraction-stream step (see DevPlan): bits 0-15
+// the entries' integer multiplier (0 in padding steps), bit 31 set on the
+// last step of the lane's item, and then bits 16-30 the item's destination
+// as a slot index from the LDS base (slots of the layout's unit: 8 bytes
+// for one row per workgroup, 16 for interleaved pairs).
+constexpr uint32_t kStreamEnd = 0x80000000u;
+constexpr uint32_t kStreamMultMask = 0xffffu;
+constexpr uint32_t kStreamDestMask = 0x7fffu;
+constexpr int kStreamDestShift = 16;
+constexpr uint32_t stream_mult(uint32_t w) { return w & kStreamMultMask; }
+constexpr uint32_t stream_dest_slot(uint32_t w) { return (w >> kStreamDestShift) & kStreamDestMask; }
 
 constexpr int kLdsM = 2;      // m[] starts at this double of the atom kernels' LDS (lds[0] is a zero)
 
@@ -94,12 +91,12 @@ struct DevPlan {
     // that share a multiplier: tbl[s * 64 + lane] = (e0, e1, e2, w) with
     // e = a | b << 16, a and b LDS byte offsets of the operands (a from the
     // workgroup's LDS base; b from it in the forward pass, from the adjoint
-    // region in the reverse pass; 0 for an unused entry) and w the FP32
-    // multiplier, so that the lane's sum gains w (a0 b0 + a1 b1 + a2 b2).
-    // The last step of a group of 64 items has the sign bit of w set on
-    // every lane: e2 is then the lane's destination (stream_dest), e0 and e1
-    // its last entries (the sum gains w (a0 b0 + a1 b1)), and the lane stores
-    // its sum there and restarts from zero.  Multipliers are positive FP32 values with two zero low bits.  Rows split into chunks leave partial sums, combined per layer by
+    // region in the reverse pass; 0 for an unused entry: the zero slot) and
+    // w the multiplier word (stream_mult), so that the lane's sum gains
+    // mult (a0 b0 + a1 b1 + a2 b2).  Each lane walks its own sequence of
+    // items: on the last step of an item (kStreamEnd in w, per lane) the
+    // lane stores its sum at stream_dest_slot(w) and restarts from zero.
+    // Rows split into chunks leave partial sums, combined per layer by
     // comb[cb_beg[L] .. cb_beg[L+1]] = (moment, first partial, end partial,
     // -) in a fixed order.
     int fwd_cb_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2];
@@ -180,7 +177,7 @@ struct HostPlan {
 // entry).  Offsets are in slots from the LDS base.  With `absolute`, the
 // operand fields hold slot indices from the LDS base; otherwise they hold
 // byte offsets (a and the forward b from the LDS base, the reverse b from
-// dm_off).  Group destinations are always byte offsets from the LDS base.
+// dm_off).  Item destinations are slot indices from the LDS base.
 struct StreamLayout {
     int unit = 8;
     int m_off = kLdsM;
@@ -191,7 +188,8 @@ struct StreamLayout {
 
 // Streams for a workgroup of nt threads.  part_budget caps the partial sums
 // of a layer when the chunking allows.  Returns false when an operand
-// offset does not fit the 16-bit fields.
+// offset does not fit the 16-bit fields, a destination the 15-bit field,
+// or a multiplier is not an integer in [1, 65535].
 bool build_items(const HostPlan& p, int nt, const StreamLayout& lay, int part_budget, ItemPlan& ip,
                  std::string& err);
 

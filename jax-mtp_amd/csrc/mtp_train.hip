@@ -29,10 +29,10 @@ namespace mtp {
 
 // ---------------------------------------------------------------------------
 // Dual contraction streams.  Each step of the atom kernels' streams holds up
-// to three entries of a lane with one FP32 multiplier w (operand slots a, b
-// per entry; an item's last step flagged in the sign of w, its third word
-// then the destination slot), and every entry reads two
-// 16-byte pairs.  Forward: a = (m_a, mt_a), b = (m_b, mt_b) and the sum
+// to three entries of a lane with one integer multiplier w (operand slots
+// a, b per entry; the last step of the lane's item flagged in the
+// multiplier word, which then also holds the destination slot), and every
+// entry reads two 16-byte pairs.  Forward: a = (m_a, mt_a), b = (m_b, mt_b) and the sum
 // gains w (m_a m_b, mt_a m_b + m_a mt_b).  Reverse: a = (m_o, mt_o) of the
 // operand, b = (dm_t, dmt_t) of the target it feeds, and the sum gains
 // w (m_o dm_t + mt_o dmt_t, m_o dmt_t).
@@ -66,23 +66,18 @@ __device__ __forceinline__ void dual_steps(const uint4* q, char* lb, double2& ac
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        // an item's last step has two entries, its third word is the
-        // destination slot (the empty asm keeps the third entry's loads
-        // ahead of the branch)
+        // three entries with one integer multiplier; on the last step of
+        // the lane's item (per lane) the sums go to the item's destination
+        // slot and restart from zero
         double2 t = dual_term<REV>(a0[u], b0[u], make_double2(0.0, 0.0));
         t = dual_term<REV>(a1[u], b1[u], t);
-        asm volatile("" : : "v"(a2[u].x), "v"(a2[u].y), "v"(b2[u].x), "v"(b2[u].y));
+        t = dual_term<REV>(a2[u], b2[u], t);
         const uint32_t w = q[u].w;
-        if (!(__builtin_amdgcn_readfirstlane(w) & kStreamEnd)) {
-            const double m = (double)__uint_as_float(w);
-            t = dual_term<REV>(a2[u], b2[u], t);
-            acc.x = fma(m, t.x, acc.x);
-            acc.y = fma(m, t.y, acc.y);
-        } else {  // item end (wave-uniform)
-            const double m = (double)__uint_as_float(w & kStreamMult);
-            acc.x = fma(m, t.x, acc.x);
-            acc.y = fma(m, t.y, acc.y);
-            *reinterpret_cast<double2*>(lb + ((q[u].z >> 16) << 4)) = acc;
+        const double m = (double)stream_mult(w);
+        acc.x = fma(m, t.x, acc.x);
+        acc.y = fma(m, t.y, acc.y);
+        if (w & kStreamEnd) {
+            *reinterpret_cast<double2*>(lb + ((size_t)stream_dest_slot(w) << 4)) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }

@@ -2,10 +2,10 @@
 //
 // Builds the plan of an .mtp file for a workgroup size, executes the entry
 // streams of every layer on the CPU exactly as the atom kernel walks them
-// (LDS as a byte-addressed double array, one accumulator per lane, the
-// group end read from lane 0 and required to agree on all lanes, partial
-// sums combined in their fixed order; up to three entries per step sharing a
-// multiplier, item ends flagged in its sign), and compares the moments and their
+// (LDS as a byte-addressed double array, one accumulator per lane, each
+// lane storing its sum at its own item ends, partial sums combined in their
+// fixed order; up to three entries per step sharing an integer multiplier,
+// item ends and destinations in the multiplier word), and compares the moments and their
 // adjoints with a direct evaluation of alpha_index_times in file order:
 //   forward  m[i3] += mult * m[i1] * m[i2]
 //   reverse  dE/dm from E = sum_s xi_s m[mapping[s]], entries in reverse order.
@@ -122,35 +122,24 @@ int main(int argc, char** argv)
             std::vector<double> acc(64 * W, 0.0);
             for (int s = wv.x; s < wv.x + wv.y; ++s) {
                 const U4* q = &sp.tbl[(size_t)s * 64];
-                const bool end = (q[0].w & kStreamEnd) != 0;  // the kernel reads lane 0's word
                 for (int l = 0; l < 64; ++l) {
-                    if (((q[l].w & kStreamEnd) != 0) != end) return fail("item end differs between lanes");
+                    const bool end = (q[l].w & kStreamEnd) != 0;  // per lane
                     const uint32_t e[3] = {q[l].x, q[l].y, q[l].z};
+                    const uint32_t mult = stream_mult(q[l].w);
+                    if (mult == 0 && (e[0] | e[1] | e[2]) != 0) return fail("entries in a step without a multiplier");
+                    if (!end && (q[l].w & ~kStreamMultMask) != 0) return fail("destination bits in a step that does not end an item");
                     // the kernel issues both operand reads of all three entries
                     size_t sa[3], sb[3];
                     for (int j = 0; j < 3; ++j) sa[j] = slot(e[j] & 0xffffu, false), sb[j] = slot(e[j] >> 16, rev);
-                    if (end && sa[2] != 0) return fail("the third entry of an item's last step is not the zero slot");
-                    const uint32_t mw = end ? (q[l].w & kStreamMult) : q[l].w;
-                    float mf;
-                    std::memcpy(&mf, &mw, 4);
                     for (int r = 0; r < W; ++r) {
                         double t = lds[sa[0] * W + r] * lds[sb[0] * W + r];
                         t = std::fma(lds[sa[1] * W + r], lds[sb[1] * W + r], t);
-                        if (!end) t = std::fma(lds[sa[2] * W + r], lds[sb[2] * W + r], t);
-                        acc[l * W + r] = std::fma((double)mf, t, acc[l * W + r]);
+                        t = std::fma(lds[sa[2] * W + r], lds[sb[2] * W + r], t);
+                        acc[l * W + r] = std::fma((double)mult, t, acc[l * W + r]);
                     }
                     if (end) {
-                        const uint32_t rel = stream_dest(e[2], q[l].w);
-                        if (!lay.absolute && rel >= kStreamNoDest) {
-                            if (e[2] != 0 || rel != kStreamNoDest) return fail("malformed item end without a store");
-                            for (int r = 0; r < W; ++r) {
-                                if (acc[l * W + r] != 0.0) return fail("a lane without an item summed non-zero");
-                                acc[l * W + r] = 0.0;
-                            }
-                            continue;
-                        }
-                        const size_t d = lay.absolute ? dest_slot((e[2] >> 16) * (uint32_t)lay.unit)
-                                                      : dest_slot(rel + (rev ? (uint32_t)lay.dm_off * 8u : 0u));
+                        const size_t d = dest_slot(stream_dest_slot(q[l].w) * (uint32_t)lay.unit);
+                        if (d == 0) return fail("an item stores into the zero slot");
                         for (int r = 0; r < W; ++r) {
                             lds[d * W + r] = acc[l * W + r];
                             acc[l * W + r] = 0.0;

@@ -2,9 +2,9 @@
 
 ``stream_plan_check.cpp`` builds the plan (jax-mtp_amd/csrc/mtp_plan.cpp)
 for a workgroup size and executes every layer's entry streams on the CPU
-the way the kernel walks them: lane accumulators, group ends read from
-lane 0 (and required to agree on all lanes), sums stored at the encoded LDS
-offsets, partial sums combined in their fixed order.  The moments and
+the way the kernel walks them: lane accumulators, each lane storing its
+sum at its own item ends (destination slots encoded in the multiplier
+word), partial sums combined in their fixed order.  The moments and
 their adjoints must equal a direct evaluation of alpha_index_times (the
 reference's conversion.py:95-102 recurrence and its reverse) to 1e-12.
 """

@@ -36,7 +36,7 @@ constexpr int kMaxR = 8;      // radial functions (mu) supported
 constexpr int kMaxNu = 12;    // tensor order supported
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
-constexpr int kStreamUnroll = 2;  // steps per prefetch batch of a contraction stream
+constexpr int kStreamUnroll = 3;  // steps per prefetch batch of a contraction stream
 // Multiplier word of a contraction-stream step (see DevPlan): bits 0-15
 // the entries' integer multiplier (0 in padding steps), bit 31 set on the
 // last step of the lane's item, and then bits 16-30 the item's destination

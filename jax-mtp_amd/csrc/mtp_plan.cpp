@@ -432,7 +432,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
 
     // streams
     struct StepDesc { int item = -1, run = -1, cnt = 0; bool last = false; };
-    const int pass = 16, nbank = 16;
+    const int pass = 16, nbank = 256 / lay.unit;
     for (int w = 0; w < nw; ++w) {
         const int first_step = (int)(sp.tbl.size() / 64);
         const int steps = best.wave_steps[w] > 0 ? batch_steps(best.wave_steps[w]) : 0;
@@ -473,9 +473,11 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
         }
         // Each entry slot of a step is read by two LDS instructions
         // (operand a, operand b) over all lanes.  On gfx950 the LDS serves
-        // such a read 16 lanes at a time, and two of those lanes whose
-        // slots differ but coincide modulo 16 slots (128 bytes for 8-byte
-        // slots, 256 for 16-byte slots) serialize.  Slot by slot, each lane
+        // such a read 16 lanes at a time from 64 banks of 4 bytes, and two
+        // of those lanes whose addresses differ but coincide modulo 256
+        // bytes (32 slots of 8 bytes, 16 of 16 bytes) serialize (measured:
+        // a 64-bit read with lanes at a stride of 16 bytes runs at full
+        // rate, at 32 bytes at half rate).  Slot by slot, each lane
         // takes the remaining entry of its step's run whose operands collide
         // least with those already placed in its pass (ties: entry order).
         // This fixes the order in which a lane sums its item: deterministic,

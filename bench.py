@@ -388,7 +388,7 @@ def main():
                               info["basic_count"], info["times_count"], info["scalar_count"])
     achieved = flops / (r["atom_kernel_ms"] * 1e-3) / 1e12
     value = w.natoms / (r["ms_per_step"] * 1e-3)
-    kname = {0: "atom_kernel<double>", 1: "atom_kernel_v2<double, 256>", 2: "atom_kernel_v2<double, 512>",
+    kname = {0: "atom_kernel<double, false>", 1: "atom_kernel_v2<double, 256>", 2: "atom_kernel_v2<double, 512>",
              3: "atom_kernel_v2<double, 64>", 4: "atom_kernel_pair<double, 1024>",
              5: "atom_kernel_pair<double, 512>"}[best]
     traffic = traffic_bytes(w.name, pot.VARIANTS[best])

@@ -444,13 +444,21 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                 const AtomLds2 L0 = atom_lds2_layout(f.alpha_moments_count, p.n_live, f.B(), max_nbr, f.radial_funcs_count,
                                                      p.nu_max, dv.n_parts, nt, 0, 0);
                 const int half_cu = (int)(lds_max / 2 / sizeof(double));
-                part_budget = std::max(64, half_cu - L0.total_doubles);
+                const int full_cu = (int)(lds_max / sizeof(double));
+                // (a layout beyond half the CU's LDS runs one workgroup per CU: its partials get the rest)
+                part_budget = std::max(64, (L0.total_doubles < half_cu ? half_cu : full_cu) - L0.total_doubles);
                 lay.dm_off = L0.dm;
                 lay.part_off = L0.part;
             }
             ItemPlan ip;
             std::string ierr;
-            if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) continue;  // variant unavailable
+            if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) {
+                // operand byte offsets beyond 16 bits (level 26: 8,238 live moments): 8-byte slot indices
+                if (paired) continue;  // variant unavailable
+                lay.absolute = true;
+                if (!build_items(*pv, nt, lay, part_budget, ip, ierr)) continue;
+            }
+            dv.abs_slots = lay.absolute && !paired ? 1 : 0;
             upload_streams(c, ip, dv, e);
             dv.k_layout = max_nbr;
             if (!paired && nt <= 256) {
@@ -492,7 +500,9 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
                     StreamLayout layc;
                     layc.dm_off = Lc.dm;
                     layc.part_off = Lc.part;
-                    const int budc = std::max(64, (int)(lds_max / 2 / sizeof(double)) - Lc.total_doubles);
+                    layc.absolute = lay.absolute;
+                    const int half_cu = (int)(lds_max / 2 / sizeof(double));
+                    const int budc = std::max(64, (Lc.total_doubles < half_cu ? half_cu : 2 * half_cu) - Lc.total_doubles);
                     ItemPlan ipc;
                     if (build_items(*pv, nt, layc, budc, ipc, ierr)) {
                         DevPlan dvc = dv;

@@ -324,19 +324,32 @@ size_t atom_lds2_bytes(int amc, int n_live, int B, int nslots, int R, int nu_max
 
 hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes);
 
-template <bool REV>
+// Operand addresses of an entry word x = a | b << 16: byte offsets from
+// the bases ab and bb, or (ABS) 8-byte slot indices from the LDS base lb.
+template <bool ABS>
+__device__ __forceinline__ const double* stream_op_a(uint32_t x, const char* ab, const char* lb)
+{
+    return reinterpret_cast<const double*>(ABS ? lb + ((x & 0xffffu) << 3) : ab + (x & 0xffffu));
+}
+template <bool ABS>
+__device__ __forceinline__ const double* stream_op_b(uint32_t x, const char* bb, const char* lb)
+{
+    return reinterpret_cast<const double*>(ABS ? lb + ((x >> 16) << 3) : bb + (x >> 16));
+}
+
+template <bool ABS>
 __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, const char* bb, char* lb, double& acc)
 {
     constexpr int U = kStreamUnroll;
     double a0[U], b0[U], a1[U], b1[U], a2[U], b2[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        a0[u] = *reinterpret_cast<const double*>(ab + (q[u].x & 0xffffu));
-        b0[u] = *reinterpret_cast<const double*>(bb + (q[u].x >> 16));
-        a1[u] = *reinterpret_cast<const double*>(ab + (q[u].y & 0xffffu));
-        b1[u] = *reinterpret_cast<const double*>(bb + (q[u].y >> 16));
-        a2[u] = *reinterpret_cast<const double*>(ab + (q[u].z & 0xffffu));
-        b2[u] = *reinterpret_cast<const double*>(bb + (q[u].z >> 16));
+        a0[u] = *stream_op_a<ABS>(q[u].x, ab, lb);
+        b0[u] = *stream_op_b<ABS>(q[u].x, bb, lb);
+        a1[u] = *stream_op_a<ABS>(q[u].y, ab, lb);
+        b1[u] = *stream_op_b<ABS>(q[u].y, bb, lb);
+        a2[u] = *stream_op_a<ABS>(q[u].z, ab, lb);
+        b2[u] = *stream_op_b<ABS>(q[u].z, bb, lb);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -355,12 +368,39 @@ __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, con
     }
 }
 
+// A wave's entry stream of one layer: nsteps (a multiple of kStreamUnroll)
+// steps from tb (this lane's column), table loads one batch ahead.
+template <bool ABS>
+__device__ __forceinline__ void stream_walk(const uint4* tb, int nsteps, const char* bb, char* lb)
+{
+    constexpr int U = kStreamUnroll;
+    double acc = 0.0;
+    uint4 qa[U], qb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)u * kWave];
+    for (int s0 = 0;;) {
+        if (s0 + U < nsteps) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) qb[u] = tb[(size_t)(s0 + U + u) * kWave];
+        }
+        stream_steps<ABS>(qa, lb, bb, lb, acc);
+        s0 += U;
+        if (s0 >= nsteps) break;
+        if (s0 + U < nsteps) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)(s0 + U + u) * kWave];
+        }
+        stream_steps<ABS>(qb, lb, bb, lb, acc);
+        s0 += U;
+        if (s0 >= nsteps) break;
+    }
+}
+
 template <int NT, bool REV>
 __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, double* lds, double* dm, double* part,
                                                   int tid, const double* xi)
 {
     constexpr int NW = NT / kWave;
-    constexpr int U = kStreamUnroll;
     const int lane = tid & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
     const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
@@ -369,26 +409,8 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
         const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
         char* lb = reinterpret_cast<char*>(lds);
         const char* bb = REV ? reinterpret_cast<const char*>(dm) : lb;
-        double acc = 0.0;
-        uint4 qa[U], qb[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)u * kWave];
-        for (int s0 = 0;;) {
-            if (s0 + U < nsteps) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) qb[u] = tb[(size_t)(s0 + U + u) * kWave];
-            }
-            stream_steps<REV>(qa, lb, bb, lb, acc);
-            s0 += U;
-            if (s0 >= nsteps) break;
-            if (s0 + U < nsteps) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) qa[u] = tb[(size_t)(s0 + U + u) * kWave];
-            }
-            stream_steps<REV>(qb, lb, bb, lb, acc);
-            s0 += U;
-            if (s0 >= nsteps) break;
-        }
+        if (P.abs_slots) stream_walk<true>(tb, nsteps, bb, lb);
+        else stream_walk<false>(tb, nsteps, bb, lb);
     }
     __syncthreads();
     const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];

@@ -101,6 +101,9 @@ struct DevPlan {
     // -) in a fixed order.
     int fwd_cb_beg[kMaxDepth + 2], rev_cb_beg[kMaxDepth + 2];
     int rev_bo_beg[kMaxDepth + 2], rev_xa_beg[kMaxDepth + 2];
+    int abs_slots;                   // 1: operand fields are 8-byte slot indices from the LDS base (a
+                                     // and b, both directions; StreamLayout absolute), for layouts whose
+                                     // byte offsets do not fit 16 bits; 0: byte offsets as above
     const I4* fwd_wave;
     const U4* fwd_tbl;
     const I4* fwd_comb;

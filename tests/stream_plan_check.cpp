@@ -13,8 +13,9 @@
 // 1e-12 and every structural check held.
 //
 // Build: g++ -O2 -std=c++17 -I<csrc> stream_plan_check.cpp mtp_plan.cpp mtp_format.cpp
-// Usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair]
-// (pair: two rows interleaved in 16-byte slots, operand fields as slot indices)
+// Usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair|abs]
+// (pair: two rows interleaved in 16-byte slots, operand fields as slot indices;
+// abs: one row, operand fields as 8-byte slot indices from the LDS base)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -37,10 +38,11 @@ static int fail(const char* msg)
 
 int main(int argc, char** argv)
 {
-    if (argc < 4) return fail("usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair]");
+    if (argc < 4) return fail("usage: stream_plan_check <file.mtp> <threads> <max_nbr> [seed] [pair|abs]");
     const int nt = std::atoi(argv[2]), K = std::atoi(argv[3]);
     const unsigned seed = argc > 4 ? (unsigned)std::atoi(argv[4]) : 1u;
     const bool paired = argc > 5 && std::string(argv[5]) == "pair";
+    const bool abs_slots = argc > 5 && std::string(argv[5]) == "abs";
     const int W = paired ? 2 : 1;  // rows sharing the streams (values per LDS slot)
     MtpFile f;
     std::string err;
@@ -66,6 +68,7 @@ int main(int argc, char** argv)
         lay.m_off = 1;
         lay.absolute = true;
     }
+    if (abs_slots) lay.absolute = true;
     lay.dm_off = lay.m_off + mlen;
     lay.part_off = lay.dm_off + dlen;
     ItemPlan ip;

@@ -378,9 +378,10 @@ def test_capacity_errors(jm):
 
 def test_level26_and_level28(jm, O):
     """Levels 26 and 28 (alpha tables of the reference's untrained_mtps/26.mtp, 28.mtp; no JSON trees exist
-    for them, so the oracle is the compact form).  Level 26's contraction tables overflow the streams'
-    16-bit operand offsets, so it runs on the per-target kernel (its moments and adjoints, 9,396 of each,
-    just fit one workgroup's LDS at K = 80).  Level 28's (17,366 of each, 290 KB) do not: the per-target
+    for them, so the oracle is the compact form).  Level 26's moments take more than 64 KB of LDS, so its
+    contraction streams address operands by 8-byte slot index; the stream kernels (one workgroup per CU at
+    512 threads) are the default, and every single-row variant, the per-target kernel included, is
+    checked.  Level 28's moments and adjoints (17,366 of each, 290 KB) exceed the LDS: the per-target
     kernel keeps them in a global slice per resident workgroup, and a fixed grid of workgroups walks the
     rows.  Checked on a small cluster and on a 1,372-atom FCC cell, more rows than the grid, so that
     workgroups reuse their slices."""
@@ -393,12 +394,14 @@ def test_level26_and_level28(jm, O):
     w = jm.configs.small_cluster(10, 2, 12, seed=26, cell=7.0, cutoff=5.0, max_nbr=80)
     w.params = p
     nb = w.neighbors()
-    print(f"[case] level 26, {w.natoms} atoms", flush=True)
+    ref = _run_oracle(O, w, nb)
     pot = jm.MTPPotential(p, max_atoms=w.natoms, max_nbr=80)
-    assert pot.variant == 0 and pot.info()["level"] == 26
-    gpu = _run_potential(pot, w, nb)
+    assert pot.variant == 2 and pot.info()["level"] == 26
+    for variant in (2, 1, 3, 0):
+        print(f"[case] level 26, {w.natoms} atoms, variant {variant}", flush=True)
+        pot.set_variant(variant)
+        _check(_run_potential(pot, w, nb), ref)
     pot.close()
-    _check(gpu, _run_oracle(O, w, nb))
 
     q = jm.mtp_io.untrained(28)
     q.species_count, q.radial_basis_size, q.min_dist, q.max_dist = 2, 8, 1.5, 5.0

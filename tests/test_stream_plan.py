@@ -38,3 +38,12 @@ def test_streams_reproduce_the_contraction(checker, level, threads, layout):
         r = subprocess.run([str(checker), str(POT / f"untrained_{level:02d}.mtp"), str(threads), str(max_nbr),
                             str(level), layout], capture_output=True, text=True)
         assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("level,threads", [(26, 512), (26, 256), (24, 512)])
+def test_streams_with_slot_indices(checker, level, threads):
+    """Operand fields as 8-byte slot indices from the LDS base (the layout of levels whose moments take
+    more than 64 KB of LDS, level 26; checked at level 24 too, where byte offsets would also fit)."""
+    r = subprocess.run([str(checker), str(POT / f"untrained_{level:02d}.mtp"), str(threads), "80", str(level), "abs"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr

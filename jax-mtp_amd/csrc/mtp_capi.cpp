@@ -384,6 +384,13 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     d.Sp = f.species_count; d.R = f.radial_funcs_count; d.N = f.radial_basis_size;
     d.B = f.B(); d.T = f.T(); d.amc = f.alpha_moments_count; d.S = f.S(); d.nu_max = p.nu_max;
     d.n_live = p.n_live;
+    {
+        std::vector<char> is_target(p.file.alpha_moments_count, 0);
+        for (int x : p.fwd_tgt) is_target[x] = 1;
+        d.zero_untargeted = 0;
+        for (int x = p.file.B(); x < p.file.alpha_moments_count; ++x)
+            if (!is_target[x]) d.zero_untargeted = 1;
+    }
     d.scaling = f.scaling; d.min_dist = f.min_dist; d.max_dist = f.max_dist;
     d.radial_coeffs = c->d_radial; d.species_coeffs = c->d_species; d.moment_coeffs = c->d_moment;
     d.mapping = mapping; d.xi_of = c->d_xi;

@@ -667,7 +667,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* vy = lds + L.vy;
     double* vz = lds + L.vz;
     double* rr = lds + L.r;
-    double* red = lds + L.red;
+    double* epart = lds + L.epart;  // [NW] per-wave sums of the site energy
     int* ints = reinterpret_cast<int*>(lds + L.ints);
     int* nbr_t = ints;  // [K] compact index -> neighbour species (the radial basis's only input from the slot)
     int* cidx = ints + K;
@@ -815,8 +815,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         __syncthreads();
         basic_moments2<NW>(P, tid / kWave, tid & (kWave - 1), nvp, pw, Kp2, np1, f, Rp, m);
         // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
-        // region, free once the power table and f are consumed)
-        for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
+        // region, free once the power table and f are consumed); in every
+        // level the reference ships, each non-basic moment is a target
+        if (P.zero_untargeted)
+            for (int x = P.B + tid; x < P.amc; x += NT) m[x] = 0.0;
         __syncthreads();
 
         // ---- C: contraction layers (depth 1 .. n_depth)
@@ -833,16 +835,25 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             }
             if (A.mom_out)
                 for (int x = tid; x < P.n_live; x += NT) A.mom_out[(size_t)row * P.n_live + x] = m[x];
-            red[tid] = acc;
+            // per-wave sums now, their total at the end of the row (in wave order)
+            acc = wave_sum(acc);
+            if ((tid & (kWave - 1)) == 0) epart[tid / kWave] = acc;
+            // the reverse pass overwrites the never-operand moments read above
             __syncthreads();
-            for (int w = NT / 2; w > 0; w >>= 1) {
-                if (tid < w) red[tid] += red[tid + w];
-                __syncthreads();
-            }
-            if (tid == 0 && A.Ei) A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + red[0];
         }
     }
-    if (A.fwd_only) return;
+    const bool energy = A.Ei && !(NT <= 256 && A.mom_in);
+    auto store_energy = [&]() {
+        if (energy && tid == 0) {
+            double e = 0.0;
+            for (int w = 0; w < NW; ++w) e += epart[w];
+            A.Ei[row] = P.species_coeffs[wrap_species(ti, P.Sp)] + e;
+        }
+    };
+    if (A.fwd_only) {
+        store_energy();
+        return;
+    }
 
     // ---- E: reverse layers, deepest first.  dE/dm starts from xi_of (sum of
     // the coefficients mapped to each moment), or from this row's own
@@ -925,6 +936,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             if (lane == 0) A.Wi[(size_t)row * 9 + c] = s;
         }
     }
+    store_energy();
 }
 
 // One workgroup per row.  512-thread workgroups are compiled for two per CU

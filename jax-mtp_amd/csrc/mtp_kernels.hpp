@@ -60,7 +60,7 @@ struct AtomLds {
 };
 
 struct AtomLds2 {
-    int m, dm, part, f, gp, gf, gfp, vx, vy, vz, r, red, xl, ints, coef, total_doubles;
+    int m, dm, part, f, gp, gf, gfp, vx, vy, vz, r, red, xl, ints, coef, epart, total_doubles;
 };
 
 // LDS of the balanced-item kernels (doubles).  Regions are reused across
@@ -73,13 +73,14 @@ struct AtomLds2 {
 //               powers fall in different LDS banks), then f[k * Rp + mu]
 //               (Rp = f_stride(R)); C/D: the values of the never-operand
 //               moments (m[x] for x >= n_live aliases dm[x - n_live], dead
-//               after D) and, at the end, the energy reduction; E/F:
+//               after D); E/F:
 //               adjoints, of which F reads only the basic ones [0, B): from
 //               max(B, 3K) on, F keeps the per-part gradients and the
 //               recomputed f, f'; after F, dm[0, 3K) holds the slot
 //               gradients
 //   part, vx, vy, vz, r, xl (per-row coefficients), ints, coef (the radial
-//               coefficients of the row's species, when staged)
+//               coefficients of the row's species, when staged), epart (the
+//               per-wave sums of the site energy, read at the row's end)
 // Without tables (a per-row pass over cached moments, which skips phases
 // A's radial values and B) the power table and f are left out.
 // Row stride (doubles) of the single-row kernels' power table: slots in
@@ -118,7 +119,8 @@ __host__ __device__ inline AtomLds2 atom_lds2_layout(int amc, int n_live, int B,
     L.xl = L.r + K;
     L.ints = L.xl + xi_len;
     L.coef = L.ints + (2 * K + nt / kWave + 2) / 2 + 1;
-    L.total_doubles = L.coef + coef_len;
+    L.epart = L.coef + coef_len;
+    L.total_doubles = L.epart + nt / kWave;
     return L;
 }
 

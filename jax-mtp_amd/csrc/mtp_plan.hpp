@@ -55,6 +55,7 @@ constexpr int kLdsM = 2;      // m[] starts at this double of the atom kernels' 
 struct DevPlan {
     int Sp, R, N, B, T, amc, S, nu_max;
     int n_live;                      // moments [0, n_live) are basic or read as operands
+    int zero_untargeted;             // 1: some non-basic moment is no entry's target (it stays 0)
     double scaling, min_dist, max_dist;
     const double* radial_coeffs;   // [Sp][Sp][R][N]
     const double* species_coeffs;  // [Sp]

@@ -390,7 +390,7 @@ def main():
     value = w.natoms / (r["ms_per_step"] * 1e-3)
     kname = {0: "atom_kernel<double, false>", 1: "atom_kernel_v2<double, 256>", 2: "atom_kernel_v2<double, 512>",
              3: "atom_kernel_v2<double, 64>", 4: "atom_kernel_pair<double, 1024>",
-             5: "atom_kernel_pair<double, 512>"}[best]
+             5: "atom_kernel_pair<double, 512>", 6: "atom_kernel_v2<double, 128>"}[best]
     traffic = traffic_bytes(w.name, pot.VARIANTS[best])
     cnt = profile_values(f"atom_kernel_{w.name}_counters.txt", kname)
     kt = r["atom_kernel_ms"] * 1e-3

@@ -152,14 +152,15 @@ int mtp_sync(mtp_ctx *ctx, void *stream);
  * fusion): 0 = one thread per target moment, 256 threads per atom;
  * 1 / 2 / 3 = contraction streams with 256 / 512 / 64 threads per atom;
  * 4 / 5 = two atoms per workgroup sharing the contraction streams, 1024 /
- * 512 threads (plain force evaluation only).  mtp_open picks variant 3 for
- * small contraction tables (<= 400 entries, levels up to 14) and otherwise
- * the widest single-row variant whose LDS fits (env MTP_KERNEL_VARIANT
- * overrides).  mtp_autotune times every available variant on the given
+ * 512 threads (plain force evaluation only); 6 = contraction streams with
+ * 128 threads per atom.  mtp_open picks variant 3 for small contraction
+ * tables (<= 400 entries, levels up to 14), variant 6 up to 8,000 entries
+ * (levels up to 20) and otherwise the widest single-row variant whose LDS
+ * fits (env MTP_KERNEL_VARIANT overrides).  mtp_autotune times every available variant on the given
  * device arrays (reps launches each, after one warm-up), selects the
  * fastest for ctx and reports the per-variant ms in
  * ms_per_variant[MTP_NUM_VARIANTS] (-1 = unavailable). */
-#define MTP_NUM_VARIANTS 6
+#define MTP_NUM_VARIANTS 7
 int mtp_set_variant(mtp_ctx *ctx, int variant);
 int mtp_get_variant(const mtp_ctx *ctx, int *variant, long long *lds_bytes);
 int mtp_autotune(mtp_ctx *ctx, int nrows, int nslots, const int *d_itypes, const int *d_js, const double *d_rijs,

@@ -414,7 +414,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     c->vdev[kVariantV1] = d;
     c->vlds[kVariantV1] = c->lds_bytes;
     {
-        for (int v : {kVariantV2x256, kVariantV2x512, kVariantV2x64, kVariantPair1024, kVariantPair512}) {
+        for (int v : {kVariantV2x256, kVariantV2x512, kVariantV2x64, kVariantPair1024, kVariantPair512, kVariantV2x128}) {
             const int nt = variant_threads(v);
             const bool paired = variant_paired(v);
             HostPlan pp;
@@ -534,9 +534,11 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             delete c;
             return MTP_ERR_HIP;
         }
-        // default: one wavefront per atom for small contraction tables (levels <= 14), else the widest balanced
-        // variant that fits; mtp_autotune measures instead, MTP_KERNEL_VARIANT overrides
+        // default: one wavefront per atom for small contraction tables (levels <= 14), two up to level 20,
+        // else the widest balanced variant that fits; mtp_autotune measures instead, MTP_KERNEL_VARIANT
+        // overrides
         c->variant = c->vlds[kVariantV2x512] ? kVariantV2x512 : (c->vlds[kVariantV2x256] ? kVariantV2x256 : kVariantV1);
+        if (f.T() <= 8000 && c->vlds[kVariantV2x128]) c->variant = kVariantV2x128;
         if (f.T() <= 400 && c->vlds[kVariantV2x64]) c->variant = kVariantV2x64;
         if (const char* ev = std::getenv("MTP_KERNEL_VARIANT")) {
             const int v = std::atoi(ev);

@@ -1539,6 +1539,7 @@ int atom_kernel_blocks_per_cu(int variant, size_t bytes)
         case kVariantV2x256: fn = (const void*)atom_kernel_v2<double, 256>; nt = 256; break;
         case kVariantV2x512: fn = (const void*)atom_kernel_v2<double, 512>; nt = 512; break;
         case kVariantV2x64: fn = (const void*)atom_kernel_v2<double, 64>; nt = 64; break;
+        case kVariantV2x128: fn = (const void*)atom_kernel_v2<double, 128>; nt = 128; break;
         default: return -1;
     }
     int n = 0;
@@ -1590,6 +1591,16 @@ hipError_t set_atom_kernel_lds_exact(int variant, size_t bytes)
             return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 64>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
         }
+        case kVariantV2x128: {
+            hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_v2<double, 128>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void*)atom_kernel_v2_list<128>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)atom_kernel_v2<float, 128>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
         case kVariantPair1024: {
             hipError_t e = hipFuncSetAttribute((const void*)atom_kernel_pair<double, 1024>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, b);
@@ -1633,6 +1644,8 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
             hipLaunchKernelGGL(atom_kernel_v2_list<512>, dim3(grid), dim3(512), lds_bytes, stream, P, A);
         else if (variant == kVariantV2x64)
             hipLaunchKernelGGL(atom_kernel_v2_list<64>, dim3(grid), dim3(64), lds_bytes, stream, P, A);
+        else if (variant == kVariantV2x128)
+            hipLaunchKernelGGL(atom_kernel_v2_list<128>, dim3(grid), dim3(128), lds_bytes, stream, P, A);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();
@@ -1661,6 +1674,10 @@ hipError_t launch_atoms(const DevPlan& P, const AtomArgs& A, size_t lds_bytes, b
         case kVariantV2x64:
             if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 64>), dim3(grid), dim3(64), lds_bytes, stream, P, A);
             else hipLaunchKernelGGL((atom_kernel_v2<double, 64>), dim3(grid), dim3(64), lds_bytes, stream, P, A);
+            break;
+        case kVariantV2x128:
+            if (radial_fp32) hipLaunchKernelGGL((atom_kernel_v2<float, 128>), dim3(grid), dim3(128), lds_bytes, stream, P, A);
+            else hipLaunchKernelGGL((atom_kernel_v2<double, 128>), dim3(grid), dim3(128), lds_bytes, stream, P, A);
             break;
         default:
             if (modes) return hipErrorInvalidValue;  // the per-target kernel has no optional modes

@@ -163,8 +163,9 @@ __host__ __device__ inline AtomLdsPair atom_lds_pair_layout(int amc, int n_live,
 
 // Atom-kernel variants (chosen per potential by the launch autotuner):
 // v1 = 256 threads, one thread per target moment; v2 = contraction streams
-// with 256, 512 or 64 threads per atom (one wavefront: for low levels, where
-// a wide workgroup per atom leaves the CU waiting on one atom's barriers);
+// with 256, 512, 64 or 128 threads per atom (one or two wavefronts: for low
+// and mid levels, where a wide workgroup per atom leaves the CU waiting on
+// one atom's barriers);
 // paired = two atoms per workgroup sharing the contraction streams (plain
 // force evaluation only: no per-row coefficients, basis output or row list).
 constexpr int kVariantV1 = 0;
@@ -173,16 +174,19 @@ constexpr int kVariantV2x512 = 2;
 constexpr int kVariantV2x64 = 3;
 constexpr int kVariantPair1024 = 4;  // two rows per 1024-thread workgroup (512 threads per row)
 constexpr int kVariantPair512 = 5;   // two rows per 512-thread workgroup
-constexpr int kNumVariants = 6;
+constexpr int kVariantV2x128 = 6;    // streams, 128 threads (two wavefronts per atom, mid levels)
+constexpr int kNumVariants = 7;
 constexpr bool variant_paired(int v) { return v == kVariantPair1024 || v == kVariantPair512; }
 constexpr int variant_threads(int v)
 {
-    return v == kVariantPair1024 ? 1024 : ((v == kVariantV2x512 || v == kVariantPair512) ? 512 : (v == kVariantV2x64 ? 64 : 256));
+    return v == kVariantPair1024 ? 1024
+           : ((v == kVariantV2x512 || v == kVariantPair512) ? 512
+                                                             : (v == kVariantV2x64 ? 64 : (v == kVariantV2x128 ? 128 : 256)));
 }
 // gradient parts (thread groups over the basic-moment blocks) per variant
 constexpr int variant_parts(int v)
 {
-    return (v == kVariantV2x512 || v == kVariantPair1024) ? 8 : (v == kVariantV2x64 ? 1 : 4);
+    return (v == kVariantV2x512 || v == kVariantPair1024) ? 8 : (v == kVariantV2x64 ? 1 : (v == kVariantV2x128 ? 2 : 4));
 }
 
 // global_moments: the moment and adjoint regions in a global slice of

@@ -74,7 +74,8 @@ class MTPPotential:
         capi.check(capi.lib().mtp_set_precision(self.ctx, int(self.radial_fp32)), self.ctx)
 
     VARIANTS = {0: "per-target, 256 threads", 1: "streams, 256 threads", 2: "streams, 512 threads",
-                3: "streams, 64 threads", 4: "paired rows, 1024 threads", 5: "paired rows, 512 threads"}
+                3: "streams, 64 threads", 4: "paired rows, 1024 threads", 5: "paired rows, 512 threads",
+                6: "streams, 128 threads"}
 
     @property
     def variant(self) -> int:

@@ -63,7 +63,7 @@ def _check(gpu, ref, tol=TOL):
     assert _rel(gpu["s6"], ref["s6"]) <= tol
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("level", [2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24])
 def test_levels_random_cells(jm, O, level, variant):
     w = jm.configs.small_cluster(level, 3, 24, seed=100 + level, cell=8.0, cutoff=5.0, max_nbr=80)
@@ -231,7 +231,7 @@ def test_reverse_neighbor_reuse(jm, O):
     pot.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 def test_fcc32k_level24_full_size(jm, O, variant):
     """The 32,000-atom level-24 workload at full size against the oracle."""
     w = jm.configs.fcc32k_level24(seed=0)

@@ -272,10 +272,10 @@ def test_gpu_head_network_and_gradient(jm, O, residual, level):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("exact_level,desc_variant", [(16, 3), (24, 3), (24, 1), (24, 2)])
+@pytest.mark.parametrize("exact_level,desc_variant", [(16, 3), (24, 3), (24, 1), (24, 2), (24, 6)])
 def test_gpu_head_compute_with_fallback(jm, O, exact_level, desc_variant):
-    """Descriptor variants 64 and 256 threads reuse the descriptor pass's
-    moments in the adjoint pass; 512 threads recompute them."""
+    """Descriptor variants of 64, 128 and 256 threads reuse the descriptor
+    pass's moments in the adjoint pass; 512 threads recompute them."""
     import head_oracle as HO
 
     w, nb = _level12_case(jm, seed=7, natoms=48, cell=8.6)

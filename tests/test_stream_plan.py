@@ -31,7 +31,8 @@ def checker(tmp_path_factory):
 
 
 @pytest.mark.parametrize("level", [2, 6, 10, 12, 16, 20, 24])
-@pytest.mark.parametrize("threads,layout", [(64, "one"), (256, "one"), (512, "one"), (512, "pair"), (1024, "pair")])
+@pytest.mark.parametrize("threads,layout", [(64, "one"), (128, "one"), (256, "one"), (512, "one"), (512, "pair"),
+                                            (1024, "pair")])
 def test_streams_reproduce_the_contraction(checker, level, threads, layout):
     """One row per workgroup (8-byte slots, byte offsets) and two interleaved rows (16-byte slots, slot indices)."""
     for max_nbr in (32, 80):

@@ -15,6 +15,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// The value of v on lane l (a wave-uniform constant), through two readlanes.
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ int wrap_species(int t, int Sp)
 {
     if (t < 0) t += Sp;

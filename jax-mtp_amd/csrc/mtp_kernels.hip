@@ -667,6 +667,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* vy = lds + L.vy;
     double* vz = lds + L.vz;
     double* rr = lds + L.r;
+    double* red = lds + L.red;      // [NT] the site energy's per-thread terms (end of the adjoint region)
     double* epart = lds + L.epart;  // [NW] per-wave sums of the site energy
     int* ints = reinterpret_cast<int*>(lds + L.ints);
     int* nbr_t = ints;  // [K] compact index -> neighbour species (the radial basis's only input from the slot)
@@ -841,9 +842,21 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             }
             if (A.mom_out)
                 for (int x = tid; x < P.n_live; x += NT) A.mom_out[(size_t)row * P.n_live + x] = m[x];
-            // per-wave sums now, their total at the end of the row (in wave order)
-            acc = wave_sum(acc);
-            if ((tid & (kWave - 1)) == 0) epart[tid / kWave] = acc;
+            // Per-wave sums now (eight lanes add eight values each, then lane
+            // order), their total at the end of the row (in wave order).  The
+            // wave reads only its own red[] entries: a wavefront's LDS
+            // accesses complete in issue order.  (Lane shuffles here would
+            // share their index registers with the virial's and spill them.)
+            red[tid] = acc;
+            __builtin_amdgcn_wave_barrier();
+            const int lane = tid & (kWave - 1);
+            double s8 = 0.0;
+            if (lane < 8)
+                for (int j = 0; j < 8; ++j) s8 += red[(tid - lane) + lane * 8 + j];
+            double sw = 0.0;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) sw += lane_value(s8, l);
+            if (lane == 0) epart[tid / kWave] = sw;
             // the reverse pass overwrites the never-operand moments read above
             __syncthreads();
         }

@@ -431,7 +431,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
     }
 
     // streams
-    struct StepDesc { int item = -1, run = -1, cnt = 0; bool last = false; };
+    struct StepDesc { int item = -1, run = -1, cnt = 0; };
     const int pass = 16, nbank = 256 / lay.unit;
     for (int w = 0; w < nw; ++w) {
         const int first_step = (int)(sp.tbl.size() / 64);
@@ -462,7 +462,6 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                         c -= d.cnt;
                     }
                 }
-                sd[l][st - 1].last = true;
                 const uint32_t dslot = (uint32_t)(it.dest / lay.unit);
                 if (it.dest % lay.unit != 0 || dslot > kStreamDestMask) {
                     err = "a contraction destination does not fit the stream's multiplier word";

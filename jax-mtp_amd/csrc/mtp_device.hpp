@@ -6,15 +6,6 @@
 
 namespace mtp {
 
-// Sum over the 64 lanes of a wavefront (butterfly; every lane gets the same
-// value, in a fixed order).
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 // The value of v on lane l (a wave-uniform constant), through two readlanes.
 __device__ __forceinline__ double lane_value(double v, int l)
 {
@@ -23,6 +14,54 @@ __device__ __forceinline__ double lane_value(double v, int l)
     const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+
+// v of the DPP partner lane (dpp_ctrl CTRL, all lanes active), both halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_partner(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active); every lane gets
+// the same value.  Within each row of 16 lanes, DPP moves exchange the sums
+// of lanes 1 and 2 apart (quad permutes), then of the two quads of each
+// half-row (half-row mirror) and of the two half-rows (row mirror): VALU
+// moves, no LDS round trips.  Partners add the same two values, so every
+// lane of a row holds the same bits; the four row totals are added in row
+// order through readlanes.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_partner<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_partner<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_partner<0x141>(v);  // row_half_mirror
+    v += dpp_partner<0x140>(v);  // row_mirror
+    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
+}
+
+// 1/nu for a gradient block of tensor order nu (1 <= nu <= kMaxNu): the
+// hardware reciprocal refined by two Newton steps, r <- r + r (1 - nu r)
+// (each squares the relative error of the estimate), in five instructions
+// instead of the FP64 division sequence per block and slot.  The result is
+// within an ulp of 1/nu.
+__device__ __forceinline__ double recip_order(int nu)
+{
+    const double x = (double)nu;
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// i / d for the (slot, mu) loops, 0 <= i < 2^17 and 1 <= d <= kMaxR, as a
+// multiply and shift: with magic = floor(2^20 / d) + 1 = 2^20 / d + e
+// (0 < e <= 1), i * magic / 2^20 = i / d + i e / 2^20 and i e / 2^20 < 1/d,
+// so the floor is exact there (checked exhaustively for these d and i).
+// The divisor is a kernel argument, so a plain division would be the
+// generic integer division sequence per item.
+__host__ __device__ __forceinline__ unsigned row_quotient_magic(int d) { return (1u << 20) / (unsigned)d + 1u; }
+__host__ __device__ __forceinline__ int row_quotient(int i, unsigned magic) { return (int)(((unsigned)i * magic) >> 20); }
 
 __device__ __forceinline__ int wrap_species(int t, int Sp)
 {

@@ -250,7 +250,7 @@ __device__ __forceinline__ void atom_row_v1(const DevPlan& P, const AtomArgs& A,
                     double Hx = 0.0, Hy = 0.0, Hz = 0.0;
                     grad_block_dispatch(nu, d, px, py, pz, Hx, Hy, Hz);
                     const double uH = ux * Hx + uy * Hy + uz * Hz;
-                    sdf += fpm * uH * (1.0 / (double)nu);
+                    sdf += fpm * uH * recip_order(nu);
                     sfn += fm * uH;
                     Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
                 }
@@ -465,7 +465,7 @@ __device__ __forceinline__ void slot_gradient(const DevPlan& P, int prt, int k, 
             double Hx = 0.0, Hy = 0.0, Hz = 0.0;
             grad_block_dispatch_xy<PN, DS>(nu, d, px, py, uz, Hx, Hy, Hz);
             const double uH = ux * Hx + uy * Hy + uz * Hz;
-            sdf += fpm * uH * (1.0 / (double)nu);
+            sdf += fpm * uH * recip_order(nu);
             sfn += fm * uH;
             Gx += fm * Hx; Gy += fm * Hy; Gz += fm * Hz;
         }
@@ -788,9 +788,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         double* pw = dm;
         // radial values f[k * Rp + mu]: one thread per (slot, mu); zero for
         // the padding (mu >= R, the pair's zero slot)
+        const unsigned rq = row_quotient_magic(Rp);
         if (P.coef_lds) {
             for (int i = tid; i < nvp * Rp; i += NT) {
-                const int c = i / Rp, mu = i - c * Rp;
+                const int c = row_quotient(i, rq), mu = i - c * Rp;
                 double fv = 0.0, dfv;
                 if (c < nv && mu < P.R)
                     radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N,
@@ -799,7 +800,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             }
         } else {
             for (int i = tid; i < nvp * Rp; i += NT) {
-                const int c = i / Rp, mu = i - c * Rp;
+                const int c = row_quotient(i, rq), mu = i - c * Rp;
                 double fv = 0.0, dfv;
                 if (c < nv && mu < P.R) radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
                 f[i] = fv;
@@ -902,9 +903,10 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
     double* gpart = lds + L.gp;
     double* gf = lds + L.gf;
     double* gfp = lds + L.gfp;
+    const unsigned rq = row_quotient_magic(P.R);
     if (P.coef_lds) {
         for (int i = tid; i < nv * P.R; i += NT) {
-            const int c = i / P.R, mu = i - c * P.R;
+            const int c = row_quotient(i, rq), mu = i - c * P.R;
             double fv, dfv;
             radial_one_c<RadialReal>(P, rr[c], coef + ((size_t)wrap_species(nbr_t[c], P.Sp) * P.R + mu) * P.N, fv, dfv);
             gf[mu * K + c] = fv;
@@ -912,7 +914,7 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
         }
     } else {
         for (int i = tid; i < nv * P.R; i += NT) {
-            const int c = i / P.R, mu = i - c * P.R;
+            const int c = row_quotient(i, rq), mu = i - c * P.R;
             double fv, dfv;
             radial_one<RadialReal>(P, rr[c], ti, nbr_t[c], mu, fv, dfv);
             gf[mu * K + c] = fv;

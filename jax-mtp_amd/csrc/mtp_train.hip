@@ -274,7 +274,7 @@ __device__ __forceinline__ void dual_slot(const DevPlan& P, int prt, double ux, 
         } else {
             double H[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
             dual_grad_dispatch<PN>(nu, d, px, py, uz, H, T);
-            const double inu = 1.0 / (double)nu;
+            const double inu = recip_order(nu);
             a0 = (ux * H[0] + uy * H[1] + uz * H[2]) * inu;
             a1 = (ux * T[0] + uy * T[1] + uz * T[2]) * inu;
             q = T[0] * e0 + T[1] * e1 + T[2] * e2;

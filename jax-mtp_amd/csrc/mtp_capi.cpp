@@ -308,6 +308,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     // plan buffers
     auto* mono = c->alloc<I4>(p.mono.size(), e);
     auto* mono_dst = c->alloc<I4>(p.mono_dst.size(), e);
+    auto* ctask = c->alloc<I4>(p.ctask.size(), e);
     auto* fwd_tgt = c->alloc<int>(p.fwd_tgt.size(), e);
     auto* fwd_beg = c->alloc<int>(p.fwd_beg.size(), e);
     auto* fwd_pair = c->alloc<uint32_t>(p.fwd_pair.size(), e);
@@ -362,6 +363,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_out, 16 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = upload(mono, p.mono, c->stream);
     if (e == hipSuccess) e = upload(mono_dst, p.mono_dst, c->stream);
+    if (e == hipSuccess) e = upload(ctask, p.ctask, c->stream);
     if (e == hipSuccess) e = upload(fwd_tgt, p.fwd_tgt, c->stream);
     if (e == hipSuccess) e = upload(fwd_beg, p.fwd_beg, c->stream);
     if (e == hipSuccess) e = upload(fwd_pair, p.fwd_pair, c->stream);
@@ -397,6 +399,7 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
     std::memcpy(d.blk_off, p.blk_off, sizeof(d.blk_off));
     std::memcpy(d.mu_max_of_nu, p.mu_max_of_nu, sizeof(d.mu_max_of_nu));
     d.n_mono = (int)p.mono.size(); d.mono = mono; d.mono_dst = mono_dst;
+    d.n_cwaves = p.n_cwaves; d.ctask = ctask;
     d.n_depth = p.n_depth;
     for (int q = 0; q < kMaxDepth + 2; ++q) { d.fwd_layer[q] = p.fwd_layer[q]; d.rev_layer[q] = p.rev_layer[q]; }
     d.fwd_tgt = fwd_tgt; d.fwd_beg = fwd_beg; d.fwd_pair = fwd_pair; d.fwd_mult = fwd_mult;
@@ -420,6 +423,10 @@ int open_common(MtpFile&& file, int max_atoms, int max_nbr, double cutoff, int d
             HostPlan pp;
             const HostPlan* pv = &p;
             DevPlan dv = d;
+            // basic moments by monomial columns in the 256- and 512-thread
+            // kernels; the narrower ones keep one monomial per thread
+            // (measured faster for them at levels 12-20)
+            if (nt < 256) dv.n_cwaves = 0;
             if (variant_parts(v) != p.n_parts) {  // another split of the gradient blocks
                 std::string ee;
                 build_plan(f, variant_parts(v), pp, ee);  // same validated file: cannot fail

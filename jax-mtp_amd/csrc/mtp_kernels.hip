@@ -633,6 +633,109 @@ __device__ __forceinline__ void basic_moments2(const DevPlan& P, int w, int lane
 }
 
 
+// Basic moments by monomial columns (the plan's ctask table): a task holds
+// the monomials (a, b, c0 + i), i < C, each with NMU mu, and runs on the
+// four lanes of a quad; lane s of the quad sums the slot pairs p = s, s + 4,
+// ...  Per slot pair it reads u_x^a, u_y^b, u_z^c0 and u_z (16-byte slot
+// pairs of the power table) and the radial values once, and walks the
+// column by a running product: a column of C monomials costs the LDS reads
+// of one.  The four partial sums are added over the quad by DPP moves
+// (every lane of the quad then holds the same bits) and lane 0 stores them.
+template <int NMU>
+struct ColPair {
+    double2 X, Y, Z, U;
+    double f0[NMU], f1[NMU];
+};
+
+template <int NMU, int C>
+__device__ __forceinline__ void column_load(ColPair<NMU>& v, int k, const double* px, const double* py, const double* pz,
+                                            const double* pu, const double* f, int Rp)
+{
+    v.X = *reinterpret_cast<const double2*>(px + k);
+    v.Y = *reinterpret_cast<const double2*>(py + k);
+    v.Z = *reinterpret_cast<const double2*>(pz + k);
+    if constexpr (C > 1) v.U = *reinterpret_cast<const double2*>(pu + k);
+    if constexpr (NMU == 1) {
+        v.f0[0] = f[(size_t)k * Rp];
+        v.f1[0] = f[(size_t)(k + 1) * Rp];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NMU / 2; ++j) {
+            const double2 a0 = *reinterpret_cast<const double2*>(f + (size_t)k * Rp + 2 * j);
+            const double2 a1 = *reinterpret_cast<const double2*>(f + (size_t)(k + 1) * Rp + 2 * j);
+            v.f0[2 * j] = a0.x; v.f0[2 * j + 1] = a0.y;
+            v.f1[2 * j] = a1.x; v.f1[2 * j + 1] = a1.y;
+        }
+    }
+}
+
+template <int NMU, int C>
+__device__ __forceinline__ void column_add(const ColPair<NMU>& v, double* acc)
+{
+    double q0 = v.X.x * v.Y.x * v.Z.x, q1 = v.X.y * v.Y.y * v.Z.y;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int mu = 0; mu < NMU; ++mu)
+            acc[c * NMU + mu] = fma(v.f1[mu], q1, fma(v.f0[mu], q0, acc[c * NMU + mu]));
+        if (c + 1 < C) { q0 *= v.U.x; q1 *= v.U.y; }
+    }
+}
+
+template <int NMU, int C>
+__device__ __forceinline__ void column_task(const I4* task, int s, int npairs, const double* pw, int Kp2, int np1,
+                                            const double* f, int Rp, double* m)
+{
+    const I4 hd = task[0];
+    const double* px = pw + (size_t)hd.x * Kp2;
+    const double* py = pw + (size_t)(np1 + hd.y) * Kp2;
+    const double* pz = pw + (size_t)(2 * np1 + hd.z) * Kp2;
+    const double* pu = pw + (size_t)(2 * np1 + 1) * Kp2;
+    double acc[NMU * C];
+#pragma unroll
+    for (int i = 0; i < NMU * C; ++i) acc[i] = 0.0;
+    for (int p = s; p < npairs; p += 4) {
+        ColPair<NMU> v0;
+        column_load<NMU, C>(v0, 2 * p, px, py, pz, pu, f, Rp);
+        column_add<NMU, C>(v0, acc);
+    }
+    const I4 d0 = task[1], d1 = task[2];
+    const int dst[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+    for (int i = 0; i < NMU * C; ++i) {
+        double v = acc[i];
+        v += dpp_partner<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+        v += dpp_partner<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+        if (s == 0 && dst[i] >= 0) m[dst[i]] = v;
+    }
+}
+
+// The basic moments of one row by monomial columns (see column_task); the
+// power table and f as for basic_moments2.  Called by all NW * 64 threads
+// of the row; ends with a barrier.
+template <int NW>
+__device__ __forceinline__ void basic_moments_cols(const DevPlan& P, int w, int lane, int nvp, const double* pw,
+                                                   int Kp2, int np1, const double* f, int Rp, double* m)
+{
+    const int npairs = nvp >> 1, s = lane & 3;
+    for (int g = w; g < P.n_cwaves; g += NW) {  // wave-uniform: every lane active below
+        const I4* task = P.ctask + ((size_t)g * kColTasksPerWave + (lane >> 2)) * 3;
+        const int shape = __builtin_amdgcn_readfirstlane(task[0].w);
+        switch (shape) {
+            case 0: column_task<1, 8>(task, s, npairs, pw, Kp2, np1, f, Rp, m); break;
+            case 1: column_task<2, 4>(task, s, npairs, pw, Kp2, np1, f, Rp, m); break;
+            case 2: column_task<4, 2>(task, s, npairs, pw, Kp2, np1, f, Rp, m); break;
+            case 3: column_task<6, 1>(task, s, npairs, pw, Kp2, np1, f, Rp, m); break;
+            default: column_task<8, 1>(task, s, npairs, pw, Kp2, np1, f, Rp, m); break;
+        }
+    }
+    __syncthreads();
+}
+static_assert(kColShapes == 5 && kColNmu[0] == 1 && kColLen[0] == 8 && kColNmu[1] == 2 && kColLen[1] == 4 &&
+                  kColNmu[2] == 4 && kColLen[2] == 2 && kColNmu[3] == 6 && kColLen[3] == 1 && kColNmu[4] == 8 &&
+                  kColLen[4] == 1,
+              "basic_moments_cols dispatches these shapes");
+
 template <int NW>
 __device__ __forceinline__ void basic_moments(const DevPlan& P, int w, int lane, int nv, const double* pw, int Kp,
                                               int np1, const double* f, double* m, int ms)
@@ -815,7 +918,11 @@ __device__ __forceinline__ void atom_row_v2(const DevPlan& P, const AtomArgs& A,
             for (int p = 1; p < np1; ++p) { v *= u; col[(size_t)p * Kp2] = v; }
         }
         __syncthreads();
-        basic_moments2<NW>(P, tid / kWave, tid & (kWave - 1), nvp, pw, Kp2, np1, f, Rp, m);
+        // by monomial columns where the plan has them (the 256- and
+        // 512-thread kernels: level 24 2.29 -> 2.23 ms; the 64- and
+        // 128-thread kernels at levels 12-20 are faster per monomial)
+        if (P.n_cwaves > 0) basic_moments_cols<NW>(P, tid / kWave, tid & (kWave - 1), nvp, pw, Kp2, np1, f, Rp, m);
+        else basic_moments2<NW>(P, tid / kWave, tid & (kWave - 1), nvp, pw, Kp2, np1, f, Rp, m);
         // moments that are never targets stay 0 (m[x >= n_live] lies in the dm
         // region, free once the power table and f are consumed); in every
         // level the reference ships, each non-basic moment is a target

@@ -1,6 +1,8 @@
 // Plan construction from the .mtp alpha tables (see mtp_plan.hpp).
 #include "mtp_plan.hpp"
 
+#include <array>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -126,6 +128,67 @@ bool build_plan(const MtpFile& f_in, int n_parts, HostPlan& p, std::string& err)
         const int nu = p.mono[t].w & 0xff, alpha = (p.mono[t].w >> 8) & 0xffff;
         int* d = &p.mono_dst[2 * t].x;
         for (int mu = 0; mu <= p.mu_max_of_nu[nu]; ++mu) d[mu] = p.blk_off[mu][nu] + alpha;
+    }
+
+    // --- basic moments by monomial columns ---------------------------------
+    // A task covers the monomials (a, b, c0 + i), i < C, of one column of the
+    // monomial table, all mu of each, for the slots of one quarter of the
+    // slot pairs (four lanes per task).  Its mu count and length come from a
+    // small set of shapes (NMU mu per monomial, C monomials; NMU * C <= 8
+    // accumulators), the first monomial's mu count rounded up to a shape; the
+    // column's later monomials have fewer or equal mu (every reference level)
+    // or start a new task.  Tasks are grouped by shape and padded to 16 per
+    // wavefront (dummy tasks store nothing), so a wavefront runs one shape.
+    {
+        std::vector<std::array<I4, 3>> by_shape[kColShapes];
+        const auto mono_q = [](int nu, int a, int bb) {  // canonical index of (a, bb, nu - a - bb) in its block
+            int q = 0;
+            for (int a2 = nu; a2 > a; --a2) q += nu - a2 + 1;
+            return q + (nu - a - bb);
+        };
+        for (int s = 0; s <= p.nu_max; ++s)
+            for (int a = s; a >= 0; --a) {
+                const int bb = s - a;
+                int c0 = 0;
+                while (s + c0 <= p.nu_max) {
+                    const int nmu0 = p.mu_max_of_nu[s + c0] + 1;
+                    if (nmu0 <= 0) { ++c0; continue; }
+                    int sh = 0;
+                    while (kColNmu[sh] < nmu0) ++sh;
+                    std::array<I4, 3> t;
+                    t[0] = I4{a, bb, c0, sh};
+                    int* dst = &t[1].x;
+                    for (int i = 0; i < 8; ++i) dst[i] = -1;
+                    int n = 0;
+                    while (n < kColLen[sh] && s + c0 + n <= p.nu_max) {
+                        const int nu = s + c0 + n, nmu = p.mu_max_of_nu[nu] + 1;
+                        if (nmu > kColNmu[sh] || (nmu <= 0 && n > 0)) break;
+                        for (int mu = 0; mu < nmu; ++mu)
+                            dst[n * kColNmu[sh] + mu] = p.blk_off[mu][nu] + mono_q(nu, a, bb);
+                        ++n;
+                    }
+                    by_shape[sh].push_back(t);
+                    c0 += n;
+                }
+            }
+        p.ctask.clear();
+        for (int sh = 0; sh < kColShapes; ++sh) {
+            auto& v = by_shape[sh];
+            if (v.empty()) continue;
+            while (v.size() % kColTasksPerWave) v.push_back({I4{0, 0, 0, sh}, I4{-1, -1, -1, -1}, I4{-1, -1, -1, -1}});
+            for (auto& t : v) p.ctask.insert(p.ctask.end(), t.begin(), t.end());
+        }
+        p.n_cwaves = (int)(p.ctask.size() / 3 / kColTasksPerWave);
+        // every basic moment has exactly one task destination
+        std::vector<int> seen(B, 0);
+        for (size_t t = 0; t < p.ctask.size(); t += 3)
+            for (const int d : {p.ctask[t + 1].x, p.ctask[t + 1].y, p.ctask[t + 1].z, p.ctask[t + 1].w,
+                                p.ctask[t + 2].x, p.ctask[t + 2].y, p.ctask[t + 2].z, p.ctask[t + 2].w})
+                if (d >= 0) {
+                    if (d >= B || seen[d]++) { err = "monomial-column tasks do not cover the basic moments once"; return false; }
+                }
+        for (int x = 0; x < B; ++x)
+            if (!seen[x]) { err = "monomial-column tasks do not cover the basic moments once"; return false; }
     }
 
     // --- dependency depths ------------------------------------------------

@@ -34,6 +34,12 @@ struct U4 { uint32_t x, y, z, w; };
 constexpr int kWave = 64;     // wavefront width of gfx950
 constexpr int kMaxR = 8;      // radial functions (mu) supported
 constexpr int kMaxNu = 12;    // tensor order supported
+// Shapes of the monomial-column tasks of the basic moments: NMU mu per
+// monomial, C monomials (NMU * C <= 8 accumulators).
+constexpr int kColShapes = 5;
+constexpr int kColNmu[kColShapes] = {1, 2, 4, 6, 8};
+constexpr int kColLen[kColShapes] = {8, 4, 2, 1, 1};
+constexpr int kColTasksPerWave = 16;  // four lanes per task
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
 constexpr int kStreamUnroll = 3;  // steps per prefetch batch of a contraction stream
@@ -68,6 +74,8 @@ struct DevPlan {
     int n_mono;                      // monomials of degree 0..nu_max
     const I4* mono;                // [n_mono] (a, b, c, nu | alpha_in_block << 8 | wave's max mu << 24)
     const I4* mono_dst;            // [n_mono][2]: basic moment index of (mu, monomial), mu = 0..7, or -1
+    int n_cwaves;                    // wavefronts of monomial-column tasks (kColTasksPerWave tasks each)
+    const I4* ctask;               // [n_cwaves * kColTasksPerWave][3]: (a, b, c0, shape), 8 destinations
     // forward layers
     int n_depth;
     int fwd_layer[kMaxDepth + 2];    // target index ranges per depth 1..n_depth (prefix, into fwd_tgt)
@@ -159,6 +167,8 @@ struct HostPlan {
     std::vector<int> depth;  // [amc]
     std::vector<I4> mono;
     std::vector<I4> mono_dst;   // [n_mono * 2]
+    std::vector<I4> ctask;      // [3 per task] monomial-column tasks of the basic moments
+    int n_cwaves = 0;
     std::vector<int> fwd_tgt, fwd_beg;
     std::vector<uint32_t> fwd_pair;
     std::vector<double> fwd_mult;

@@ -337,6 +337,14 @@ __device__ __forceinline__ const double* stream_op_b(uint32_t x, const char* bb,
     return reinterpret_cast<const double*>(ABS ? lb + ((x >> 16) << 3) : bb + (x >> 16));
 }
 
+// Store v at LDS byte offset off.  The atom kernels have no static LDS, so
+// the dynamic array starts at LDS address 0 and off is the address itself
+// (no add of the array's base per store).
+__device__ __forceinline__ void lds_store(uint32_t off, double v)
+{
+    *reinterpret_cast<__attribute__((address_space(3))) double*>(off) = v;
+}
+
 template <bool ABS>
 __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, const char* bb, char* lb, double& acc)
 {
@@ -361,8 +369,9 @@ __device__ __forceinline__ void stream_steps(const uint4* q, const char* ab, con
         t = fma(a2[u], b2[u], t);
         const uint32_t w = q[u].w;
         acc = fma((double)stream_mult(w), t, acc);
-        if (w & kStreamEnd) {
-            *reinterpret_cast<double*>(lb + ((size_t)stream_dest_slot(w) << 3)) = acc;
+        const uint32_t dst = stream_dest(w);
+        if (dst) {
+            lds_store(dst, acc);
             acc = 0.0;
         }
     }
@@ -1132,8 +1141,9 @@ __device__ __forceinline__ void stream_steps_pair(const uint4* q, char* lb, doub
         const double m = (double)stream_mult(w);
         acc.x = fma(m, t.x, acc.x);
         acc.y = fma(m, t.y, acc.y);
-        if (w & kStreamEnd) {
-            *reinterpret_cast<double2*>(lb + ((size_t)stream_dest_slot(w) << 4)) = acc;
+        const uint32_t dst = stream_dest(w);
+        if (dst) {
+            *reinterpret_cast<double2*>(lb + dst) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }
@@ -1677,11 +1687,40 @@ int atom_kernel_blocks_per_cu(int variant, size_t bytes)
     return n;
 }
 
+// The single-row kernels' contraction streams store by absolute LDS offset
+// (lds_store), which holds while their dynamic LDS array starts at address
+// 0: refuse to run one that has static LDS.
+template <typename Kern>
+static hipError_t no_static_lds(Kern kernel)
+{
+    hipFuncAttributes a;
+    const hipError_t e = hipFuncGetAttributes(&a, (const void*)kernel);
+    if (e != hipSuccess) return e;
+    return a.sharedSizeBytes == 0 ? hipSuccess : hipErrorInvalidConfiguration;
+}
+
+template <int NT>
+static hipError_t single_row_kernels_checked()
+{
+    hipError_t e = no_static_lds(atom_kernel_v2<double, NT>);
+    if (e == hipSuccess) e = no_static_lds(atom_kernel_v2<float, NT>);
+    if (e == hipSuccess) e = no_static_lds(atom_kernel_v2_list<NT>);
+    return e;
+}
+
 hipError_t set_atom_kernel_lds(int variant, size_t bytes)
 {
     static size_t cur[kNumVariants] = {};
     if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
     if (bytes <= cur[variant]) return hipSuccess;
+    if (cur[variant] == 0) {
+        hipError_t c = hipSuccess;
+        if (variant == kVariantV2x64) c = single_row_kernels_checked<64>();
+        else if (variant == kVariantV2x128) c = single_row_kernels_checked<128>();
+        else if (variant == kVariantV2x256) c = single_row_kernels_checked<256>();
+        else if (variant == kVariantV2x512) c = single_row_kernels_checked<512>();
+        if (c != hipSuccess) return c;
+    }
     hipError_t e = set_atom_kernel_lds_exact(variant, bytes);
     if (e == hipSuccess) cur[variant] = bytes;
     return e;

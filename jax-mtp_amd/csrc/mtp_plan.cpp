@@ -424,7 +424,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
         }
         if (a > 0xffffu || b > 0xffffu) return false;
         const double mv = mult[e];
-        if (!(mv >= 1.0 && mv <= (double)kStreamMultMask && mv == std::floor(mv))) return false;
+        if (!(mv >= 1.0 && mv <= (double)kStreamMultMax && mv == std::floor(mv))) return false;
         x = a | (b << 16);
         mw = (uint32_t)mv;
         return true;
@@ -525,12 +525,12 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                         c -= d.cnt;
                     }
                 }
-                const uint32_t dslot = (uint32_t)(it.dest / lay.unit);
-                if (it.dest % lay.unit != 0 || dslot > kStreamDestMask) {
+                const uint32_t doff = (uint32_t)it.dest;
+                if (it.dest <= 0 || it.dest % lay.unit != 0 || doff > kStreamDestMask) {
                     err = "a contraction destination does not fit the stream's multiplier word";
                     return false;
                 }
-                at(st - 1, l).w = kStreamEnd | (dslot << kStreamDestShift);
+                at(st - 1, l).w = doff;
             }
         }
         // Each entry slot of a step is read by two LDS instructions
@@ -580,7 +580,7 @@ bool stream_layer(const std::vector<int>& beg, const std::vector<uint32_t>& pair
                         add(bb[(pb / lay.unit) % nbank], pb);
                         U4& q = at(st, l);
                         (j == 0 ? q.x : (j == 1 ? q.y : q.z)) = px;
-                        q.w = (q.w & ~kStreamMultMask) | pm;
+                        q.w = (q.w & kStreamDestMask) | (pm << kStreamMultShift);
                     }
                 }
             }

@@ -43,17 +43,17 @@ constexpr int kColTasksPerWave = 16;  // four lanes per task
 constexpr int kMaxN = 32;     // Chebyshev terms supported
 constexpr int kMaxDepth = 8;  // contraction layers supported
 constexpr int kStreamUnroll = 3;  // steps per prefetch batch of a contraction stream
-// Multiplier word of a contraction-stream step (see DevPlan): bits 0-15
-// the entries' integer multiplier (0 in padding steps), bit 31 set on the
-// last step of the lane's item, and then bits 16-30 the item's destination
-// as a slot index from the LDS base (slots of the layout's unit: 8 bytes
-// for one row per workgroup, 16 for interleaved pairs).
-constexpr uint32_t kStreamEnd = 0x80000000u;
-constexpr uint32_t kStreamMultMask = 0xffffu;
-constexpr uint32_t kStreamDestMask = 0x7fffu;
-constexpr int kStreamDestShift = 16;
-constexpr uint32_t stream_mult(uint32_t w) { return w & kStreamMultMask; }
-constexpr uint32_t stream_dest_slot(uint32_t w) { return (w >> kStreamDestShift) & kStreamDestMask; }
+// Multiplier word of a contraction-stream step (see DevPlan): bits 0-17
+// the item's destination as an LDS byte offset from the LDS base on the last
+// step of the lane's item, else 0 (the zero slot at offset 0 is never a
+// destination, so a nonzero field is the item's end); bits 18-31 the
+// entries' integer multiplier (0 in padding steps).  The kernel takes the
+// store address with one AND and the multiplier with one shift.
+constexpr uint32_t kStreamDestMask = 0x3ffffu;
+constexpr int kStreamMultShift = 18;
+constexpr uint32_t kStreamMultMax = 0x3fffu;
+constexpr uint32_t stream_mult(uint32_t w) { return w >> kStreamMultShift; }
+constexpr uint32_t stream_dest(uint32_t w) { return w & kStreamDestMask; }
 
 constexpr int kLdsM = 2;      // m[] starts at this double of the atom kernels' LDS (lds[0] is a zero)
 
@@ -103,8 +103,8 @@ struct DevPlan {
     // region in the reverse pass; 0 for an unused entry: the zero slot) and
     // w the multiplier word (stream_mult), so that the lane's sum gains
     // mult (a0 b0 + a1 b1 + a2 b2).  Each lane walks its own sequence of
-    // items: on the last step of an item (kStreamEnd in w, per lane) the
-    // lane stores its sum at stream_dest_slot(w) and restarts from zero.
+    // items: on the last step of an item (stream_dest(w) nonzero, per lane)
+    // the lane stores its sum at that LDS byte offset and restarts from zero.
     // Rows split into chunks leave partial sums, combined per layer by
     // comb[cb_beg[L] .. cb_beg[L+1]] = (moment, first partial, end partial,
     // -) in a fixed order.
@@ -191,7 +191,7 @@ struct HostPlan {
 // entry).  Offsets are in slots from the LDS base.  With `absolute`, the
 // operand fields hold slot indices from the LDS base; otherwise they hold
 // byte offsets (a and the forward b from the LDS base, the reverse b from
-// dm_off).  Item destinations are slot indices from the LDS base.
+// dm_off).  Item destinations are byte offsets from the LDS base.
 struct StreamLayout {
     int unit = 8;
     int m_off = kLdsM;
@@ -202,8 +202,8 @@ struct StreamLayout {
 
 // Streams for a workgroup of nt threads.  part_budget caps the partial sums
 // of a layer when the chunking allows.  Returns false when an operand
-// offset does not fit the 16-bit fields, a destination the 15-bit field,
-// or a multiplier is not an integer in [1, 65535].
+// offset does not fit the 16-bit fields, a destination the 18-bit field,
+// or a multiplier is not an integer in [1, kStreamMultMax].
 bool build_items(const HostPlan& p, int nt, const StreamLayout& lay, int part_budget, ItemPlan& ip,
                  std::string& err);
 

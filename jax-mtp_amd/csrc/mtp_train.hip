@@ -76,8 +76,9 @@ __device__ __forceinline__ void dual_steps(const uint4* q, char* lb, double2& ac
         const double m = (double)stream_mult(w);
         acc.x = fma(m, t.x, acc.x);
         acc.y = fma(m, t.y, acc.y);
-        if (w & kStreamEnd) {
-            *reinterpret_cast<double2*>(lb + ((size_t)stream_dest_slot(w) << 4)) = acc;
+        const uint32_t dst = stream_dest(w);
+        if (dst) {
+            *reinterpret_cast<double2*>(lb + dst) = acc;
             acc = make_double2(0.0, 0.0);
         }
     }

@@ -126,11 +126,10 @@ int main(int argc, char** argv)
             for (int s = wv.x; s < wv.x + wv.y; ++s) {
                 const U4* q = &sp.tbl[(size_t)s * 64];
                 for (int l = 0; l < 64; ++l) {
-                    const bool end = (q[l].w & kStreamEnd) != 0;  // per lane
+                    const bool end = stream_dest(q[l].w) != 0;  // per lane
                     const uint32_t e[3] = {q[l].x, q[l].y, q[l].z};
                     const uint32_t mult = stream_mult(q[l].w);
                     if (mult == 0 && (e[0] | e[1] | e[2]) != 0) return fail("entries in a step without a multiplier");
-                    if (!end && (q[l].w & ~kStreamMultMask) != 0) return fail("destination bits in a step that does not end an item");
                     // the kernel issues both operand reads of all three entries
                     size_t sa[3], sb[3];
                     for (int j = 0; j < 3; ++j) sa[j] = slot(e[j] & 0xffffu, false), sb[j] = slot(e[j] >> 16, rev);
@@ -141,7 +140,7 @@ int main(int argc, char** argv)
                         acc[l * W + r] = std::fma((double)mult, t, acc[l * W + r]);
                     }
                     if (end) {
-                        const size_t d = dest_slot(stream_dest_slot(q[l].w) * (uint32_t)lay.unit);
+                        const size_t d = dest_slot(stream_dest(q[l].w));
                         if (d == 0) return fail("an item stores into the zero slot");
                         for (int r = 0; r < W; ++r) {
                             lds[d * W + r] = acc[l * W + r];

@@ -72,6 +72,46 @@ def test_levels_random_cells(jm, O, level, variant):
     _check(_run_gpu(jm, w, nb, variant=variant), _run_oracle(O, w, nb))
 
 
+def _wide_mu_params(jm, seed):
+    """A potential whose basic moments carry eight radial functions at degree 0 and seven at degree 1
+    (every shipped level has at most six): the widest monomial-column shape of the 256- and
+    512-thread kernels (eight mu per monomial).  Scalars: the eight degree-0 moments and four dot
+    products of degree-1 moments, one of them a square."""
+    rng = np.random.default_rng(seed)
+    p = jm.configs.random_params(2, 2, 8, 2.0, 5.0, rng)
+    R = 8
+    basic = [[mu, 0, 0, 0] for mu in range(8)]
+    for mu in range(7):
+        basic += [[mu, 1, 0, 0], [mu, 0, 1, 0], [mu, 0, 0, 1]]
+    vec = lambda mu: 8 + 3 * mu  # first component of M_{mu,1}
+    pairs = [(0, 1), (2, 3), (4, 6), (5, 5)]
+    times = []
+    for q, (a, b) in enumerate(pairs):
+        t = len(basic) + q
+        for c in range(3):
+            times.append([vec(a) + c, vec(b) + c, 1, t])
+    p.radial_funcs_count = R
+    p.alpha_index_basic = np.array(basic, dtype=np.int64)
+    p.alpha_index_basic_count = len(basic)
+    p.alpha_index_times = np.array(times, dtype=np.int64)
+    p.alpha_index_times_count = len(times)
+    p.alpha_moments_count = len(basic) + len(pairs)
+    p.alpha_moment_mapping = np.array(list(range(8)) + [len(basic) + q for q in range(len(pairs))], dtype=np.int64)
+    p.alpha_scalar_moments = len(p.alpha_moment_mapping)
+    p.moment_coeffs = rng.normal(0.0, 0.1, p.alpha_scalar_moments)
+    p.radial_coeffs = rng.normal(0.0, 0.1, (2, 2, R, 8))
+    return p
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+def test_eight_radial_functions(jm, O, variant):
+    w = jm.configs.small_cluster(2, 2, 24, seed=71, cell=8.0, cutoff=5.0, max_nbr=80)
+    w.params = _wide_mu_params(jm, 5)
+    nb = w.neighbors()
+    print(f"[case] eight radial functions variant={variant} natoms={w.natoms} maxn={nb['counts'].max()}", flush=True)
+    _check(_run_gpu(jm, w, nb, variant=variant), _run_oracle(O, w, nb))
+
+
 def test_fcc256_level10(jm, O):
     w = jm.configs.fcc256_level10(seed=0)
     nb = w.neighbors()

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(jm):
     assert declared == set(jm.capi.SIGNATURES), declared ^ set(jm.capi.SIGNATURES)
 
 
-@pytest.mark.parametrize("level", [2, 6, 10, 16, 24])
+@pytest.mark.parametrize("level", [2, 6, 10, 16, 24, 28])
 def test_mtp_readers_agree(jm, O, tmp_path, level):
     p = jm.configs.random_params(level, 2, 8, 1.9, 5.0, np.random.default_rng(level))
     path = tmp_path / "p.mtp"

@@ -195,12 +195,13 @@ int main(int argc, char** argv)
     int steps = 0;
     for (const I4& w : ip.fwd.wave) steps += w.y;
     for (const I4& w : ip.rev.wave) steps += w.y;
-    // LDS passes per operand read (gfx950: 16 lanes per pass, slots that
-    // coincide modulo 16 collide); a pass costs the largest number of
-    // distinct slots that collide
+    // LDS passes per operand read (gfx950: 16 lanes per pass from 64 banks
+    // of 4 bytes, so slots whose addresses coincide modulo 256 bytes
+    // collide: 32 slots of 8 bytes, 16 of 16 bytes); a pass costs the
+    // largest number of distinct slots that collide
     double passes = 0.0, ideal = 0.0;
     {
-        const int pass = 16, nbank = 16;
+        const int pass = 16, nbank = 256 / lay.unit;
         for (int dir = 0; dir < 2; ++dir) {
             const StreamPlan& sp = dir ? ip.rev : ip.fwd;
             for (size_t st = 0; st < sp.tbl.size() / 64; ++st) {

@@ -414,6 +414,28 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
     const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
     const int nsteps = wv.y;
+    // This thread's first row of the layer's combination and (reverse) its
+    // first base-only and xi-added rows: their table entries and xi values
+    // are global loads, issued here so that the stream walk hides their
+    // latency (after the walk's barrier they would be two dependent global
+    // round trips on every layer's critical path).
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    const int cA = c0 + tid;
+    I4 cbA = {0, 0, 0, 0};
+    double xA = 0.0;
+    if (cA < c1) {
+        cbA = comb[cA];
+        if (REV) xA = xi[cbA.x];
+    }
+    int boX = -1, xaX = -1;
+    double boV = 0.0, xaV = 0.0;
+    if (REV) {
+        const int tb = P.rev_bo_beg[L] + tid, ta = P.rev_xa_beg[L] + tid;
+        if (tb < P.rev_bo_beg[L + 1]) { boX = P.rev_bo[tb]; boV = xi[boX]; }
+        if (ta < P.rev_xa_beg[L + 1]) { xaX = P.rev_xa[ta]; xaV = xi[xaX]; }
+    }
     if (nsteps > 0) {
         const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
         char* lb = reinterpret_cast<char*>(lds);
@@ -422,22 +444,26 @@ __device__ __forceinline__ void contraction_layer(const DevPlan& P, int L, doubl
         else stream_walk<false>(tb, nsteps, bb, lb);
     }
     __syncthreads();
-    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
-    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
-    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
     double* dst = REV ? dm : lds + kLdsM;
-    for (int c = c0 + tid; c < c1; c += NT) {
+    if (cA < c1) {
+        double acc = xA;
+        for (int p = cbA.y; p < cbA.z; ++p) acc += part[p];
+        dst[cbA.x] = acc;
+    }
+    for (int c = cA + NT; c < c1; c += NT) {
         const I4 cb = comb[c];
         double acc = REV ? xi[cb.x] : 0.0;
         for (int p = cb.y; p < cb.z; ++p) acc += part[p];
         dst[cb.x] = acc;
     }
     if (REV) {
-        for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
+        if (boX >= 0) dm[boX] = boV;
+        for (int t = P.rev_bo_beg[L] + tid + NT; t < P.rev_bo_beg[L + 1]; t += NT) {
             const int x = P.rev_bo[t];
             dm[x] = xi[x];
         }
-        for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
+        if (xaX >= 0) dm[xaX] += xaV;
+        for (int t = P.rev_xa_beg[L] + tid + NT; t < P.rev_xa_beg[L + 1]; t += NT) {
             const int x = P.rev_xa[t];
             dm[x] += xi[x];
         }

@@ -96,6 +96,26 @@ __device__ __forceinline__ void dual_layer(const DevPlan& P, int L, double* lds,
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
     const I4 wv = (REV ? P.rev_wave : P.fwd_wave)[L * NW + w];
     const int nsteps = wv.y;
+    // this thread's first combination, base-only and xi-added rows: their
+    // table entries and xi values (dependent global loads) are issued before
+    // the stream walk, which hides their latency
+    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
+    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
+    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
+    const int cA = c0 + tid;
+    I4 cbA = {0, 0, 0, 0};
+    double xA = 0.0;
+    if (cA < c1) {
+        cbA = comb[cA];
+        if (REV) xA = P.xi_of[cbA.x];
+    }
+    int boX = -1, xaX = -1;
+    double boV = 0.0, xaV = 0.0;
+    if (REV) {
+        const int tb0 = P.rev_bo_beg[L] + tid, ta0 = P.rev_xa_beg[L] + tid;
+        if (tb0 < P.rev_bo_beg[L + 1]) { boX = P.rev_bo[tb0]; boV = P.xi_of[boX]; }
+        if (ta0 < P.rev_xa_beg[L + 1]) { xaX = P.rev_xa[ta0]; xaV = P.xi_of[xaX]; }
+    }
     if (nsteps > 0) {
         const uint4* tb = reinterpret_cast<const uint4*>(REV ? P.rev_tbl : P.fwd_tbl) + (size_t)wv.x * kWave + lane;
         char* lb = reinterpret_cast<char*>(lds);
@@ -121,13 +141,10 @@ __device__ __forceinline__ void dual_layer(const DevPlan& P, int L, double* lds,
         }
     }
     __syncthreads();
-    const int c0 = REV ? P.rev_cb_beg[L] : P.fwd_cb_beg[L];
-    const int c1 = REV ? P.rev_cb_beg[L + 1] : P.fwd_cb_beg[L + 1];
-    const I4* comb = REV ? P.rev_comb : P.fwd_comb;
     double2* dst = REV ? dm2 : m2;
-    for (int c = c0 + tid; c < c1; c += NT) {
-        const I4 cb = comb[c];
-        const double x0 = REV ? P.xi_of[cb.x] : 0.0;
+    for (int c = cA; c < c1; c += NT) {
+        const I4 cb = c == cA ? cbA : comb[c];
+        const double x0 = REV ? (c == cA ? xA : P.xi_of[cb.x]) : 0.0;
         double2 acc = make_double2(alpha * x0, x0);
         for (int p = cb.y; p < cb.z; ++p) {
             const double2 v = part[p];
@@ -138,13 +155,15 @@ __device__ __forceinline__ void dual_layer(const DevPlan& P, int L, double* lds,
     }
     if (REV) {
         for (int t = P.rev_bo_beg[L] + tid; t < P.rev_bo_beg[L + 1]; t += NT) {
-            const int x = P.rev_bo[t];
-            const double xi = P.xi_of[x];
+            const bool first = t == P.rev_bo_beg[L] + tid;
+            const int x = first ? boX : P.rev_bo[t];
+            const double xi = first ? boV : P.xi_of[x];
             dm2[x] = make_double2(alpha * xi, xi);
         }
         for (int t = P.rev_xa_beg[L] + tid; t < P.rev_xa_beg[L + 1]; t += NT) {
-            const int x = P.rev_xa[t];
-            const double xi = P.xi_of[x];
+            const bool first = t == P.rev_xa_beg[L] + tid;
+            const int x = first ? xaX : P.rev_xa[t];
+            const double xi = first ? xaV : P.xi_of[x];
             double2 v = dm2[x];
             v.x += alpha * xi;
             v.y += xi;

@@ -380,8 +380,11 @@ def test_widest_rows(jm, O, variant):
     _check(_run_gpu(jm, w, nb, variant=variant), _run_oracle(O, w, nb))
 
 
-def test_empty_and_single_slot(jm, O):
-    """No rows (energy = 0, nothing launched but the totals) and rows with a single neighbor."""
+@pytest.mark.parametrize("variant", [3, 1, 2])
+def test_empty_and_single_slot(jm, O, variant):
+    """No rows (energy = 0, nothing launched but the totals), rows with a single neighbor and rows with
+    none, on the 64-thread kernel (one monomial per thread) and the 256- and 512-thread kernels
+    (monomial columns: a quad with one slot pair or none)."""
     import torch
 
     w = jm.configs.small_cluster(10, 1, 6, seed=5, cell=6.0, cutoff=5.0, max_nbr=40)
@@ -399,11 +402,13 @@ def test_empty_and_single_slot(jm, O):
     keep = np.argmax(nb["jtypes"] >= 0, axis=1)
     A = nb["js"].shape[0]
     cols = np.stack([keep, np.full(A, 39)], axis=1)
+    cols[::3, 0] = 39  # every third row has no neighbor at all
     rows = np.arange(A)[:, None]
     one = dict(nb, js=nb["js"][rows, cols].copy(), jtypes=nb["jtypes"][rows, cols].copy(),
                rij=nb["rij"][rows, cols].copy())
     w1 = jm.configs.Workload(w.name, w.params, w.pos, w.types, w.cell, w.cutoff, 2)
-    _check(_run_gpu(jm, w1, one), _run_oracle(O, w1, one))
+    print(f"[case] single and empty slots, variant {variant}", flush=True)
+    _check(_run_gpu(jm, w1, one, variant=variant), _run_oracle(O, w1, one))
 
 
 def test_capacity_errors(jm):
